@@ -1,0 +1,207 @@
+/*
+ * svox2_amd.h - C ABI of the MI355X (gfx950) sparse-voxel renderer.
+ *
+ * Every entry point takes plain device pointers and sizes, launches HIP
+ * kernels on the caller's stream (a hipStream_t passed as void*; NULL is the
+ * null stream), returns 0 on success and a nonzero status otherwise, and never
+ * synchronises the host.  svox2_last_error() gives the message of the last
+ * failure on the calling thread.  No entry point allocates device memory: any
+ * scratch space is passed in by the caller (see *_workspace_bytes).
+ *
+ * Each function stands in for one binding of the reference's pybind11 module
+ * svox2.csrc (svox2/csrc/svox2.cpp:11-72); the exact line is cited beside it.
+ * Tensors that the reference takes as torch::Tensor become pointers here:
+ *   links        int32  [size0][size1][size2], row-major, values >= 0 are rows,
+ *                        < 0 are empty cells (-1 - k after accel_dist_prop)
+ *   density_data rows x 1          (storage = F32 or BF16)
+ *   sh_data      rows x sh_data_dim (storage = F32 or BF16), channel layout
+ *                        [R0..R(B-1), G0..G(B-1), B0..B(B-1)], B = basis_dim
+ *   gradients    always float32, same row count as the data
+ *   rays         origins/dirs float32 [n][3]
+ */
+#ifndef SVOX2_AMD_H_
+#define SVOX2_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVOX2_OK 0
+#define SVOX2_ERR_INVALID_ARGUMENT 1
+#define SVOX2_ERR_UNSUPPORTED 2
+#define SVOX2_ERR_HIP 3
+
+#define SVOX2_STORAGE_F32 0
+#define SVOX2_STORAGE_BF16 1
+
+#define SVOX2_BASIS_TYPE_SH 1
+
+/* rmsprop_step indexer kinds (svox2/csrc/optim_kernel.cu:175-215) */
+#define SVOX2_INDEXER_ALL 0   /* every row (0-dim indexer in the reference) */
+#define SVOX2_INDEXER_MASK 1  /* uint8/bool mask, one byte per row          */
+#define SVOX2_INDEXER_INDEX 2 /* int64 row indices                          */
+
+/* Grid descriptor; replaces SparseGridSpec (svox2/csrc/include/data_spec.hpp:18-52)
+ * and its packed form (data_spec_packed.cuh:11-62).  World -> grid coordinates
+ * are p_grid = p_world * scaling + offset, as SparseGrid._to_cpp computes them
+ * (svox2/svox2.py:2135-2161). */
+typedef struct svox2_grid {
+    const int32_t* links;
+    const void* density_data;
+    const void* sh_data;
+    int64_t rows;
+    int32_t size[3];
+    int32_t basis_dim;   /* 1, 4 or 9 */
+    int32_t sh_data_dim; /* 3 * basis_dim */
+    int32_t basis_type;  /* SVOX2_BASIS_TYPE_SH */
+    int32_t storage;     /* SVOX2_STORAGE_* for density_data and sh_data */
+    float offset[3];
+    float scaling[3];
+} svox2_grid_t;
+
+/* RenderOptions (svox2/csrc/include/data_spec.hpp:115-135). */
+typedef struct svox2_render_options {
+    float background_brightness;
+    float step_size;
+    float sigma_thresh;
+    float stop_thresh;
+    float near_clip;
+    int32_t use_spheric_clip;
+    int32_t last_sample_opaque;
+} svox2_render_options_t;
+
+/* CameraSpec (data_spec.hpp:84-102): c2w is the 3x4 OpenCV camera-to-world
+ * matrix, row-major. */
+typedef struct svox2_camera {
+    float c2w[12];
+    float fx, fy, cx, cy;
+    int32_t width, height;
+    float ndc_coeffx, ndc_coeffy;
+} svox2_camera_t;
+
+/* GridOutputGrads (data_spec.hpp:54-82).  Any pointer may be NULL. */
+typedef struct svox2_grads {
+    float* grad_density_out; /* rows x 1  */
+    float* grad_sh_out;      /* rows x sh_data_dim */
+    uint8_t* mask_out;       /* rows, set to 1 for every row a sample touched */
+} svox2_grads_t;
+
+/* Forward render of n_rays rays; rgb_out [n][3].  ray_clip, if not NULL, is
+ * [n][2] (t_near, t_far) in world distance along the ray direction vector and
+ * narrows each ray's [tmin, tmax].  log_transmit_out, if not NULL, receives the
+ * final log transmittance per ray.
+ * Replaces volume_render_cuvol (svox2.cpp:28; render_lerp_kernel_cuvol.cu:918). */
+int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
+                              const float* dirs, const float* ray_clip,
+                              int64_t n_rays, const svox2_render_options_t* opt,
+                              float* rgb_out, float* log_transmit_out,
+                              void* stream);
+
+/* Forward render of a pinhole image; rgb_out [height][width][3].
+ * Replaces volume_render_cuvol_image (svox2.cpp:31; render_lerp_kernel_cuvol.cu:960). */
+int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
+                                    const svox2_camera_t* cam,
+                                    const svox2_render_options_t* opt,
+                                    float* rgb_out, float* log_transmit_out,
+                                    void* stream);
+
+/* Backward of svox2_volume_render_cuvol given dL/drgb (grad_out [n][3]) and
+ * the forward colours (color_cache [n][3]); accumulates into grads.
+ * Replaces volume_render_cuvol_backward (svox2.cpp:33; render_lerp_kernel_cuvol.cu:1010). */
+int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
+                                       const float* origins, const float* dirs,
+                                       const float* ray_clip, int64_t n_rays,
+                                       const svox2_render_options_t* opt,
+                                       const float* grad_out,
+                                       const float* color_cache,
+                                       const svox2_grads_t* grads,
+                                       void* stream);
+
+/* Forward render plus the gradient of mean((rgb - rgb_gt)^2) (over 3n terms),
+ * with optional beta and Cauchy sparsity terms, in one launch; writes rgb_out
+ * and accumulates into grads.
+ * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
+int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
+                                    const float* origins, const float* dirs,
+                                    const float* ray_clip, int64_t n_rays,
+                                    const svox2_render_options_t* opt,
+                                    const float* rgb_gt, float beta_loss,
+                                    float sparsity_loss, float* rgb_out,
+                                    const svox2_grads_t* grads, void* stream);
+
+/* Per-ray march counters [n][4] (int32): loop iterations, iterations not
+ * skipped by the empty-space skip, samples with sigma > sigma_thresh, and a
+ * FNV-1a hash of every visited cell, sample t and contribution flag plus the
+ * final log transmittance.  No counterpart in the reference: it measures the
+ * work of svox2_volume_render_cuvol on the same inputs and pins its sample
+ * indexing bit for bit. */
+int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid,
+                                    const float* origins, const float* dirs,
+                                    const float* ray_clip, int64_t n_rays,
+                                    const svox2_render_options_t* opt,
+                                    int32_t* stats_out, void* stream);
+
+/* Trilinear samples of density [n][1] and, if want_colors, sh [n][sh_data_dim]
+ * at points (world coordinates through grid->offset/scaling).
+ * Replaces sample_grid (svox2.cpp:57; svox2_kernel.cu:181). */
+int svox2_sample_grid(const svox2_grid_t* grid, const float* points,
+                      int64_t n_points, int32_t want_colors,
+                      float* density_out, float* sh_out, void* stream);
+
+/* Adjoint of svox2_sample_grid; accumulates into grad_density_out / grad_sh_out.
+ * Replaces sample_grid_backward (svox2.cpp:59; svox2_kernel.cu:219). */
+int svox2_sample_grid_backward(const svox2_grid_t* grid, const float* points,
+                               int64_t n_points,
+                               const float* grad_out_density,
+                               const float* grad_out_sh,
+                               float* grad_density_out, float* grad_sh_out,
+                               int32_t want_colors, void* stream);
+
+/* Sparse total-variation gradient over n_cells flat cell indices (rand_cells),
+ * channels [start_dim, end_dim) of data (rows x data_dim), scaled by
+ * scale / n_cells and accumulated into grad_data; mask_out (NULL allowed) is
+ * set for every row that receives a gradient.
+ * Replaces tv_grad_sparse (svox2.cpp:45; loss_kernel.cu:558). */
+int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
+                         const void* data, int32_t storage, int64_t rows,
+                         int32_t data_dim, const int32_t* rand_cells,
+                         int64_t n_cells, uint8_t* mask_out, int32_t start_dim,
+                         int32_t end_dim, float scale, int32_t use_logalpha,
+                         float logalpha_delta, int32_t ignore_edge,
+                         int32_t ignore_last_z, float ndc_coeffx,
+                         float ndc_coeffy, float* grad_data, void* stream);
+
+/* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
+ * rms and grad; grad is zeroed where updated.  indexer_kind selects every row,
+ * a byte mask (n_index = rows) or int64 row indices (n_index entries).
+ * Replaces rmsprop_step (svox2.cpp:66; optim_kernel.cu:154). */
+int svox2_rmsprop_step(void* data, int32_t storage, float* rms, float* grad,
+                       int64_t rows, int32_t dim, int32_t indexer_kind,
+                       const void* indexer, int64_t n_index, float beta,
+                       float lr, float epsilon, float minval, float lr_last,
+                       void* stream);
+
+/* Bytes of scratch svox2_accel_dist_prop needs for a links table of this size. */
+size_t svox2_accel_dist_prop_workspace_bytes(const int32_t size[3]);
+
+/* Rewrites every empty link in place to -1 - k, k = number of consecutive
+ * empty ancestor levels of the occupancy pyramid.
+ * Replaces accel_dist_prop (svox2.cpp:71; misc_kernel.cu:356). */
+int svox2_accel_dist_prop(int32_t* links, const int32_t size[3],
+                          void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* svox2_last_error(void);
+
+/* Library version string. */
+const char* svox2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVOX2_AMD_H_ */

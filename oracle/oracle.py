@@ -1,0 +1,655 @@
+"""TEST INFRASTRUCTURE ONLY - the oracle.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+may import this module, and only as the checker; the product (svox2_amd/)
+never does.
+
+CPU restatements of the reference algorithm for the hot path, each citing the
+reference lines it follows:
+
+* ``MarchRef`` - ctypes wrapper of march_ref.c, the FP32 restatement of the
+  kernel's index arithmetic (sample positions, skips, density, stop test) and
+  per-ray counters; bit-exact with the GPU kernel by construction.
+* ``render_fp64`` - the reference's compositing (svox2/svox2.py:658-884 plus the
+  kernel's thresholds, render_lerp_kernel_cuvol.cu:30-120) evaluated in FP64
+  with PyTorch autograd over the exact FP32 sample list of ``MarchRef``: the
+  oracle for colours and gradients.
+* ``volume_render_gradcheck_lerp`` - a line-by-line, dtype-generic restatement
+  of SparseGrid._volume_render_gradcheck_lerp (svox2/svox2.py:658-884) with the
+  grid passed explicitly; the CPU baseline and a cross-check of render_fp64.
+* ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
+  ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
+  (loss_kernel.cu:178-246,558-604), ``rmsprop_step`` (optim_kernel.cu:15-93),
+  ``accel_dist_prop`` (misc_kernel.cu:109-181,356-416).
+
+Parity pins (tests/test_oracle_*.py): closed-form compositing of uniform
+fields, background-only cases, F.grid_sample, exactness on linear fields, the
+SH closed form and orthonormality, Morton bijectivity, FP64 finite
+differences, autograd of a TV value, brute-force integer pyramids, and
+agreement of the two march restatements with each other.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(HERE, "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "libmarch_ref.so")
+
+SH_C0 = 0.28209479177387814
+SH_C1 = 0.4886025119029199
+SH_C2 = (1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+         -1.0925484305920792, 0.5462742152960396)
+
+
+def build(force: bool = True) -> str:
+    """Compiles march_ref.c with gcc (FP32, no contraction) into _build/."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    if not force and os.path.exists(LIB_PATH):
+        return LIB_PATH
+    src = os.path.join(HERE, "march_ref.c")
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-ffp-contract=off",
+           "-fno-fast-math", "-o", tmp, src, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+# ---------------------------------------------------------------------------
+# C restatement (march_ref.c)
+# ---------------------------------------------------------------------------
+class _RefGrid(ctypes.Structure):
+    _fields_ = [
+        ("links", ctypes.c_void_p), ("density", ctypes.c_void_p), ("sh", ctypes.c_void_p),
+        ("rows", ctypes.c_int64), ("size", ctypes.c_int32 * 3), ("basis_dim", ctypes.c_int32),
+        ("offset", ctypes.c_float * 3), ("scaling", ctypes.c_float * 3),
+    ]
+
+
+class _RefOpt(ctypes.Structure):
+    _fields_ = [
+        ("bg", ctypes.c_float), ("step", ctypes.c_float), ("sigma_thresh", ctypes.c_float),
+        ("stop_thresh", ctypes.c_float), ("near_clip", ctypes.c_float),
+        ("spheric", ctypes.c_int32), ("last_opaque", ctypes.c_int32), ("skip", ctypes.c_int32),
+    ]
+
+
+@dataclass
+class Opts:
+    """Render options with the reference's meaning (data_spec.hpp:115-135)."""
+    background_brightness: float = 1.0
+    step_size: float = 0.5
+    sigma_thresh: float = 1e-10
+    stop_thresh: float = 1e-7
+    near_clip: float = 0.0
+    use_spheric_clip: bool = False
+    last_sample_opaque: bool = False
+
+
+@dataclass
+class Grid:
+    """Host copy of a grid: links (X,Y,Z) int32, density (R,) and sh (R,C) as
+    float64 numpy arrays holding exactly the values the GPU stores, plus the
+    world->grid affine (p_grid = p * scaling + offset)."""
+    links: np.ndarray
+    density: np.ndarray
+    sh: np.ndarray
+    basis_dim: int
+    offset: np.ndarray
+    scaling: np.ndarray
+
+    @property
+    def size(self):
+        return self.links.shape
+
+
+def _c_ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class MarchRef:
+    """The FP32 C restatement bound through ctypes."""
+
+    def __init__(self):
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        G, O = ctypes.POINTER(_RefGrid), ctypes.POINTER(_RefOpt)
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        lib.ref_render.argtypes = [G, vp, vp, vp, i64, O, vp, vp, vp]
+        lib.ref_render.restype = ctypes.c_int
+        lib.ref_samples.argtypes = [G, vp, vp, vp, i64, O, i64, vp, vp, vp, vp, vp, vp]
+        lib.ref_samples.restype = ctypes.c_int64
+        lib.ref_ray_setup.argtypes = [G, vp, vp, vp, i64, O, vp]
+        lib.ref_ray_setup.restype = ctypes.c_int
+        lib.ref_camera_rays.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_int32, ctypes.c_int32, vp, vp]
+        lib.ref_camera_rays.restype = ctypes.c_int
+        self.lib = lib
+
+    @staticmethod
+    def _grid(g: Grid, keep: list) -> _RefGrid:
+        links = np.ascontiguousarray(g.links, dtype=np.int32)
+        dens = np.ascontiguousarray(g.density, dtype=np.float32).reshape(-1)
+        sh = None if g.sh is None else np.ascontiguousarray(g.sh, dtype=np.float32)
+        keep += [links, dens, sh]
+        s = _RefGrid()
+        s.links, s.density, s.sh = (links.ctypes.data, dens.ctypes.data,
+                                    None if sh is None else sh.ctypes.data)
+        s.rows = dens.shape[0]
+        for i in range(3):
+            s.size[i] = links.shape[i]
+            s.offset[i] = float(np.float32(g.offset[i]))
+            s.scaling[i] = float(np.float32(g.scaling[i]))
+        s.basis_dim = g.basis_dim
+        return s
+
+    @staticmethod
+    def _opt(o: Opts, skip: bool = True) -> _RefOpt:
+        s = _RefOpt()
+        s.bg, s.step, s.sigma_thresh = o.background_brightness, o.step_size, o.sigma_thresh
+        s.stop_thresh, s.near_clip = o.stop_thresh, o.near_clip
+        s.spheric, s.last_opaque, s.skip = int(o.use_spheric_clip), int(o.last_sample_opaque), int(skip)
+        return s
+
+    @staticmethod
+    def _rays(origins, dirs, clip, keep):
+        o = np.ascontiguousarray(origins, dtype=np.float32)
+        d = np.ascontiguousarray(dirs, dtype=np.float32)
+        c = None if clip is None else np.ascontiguousarray(clip, dtype=np.float32)
+        keep += [o, d, c]
+        return o, d, c
+
+    def render(self, g: Grid, origins, dirs, opt: Opts, clip=None, skip: bool = True):
+        """FP32 colours (n,3), final log transmittance (n,), counters (n,4)."""
+        keep: list = []
+        gs = self._grid(g, keep)
+        o, d, c = self._rays(origins, dirs, clip, keep)
+        n = o.shape[0]
+        rgb = np.zeros((n, 3), np.float32)
+        lt = np.zeros((n,), np.float32)
+        st = np.zeros((n, 4), np.int32)
+        self.lib.ref_render(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), n,
+                            ctypes.byref(self._opt(opt, skip)), _c_ptr(rgb), _c_ptr(lt),
+                            _c_ptr(st))
+        return rgb, lt, st
+
+    def samples(self, g: Grid, origins, dirs, opt: Opts, clip=None, skip: bool = True):
+        """Every evaluated sample, in ray order: dict of ray (S,), t (S,),
+        l (S,3) int, p (S,3) float32, ws (S,), flag (S,) (bit 0: FP32 density
+        above sigma_thresh; bit 1: the FP32 march has not stopped before it)."""
+        keep: list = []
+        gs = self._grid(g, keep)
+        o, d, c = self._rays(origins, dirs, clip, keep)
+        n = o.shape[0]
+        ops = self._opt(opt, skip)
+        cap = max(1024, n * 64)
+        while True:
+            ray = np.empty(cap, np.int32)
+            t = np.empty(cap, np.float32)
+            l = np.empty((cap, 3), np.int32)
+            p = np.empty((cap, 3), np.float32)
+            ws = np.empty(cap, np.float32)
+            flag = np.empty(cap, np.uint8)
+            cnt = self.lib.ref_samples(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), n,
+                                       ctypes.byref(ops), cap, _c_ptr(ray), _c_ptr(t), _c_ptr(l),
+                                       _c_ptr(p), _c_ptr(ws), _c_ptr(flag))
+            if cnt <= cap:
+                return dict(ray=ray[:cnt], t=t[:cnt], l=l[:cnt], p=p[:cnt], ws=ws[:cnt],
+                            flag=flag[:cnt])
+            cap = int(cnt)
+
+    def ray_setup(self, g: Grid, origins, dirs, opt: Opts, clip=None) -> np.ndarray:
+        keep: list = []
+        gs = self._grid(g, keep)
+        o, d, c = self._rays(origins, dirs, clip, keep)
+        out = np.zeros((o.shape[0], 9), np.float32)
+        self.lib.ref_ray_setup(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), o.shape[0],
+                               ctypes.byref(self._opt(opt)), _c_ptr(out))
+        return out
+
+    def camera_rays(self, c2w, fx, fy, cx, cy, width, height):
+        m = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32)[:3, :4])
+        o = np.zeros((width * height, 3), np.float32)
+        d = np.zeros((width * height, 3), np.float32)
+        self.lib.ref_camera_rays(_c_ptr(m), fx, fy, cx, cy, width, height, _c_ptr(o), _c_ptr(d))
+        return o, d
+
+
+_MARCH: Optional[MarchRef] = None
+
+
+def march() -> MarchRef:
+    global _MARCH
+    if _MARCH is None:
+        _MARCH = MarchRef()
+    return _MARCH
+
+
+# ---------------------------------------------------------------------------
+# SH basis, Morton codes (svox2/utils.py)
+# ---------------------------------------------------------------------------
+def eval_sh_bases(basis_dim: int, dirs: torch.Tensor) -> torch.Tensor:
+    """Real SH basis, degree <= 2 (svox2/utils.py:115-162)."""
+    out = torch.empty((*dirs.shape[:-1], basis_dim), dtype=dirs.dtype, device=dirs.device)
+    out[..., 0] = SH_C0
+    if basis_dim > 1:
+        x, y, z = dirs.unbind(-1)
+        out[..., 1] = -SH_C1 * y
+        out[..., 2] = SH_C1 * z
+        out[..., 3] = -SH_C1 * x
+        if basis_dim > 4:
+            out[..., 4] = SH_C2[0] * x * y
+            out[..., 5] = SH_C2[1] * y * z
+            out[..., 6] = SH_C2[2] * (2.0 * z * z - x * x - y * y)
+            out[..., 7] = SH_C2[3] * x * z
+            out[..., 8] = SH_C2[4] * (x * x - y * y)
+    return out
+
+
+def morton_code_3(x: int, y: int, z: int) -> int:
+    """Bit interleave, x most significant per triple (svox2/utils.py:46-66),
+    written as a plain loop over bits."""
+    code = 0
+    for b in range(10):
+        code |= ((x >> b) & 1) << (3 * b + 2)
+        code |= ((y >> b) & 1) << (3 * b + 1)
+        code |= ((z >> b) & 1) << (3 * b)
+    return code
+
+
+def gen_morton(D: int) -> np.ndarray:
+    a = np.arange(D)
+    X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
+    out = np.zeros((D, D, D), np.int64)
+    for b in range(10):
+        out |= ((X >> b) & 1) << (3 * b + 2)
+        out |= ((Y >> b) & 1) << (3 * b + 1)
+        out |= ((Z >> b) & 1) << (3 * b)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# FP64 renderer over the exact sample list
+# ---------------------------------------------------------------------------
+_CORNERS = [(dx, dy, dz) for dx in (0, 1) for dy in (0, 1) for dz in (0, 1)]
+
+
+def corner_gather(links: np.ndarray, l: np.ndarray) -> np.ndarray:
+    """(S, 8) corner links of cells l (S, 3), corner order (dx, dy, dz)."""
+    return np.stack([links[l[:, 0] + dx, l[:, 1] + dy, l[:, 2] + dz] for dx, dy, dz in _CORNERS],
+                    axis=1)
+
+
+def corner_weights(p: torch.Tensor) -> torch.Tensor:
+    """(S, 8) trilinear weights of fractional offsets p (S, 3)."""
+    ws = []
+    for dx, dy, dz in _CORNERS:
+        wx = p[:, 0] if dx else 1 - p[:, 0]
+        wy = p[:, 1] if dy else 1 - p[:, 1]
+        wz = p[:, 2] if dz else 1 - p[:, 2]
+        ws.append(wx * wy * wz)
+    return torch.stack(ws, dim=1)
+
+
+def _composite(grid: Grid, dirs: np.ndarray, opt: Opts, density: torch.Tensor,
+               sh: torch.Tensor, smp: dict, n: int, thresholds: bool,
+               extra: Optional[dict] = None) -> torch.Tensor:
+    """Colours of n rays from their samples (ray ids 0..n-1, grouped by ray);
+    ``extra`` (if given) receives per-ray optical depth and end transmittance
+    and per-sample sigma and counted flags."""
+    B = grid.basis_dim
+    dt = torch.float64
+    bg = float(opt.background_brightness)
+    ray = torch.from_numpy(smp["ray"].astype(np.int64))
+    S = ray.numel()
+    if S == 0:
+        if extra is not None:
+            z = torch.zeros(n, dtype=dt)
+            extra.update(depth=z, t_end=torch.ones(n, dtype=dt), sigma=torch.zeros(0, dtype=dt),
+                         counted=torch.zeros(0, dtype=torch.bool))
+        return torch.full((n, 3), bg, dtype=dt) + 0 * (density.sum() + sh.sum())
+    basis = eval_sh_bases(B, torch.from_numpy(dirs).to(dt))  # at the raw dirs, as the kernel
+    lk = torch.from_numpy(corner_gather(grid.links, smp["l"]).astype(np.int64))
+    rows = density.shape[0]
+    valid = (lk >= 0) & (lk < rows)
+    lk0 = torch.where(valid, lk, torch.zeros_like(lk))
+    w = corner_weights(torch.from_numpy(smp["p"]).to(dt)) * valid.to(dt)
+    sigma = (w * density[lk0]).sum(1)
+    sh_s = (w[..., None] * sh[lk0]).sum(1)
+    col = (sh_s.view(S, 3, B) * basis[ray][:, None, :]).sum(-1) + 0.5
+    col = torch.clamp_min(col, 0.0)
+    ws = torch.from_numpy(smp["ws"].astype(np.float64))
+    if thresholds:
+        contrib = (sigma > opt.sigma_thresh).detach()
+        pcnt = torch.where(contrib, ws * sigma, torch.zeros_like(sigma))
+    else:
+        contrib = torch.ones(S, dtype=torch.bool)
+        pcnt = ws * torch.relu(sigma)
+    # exclusive per-ray prefix of optical depth, summed inside each ray only
+    first = torch.ones(S, dtype=torch.bool)
+    first[1:] = ray[1:] != ray[:-1]
+    start_idx = torch.nonzero(first).flatten()
+    seg = torch.cumsum(first.long(), 0) - 1
+    pos = torch.arange(S) - start_idx[seg]
+    L = int(pos.max()) + 1
+    pad = torch.zeros((start_idx.numel(), L), dtype=dt)
+    pad = pad.index_put((seg, pos), pcnt)
+    excl = (torch.cumsum(pad, 1) - pad)[seg, pos]
+    log_before = -excl
+    log_after = log_before - pcnt
+    if thresholds and opt.stop_thresh > 0:
+        stop = (contrib & (torch.exp(log_after) < opt.stop_thresh)).detach()
+        sb = torch.cumsum(stop.long(), 0) - stop.long()
+        alive = (sb - sb[start_idx][seg]) == 0
+        ray_stopped = torch.zeros(n, dtype=torch.long).index_add_(0, ray, stop.long()) > 0
+    else:
+        alive = torch.ones(S, dtype=torch.bool)
+        ray_stopped = torch.zeros(n, dtype=torch.bool)
+    weight = torch.exp(log_before) * (1.0 - torch.exp(-pcnt))
+    cw = weight * (alive & contrib).to(dt)
+    rgb = torch.zeros((n, 3), dtype=dt).index_add(0, ray, cw[:, None] * col)
+    depth = torch.zeros(n, dtype=dt).index_add(0, ray, pcnt * alive.to(dt))
+    t_end = torch.exp(-depth) * (~ray_stopped).to(dt)
+    if extra is not None:
+        extra.update(depth=depth, t_end=t_end, sigma=sigma, counted=(alive & contrib))
+    return rgb + t_end[:, None] * bg
+
+
+def _ray_chunks(smp: dict, n: int, max_samples: int):
+    """Splits rays [0, n) into ranges holding at most ~max_samples samples."""
+    counts = np.bincount(smp["ray"], minlength=n)
+    bounds = np.concatenate([[0], np.cumsum(counts)])
+    r0 = 0
+    while r0 < n:
+        r1 = r0 + 1
+        while r1 < n and bounds[r1 + 1] - bounds[r0] <= max_samples:
+            r1 += 1
+        s0, s1 = bounds[r0], bounds[r1]
+        sub = {k: v[s0:s1] for k, v in smp.items()}
+        sub["ray"] = sub["ray"] - r0
+        yield r0, r1, sub
+        r0 = r1
+
+
+def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
+                sh: torch.Tensor, clip=None, samples=None, thresholds: bool = True,
+                max_samples: int = 2_000_000):
+    """FP64 colours (n, 3) of the reference compositing over the kernel's exact
+    sample positions; differentiable in ``density`` (R,) and ``sh`` (R, C).
+
+    Compositing (svox2.py:767-785 and render_lerp_kernel_cuvol.cu:85-107):
+    w_i = T_i (1 - exp(-ws_i sigma_i)), colour_i = max(sh_i . basis + 0.5, 0),
+    C = sum_i w_i colour_i + T_end bg.  With ``thresholds`` (kernel semantics)
+    samples with sigma <= sigma_thresh are skipped and a ray stops after the
+    first sample whose transmittance falls below stop_thresh (T_end = 0);
+    without, sigma is clamped at 0 and every sample counts (the reference's
+    pure-PyTorch semantics).
+    """
+    origins = np.asarray(origins, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    n = origins.shape[0]
+    if samples is None:
+        samples = march().samples(grid, origins, dirs, opt, clip=clip)
+    outs = []
+    for r0, r1, sub in _ray_chunks(samples, n, max_samples):
+        outs.append(_composite(grid, dirs[r0:r1], opt, density, sh, sub, r1 - r0, thresholds))
+    return torch.cat(outs, 0) if outs else torch.zeros((0, 3), dtype=torch.float64)
+
+
+def touched_rows(grid: Grid, samples: dict) -> np.ndarray:
+    """Bool (R,) mask of rows that are a corner of a contributing sample of the
+    FP32 march (the kernel's mask_out, render_util.cuh:137-140)."""
+    sel = (samples["flag"] & 3) == 3
+    lk = corner_gather(grid.links, samples["l"][sel]).reshape(-1)
+    R = np.asarray(grid.density).reshape(-1).shape[0]
+    lk = lk[(lk >= 0) & (lk < R)]
+    m = np.zeros(R, bool)
+    m[lk] = True
+    return m
+
+
+def mse_grads_fp64(grid: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, samples=None,
+                   max_samples: int = 2_000_000, beta_loss: float = 0.0,
+                   sparsity_loss: float = 0.0):
+    """FP64 colours and the gradient, by autograd and ray chunk by ray chunk, of
+    mean((rgb - gt)^2) plus the two terms the fused kernel folds in
+    (render_lerp_kernel_cuvol.cu:257-262, :345-348):
+      beta:     sum_r b_r log T_r, b_r = beta/n (1 - T_r / (1 - T_r + 1e-3)) held
+                constant, T_r the ray's final transmittance (0 once stopped);
+      sparsity: sparsity * sum of log(1 + 2 sigma^2) over contributing samples."""
+    origins = np.asarray(origins, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    n = origins.shape[0]
+    gt = torch.as_tensor(np.asarray(rgb_gt, np.float64))
+    dens = torch.from_numpy(np.asarray(grid.density, np.float64).reshape(-1)).requires_grad_(True)
+    sh = torch.from_numpy(np.asarray(grid.sh, np.float64)).requires_grad_(True)
+    if samples is None:
+        samples = march().samples(grid, origins, dirs, opt, clip=clip)
+    rgbs = []
+    for r0, r1, sub in _ray_chunks(samples, n, max_samples):
+        extra = {}
+        rgb = _composite(grid, dirs[r0:r1], opt, dens, sh, sub, r1 - r0, True, extra)
+        loss = ((rgb - gt[r0:r1]) ** 2).sum() / (3.0 * n)
+        if beta_loss > 0:
+            t_end = extra["t_end"].detach()
+            b = beta_loss / n * (1.0 - t_end / (1.0 - t_end + 1e-3))
+            loss = loss - (b * extra["depth"]).sum()
+        if sparsity_loss > 0:
+            s_ = extra["sigma"][extra["counted"]]
+            loss = loss + sparsity_loss * torch.log1p(2.0 * s_ * s_).sum()
+        loss.backward()
+        rgbs.append(rgb.detach())
+    rgb = torch.cat(rgbs, 0) if rgbs else torch.zeros((0, 3), dtype=torch.float64)
+    gd = dens.grad if dens.grad is not None else torch.zeros_like(dens)
+    gs = sh.grad if sh.grad is not None else torch.zeros_like(sh)
+    return rgb.numpy(), gd.numpy(), gs.numpy()
+
+
+# ---------------------------------------------------------------------------
+# Literal restatement of _volume_render_gradcheck_lerp (svox2.py:658-884)
+# ---------------------------------------------------------------------------
+def volume_render_gradcheck_lerp(links: torch.Tensor, density: torch.Tensor, sh: torch.Tensor,
+                                 basis_dim: int, radius, center, origins: torch.Tensor,
+                                 dirs: torch.Tensor, step_size: float = 0.5,
+                                 near_clip: float = 0.0, background_brightness: float = 1.0,
+                                 dtype=torch.float32):
+    """The reference's pure-PyTorch renderer, restated with the grid passed in
+    and every tensor in ``dtype`` (the reference hard-codes float32 in
+    _fetch_links, svox2.py:562-569, and its accumulators, :695-696).
+    density (R, 1), sh (R, C), links (X, Y, Z) long/int."""
+    radius = torch.as_tensor(radius, dtype=dtype).expand(3)
+    center = torch.as_tensor(center, dtype=dtype).expand(3)
+    gsz = torch.tensor(links.shape, dtype=dtype)
+    offset = 0.5 * (1.0 - center / radius)
+    scaling = 0.5 / radius
+    # world2grid (svox2.py:1496-1509)
+    origins = torch.addcmul(offset * gsz - 0.5, origins.to(dtype), scaling * gsz)
+    dirs = dirs.to(dtype)
+    dirs = dirs / torch.norm(dirs, dim=-1, keepdim=True)
+    viewdirs = dirs
+    B = dirs.size(0)
+    dirs = dirs * (scaling * gsz)
+    delta_scale = 1.0 / dirs.norm(dim=1)
+    dirs = dirs * delta_scale.unsqueeze(-1)
+    sh_mult = eval_sh_bases(basis_dim, viewdirs)
+    invdirs = 1.0 / dirs
+    t1 = (-0.5 - origins) * invdirs
+    t2 = (gsz - 0.5 - origins) * invdirs
+    t = torch.min(t1, t2)
+    t[dirs == 0] = -1e9
+    t = torch.max(t, dim=-1).values.clamp_min_(near_clip)
+    tmax = torch.max(t1, t2)
+    tmax[dirs == 0] = 1e9
+    tmax = torch.min(tmax, dim=-1).values
+
+    log_light = torch.zeros(B, dtype=dtype)
+    out_rgb = torch.zeros((B, 3), dtype=dtype)
+    good = torch.arange(B)
+    mask = t <= tmax
+    good, origins, dirs, t, sh_mult, tmax = (good[mask], origins[mask], dirs[mask], t[mask],
+                                             sh_mult[mask], tmax[mask])
+    C = sh.size(1)
+
+    def fetch(lk):
+        m = lk >= 0
+        s = torch.zeros((lk.size(0), 1), dtype=dtype)
+        c = torch.zeros((lk.size(0), C), dtype=dtype)
+        idx = lk[m].long()
+        s[m] = density[idx].to(dtype)
+        c[m] = sh[idx].to(dtype)
+        return s, c
+
+    while good.numel() > 0:
+        pos = origins + t[:, None] * dirs
+        pos = pos.clamp_min(0.0)
+        pos = torch.stack([torch.clamp_max(pos[:, i], gsz[i] - 1) for i in range(3)], -1)
+        l = pos.to(torch.long).clamp_min(0)
+        l = torch.stack([torch.clamp_max(l[:, i], int(gsz[i]) - 2) for i in range(3)], -1)
+        pos = pos - l
+        lx, ly, lz = l.unbind(-1)
+        vals = [fetch(links[lx + dx, ly + dy, lz + dz]) for dx, dy, dz in _CORNERS]
+        wa, wb = 1.0 - pos, pos
+
+        def tri(i):
+            c00 = vals[0][i] * wa[:, 2:] + vals[1][i] * wb[:, 2:]
+            c01 = vals[2][i] * wa[:, 2:] + vals[3][i] * wb[:, 2:]
+            c10 = vals[4][i] * wa[:, 2:] + vals[5][i] * wb[:, 2:]
+            c11 = vals[6][i] * wa[:, 2:] + vals[7][i] * wb[:, 2:]
+            c0 = c00 * wa[:, 1:2] + c01 * wb[:, 1:2]
+            c1 = c10 * wa[:, 1:2] + c11 * wb[:, 1:2]
+            return c0 * wa[:, :1] + c1 * wb[:, :1]
+
+        sigma = tri(0)
+        rgb = tri(1)
+        log_att = -step_size * torch.relu(sigma[..., 0]) * delta_scale[good]
+        weight = torch.exp(log_light[good]) * (1.0 - torch.exp(log_att))
+        rgb = torch.clamp_min((sh_mult.unsqueeze(-2) * rgb.reshape(-1, 3, basis_dim)).sum(-1)
+                              + 0.5, 0.0)
+        out_rgb = out_rgb.index_add(0, good, weight[:, None] * rgb)
+        log_light = log_light.index_add(0, good, log_att)
+        t = t + step_size
+        mask = t <= tmax
+        good, origins, dirs, t, sh_mult, tmax = (good[mask], origins[mask], dirs[mask], t[mask],
+                                                 sh_mult[mask], tmax[mask])
+    if background_brightness:
+        out_rgb = out_rgb + torch.exp(log_light).unsqueeze(-1) * background_brightness
+    return out_rgb
+
+
+# ---------------------------------------------------------------------------
+# Sampling, TV, RMSprop, accel_dist_prop
+# ---------------------------------------------------------------------------
+def sample_trilerp(links: np.ndarray, density: torch.Tensor, sh: torch.Tensor, pts_grid: np.ndarray):
+    """Trilinear samples at grid coordinates (svox2.py:604-653): clamp to
+    [0, size-1], lower corner min(trunc, size-2), empty corners are 0.
+    Returns (density (P,1), sh (P,C)) in the tensors' dtype (differentiable)."""
+    size = np.array(links.shape)
+    p = np.clip(pts_grid.astype(np.float64), 0.0, size - 1.0)
+    l = np.minimum(p.astype(np.int64), size - 2)
+    frac = torch.from_numpy(p - l).to(density.dtype)
+    lk = torch.from_numpy(corner_gather(links, l).astype(np.int64))
+    valid = (lk >= 0) & (lk < density.shape[0])
+    lk0 = torch.where(valid, lk, torch.zeros_like(lk))
+    w = corner_weights(frac) * valid.to(density.dtype)
+    d = (w * density.reshape(-1)[lk0]).sum(1, keepdim=True)
+    s = (w[..., None] * sh[lk0]).sum(1)
+    return d, s
+
+
+def tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray, start_dim: int,
+                   end_dim: int, scale: float, ignore_edge: bool, ignore_last_z: bool):
+    """Gradient and touched-row mask of the sparse TV term
+    (loss_kernel.cu:178-246 with the launcher's scale / n_cells, :597),
+    including its edge rules.  Plain float64 loops over cells."""
+    data = np.asarray(data, np.float64)
+    R, D = data.shape
+    grad = np.zeros_like(data)
+    mask = np.zeros(R, bool)
+    s0, s1, s2 = links.shape
+    sc = np.array([s0, s1, s2], np.float64) / 256.0
+    n = len(rand_cells)
+    scale_c = scale / n
+    for xyz in np.asarray(rand_cells, np.int64):
+        z = xyz % s2
+        y = (xyz // s2) % s1
+        x = xyz // (s1 * s2)
+        l000 = int(links[x, y, z])
+        if ignore_edge and l000 == 0:
+            continue
+        l001 = int(links[x, y, z + 1]) if (z + 1 < s2 and (not ignore_last_z or z != s2 - 2)) else 0
+        l010 = int(links[x, y + 1, z]) if y + 1 < s1 else 0
+        l100 = int(links[x + 1, y, z]) if x + 1 < s0 else 0
+        if ignore_last_z and z == s2 - 2:
+            continue
+        for idx in range(start_dim, end_dim):
+            v000 = data[l000, idx] if l000 >= 0 else 0.0
+            nul = v000 if ignore_edge else 0.0
+            v001 = data[l001, idx] if l001 >= 0 else nul
+            v010 = data[l010, idx] if l010 >= 0 else nul
+            v100 = data[l100, idx] if l100 >= 0 else nul
+            dx, dy, dz = v100 - v000, v010 - v000, v001 - v000
+            idelta = scale_c / math.sqrt(1e-9 + dx * dx + dy * dy + dz * dz)
+            dx, dy, dz = dx * sc[0], dy * sc[1], dz * sc[2]
+            for lk, val in ((l000, -(dx + dy + dz)), (l001, dz), (l010, dy), (l100, dx)):
+                if lk >= 0 and val != 0.0:
+                    grad[lk, idx] += val * idelta
+                    mask[lk] = True
+    return grad, mask
+
+
+def rmsprop_step(data, rms, grad, rows_sel, beta, lr, eps, minval, lr_last):
+    """Closed form per element (optim_kernel.cu:15-25) on the selected rows;
+    returns new (data, rms, grad) as float64 arrays."""
+    data, rms, grad = (np.array(a, np.float64) for a in (data, rms, grad))
+    D = data.shape[1]
+    lrs = np.full(D, lr, np.float64)
+    lrs[-1] = lr_last
+    for r in rows_sel:
+        g = grad[r]
+        g2 = g * g
+        rms[r] = np.where(rms[r] == 0.0, g2, beta * rms[r] + (1 - beta) * g2)
+        data[r] = np.maximum(data[r] - lrs * g / (np.sqrt(rms[r]) + eps), minval)
+        grad[r] = 0.0
+    return data, rms, grad
+
+
+def accel_dist_prop(links: np.ndarray) -> np.ndarray:
+    """Empty links rewritten to -1 - k, k = consecutive empty ancestor levels of
+    the occupancy pyramid (misc_kernel.cu:109-181,356-416), by explicit
+    per-level boolean pyramids."""
+    links = np.array(links, np.int32)
+    occ = links >= 0
+    levels = []
+    cur = occ
+    while all(s > 1 for s in cur.shape):
+        nshape = tuple((s + 1) // 2 for s in cur.shape)
+        nxt = np.zeros(nshape, bool)
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    sub = cur[dx::2, dy::2, dz::2]
+                    nxt[:sub.shape[0], :sub.shape[1], :sub.shape[2]] |= sub
+        levels.append(nxt)
+        cur = nxt
+    out = links.copy()
+    X, Y, Z = np.nonzero(~occ)
+    res = np.full(X.shape, -1, np.int32)
+    done = np.zeros(X.shape, bool)
+    x, y, z = X.copy(), Y.copy(), Z.copy()
+    for lev in levels:
+        x, y, z = x >> 1, y >> 1, z >> 1
+        hit = lev[x, y, z] & ~done
+        done |= hit
+        res = np.where(done, res, res - 1)
+    out[X, Y, Z] = res
+    return out

@@ -1,0 +1,566 @@
+"""Python face of the gfx950 kernel library, shaped like the reference's
+pybind11 module ``svox2.csrc`` (svox2/csrc/svox2.cpp:74-172).
+
+The spec classes (``SparseGridSpec``, ``RaysSpec``, ``CameraSpec``,
+``RenderOptions``, ``GridOutputGrads``) carry the same fields as the reference's
+C++ structs (svox2/csrc/include/data_spec.hpp:18-135), and the functions keep
+the reference's names, argument order and in-place semantics.  Underneath, each
+call validates its tensors the way ``CHECK_INPUT`` does (svox2/csrc/include/
+util.hpp:3-12: device tensor, contiguous, expected dtype) and raises
+``RuntimeError`` on violation, then calls the C ABI declared in
+``include/svox2_amd.h`` through ctypes with raw device pointers and PyTorch's
+current HIP stream.  There is no fallback: if the library or a GPU is missing,
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import torch
+
+__all__ = [
+    "SparseGridSpec", "RaysSpec", "CameraSpec", "RenderOptions", "GridOutputGrads",
+    "volume_render_cuvol", "volume_render_cuvol_image", "volume_render_cuvol_backward",
+    "volume_render_cuvol_fused", "volume_render_cuvol_stats", "tv_grad_sparse",
+    "rmsprop_step", "sample_grid", "sample_grid_backward", "accel_dist_prop",
+    "library_path", "load_library", "EXPORTED_SYMBOLS",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libsvox2_amd.so")
+
+STORAGE_F32 = 0
+STORAGE_BF16 = 1
+BASIS_TYPE_SH = 1
+INDEXER_ALL, INDEXER_MASK, INDEXER_INDEX = 0, 1, 2
+
+# Every function the C header declares (checked against include/svox2_amd.h by
+# the tests).
+EXPORTED_SYMBOLS = (
+    "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_image",
+    "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
+    "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
+    "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_accel_dist_prop_workspace_bytes",
+    "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
+)
+
+
+# ---------------------------------------------------------------------------
+# ctypes mirrors of the C structs
+# ---------------------------------------------------------------------------
+class _Grid(ctypes.Structure):
+    _fields_ = [
+        ("links", ctypes.c_void_p),
+        ("density_data", ctypes.c_void_p),
+        ("sh_data", ctypes.c_void_p),
+        ("rows", ctypes.c_int64),
+        ("size", ctypes.c_int32 * 3),
+        ("basis_dim", ctypes.c_int32),
+        ("sh_data_dim", ctypes.c_int32),
+        ("basis_type", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
+        ("offset", ctypes.c_float * 3),
+        ("scaling", ctypes.c_float * 3),
+    ]
+
+
+class _Opt(ctypes.Structure):
+    _fields_ = [
+        ("background_brightness", ctypes.c_float),
+        ("step_size", ctypes.c_float),
+        ("sigma_thresh", ctypes.c_float),
+        ("stop_thresh", ctypes.c_float),
+        ("near_clip", ctypes.c_float),
+        ("use_spheric_clip", ctypes.c_int32),
+        ("last_sample_opaque", ctypes.c_int32),
+    ]
+
+
+class _Cam(ctypes.Structure):
+    _fields_ = [
+        ("c2w", ctypes.c_float * 12),
+        ("fx", ctypes.c_float), ("fy", ctypes.c_float),
+        ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+        ("ndc_coeffx", ctypes.c_float), ("ndc_coeffy", ctypes.c_float),
+    ]
+
+
+class _Grads(ctypes.Structure):
+    _fields_ = [
+        ("grad_density_out", ctypes.c_void_p),
+        ("grad_sh_out", ctypes.c_void_p),
+        ("mask_out", ctypes.c_void_p),
+    ]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Loads (once) and returns the kernel library; raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or library_path
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"svox2_amd kernel library not found at {p}; build it with "
+            "`python -m svox2_amd.build` (hipcc, gfx950)")
+    # torch is imported first so that the library binds to the HIP runtime
+    # torch already loaded (same soname), giving one runtime per process.
+    lib = ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+    vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+    G, O, Cm, Gr = (ctypes.POINTER(_Grid), ctypes.POINTER(_Opt), ctypes.POINTER(_Cam),
+                    ctypes.POINTER(_Grads))
+    sig = {
+        "svox2_volume_render_cuvol": [G, vp, vp, vp, i64, O, vp, vp, vp],
+        "svox2_volume_render_cuvol_image": [G, Cm, O, vp, vp, vp],
+        "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
+        "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, Gr, vp],
+        "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
+        "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
+        "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i32, vp],
+        "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, vp, i64, vp, i32,
+                                 i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
+        "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i32, vp, i64, f32, f32, f32, f32,
+                               f32, vp],
+        "svox2_accel_dist_prop": [vp, ctypes.POINTER(i32), vp, ctypes.c_size_t, vp],
+    }
+    for name, argtypes in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
+    lib.svox2_accel_dist_prop_workspace_bytes.restype = ctypes.c_size_t
+    lib.svox2_last_error.restype = ctypes.c_char_p
+    lib.svox2_version.restype = ctypes.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _call(name: str, *args) -> None:
+    lib = load_library()
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        msg = lib.svox2_last_error().decode(errors="replace")
+        raise RuntimeError(f"{name} failed (status {st}): {msg}")
+
+
+# ---------------------------------------------------------------------------
+# Spec classes (same fields as the reference's C++ structs)
+# ---------------------------------------------------------------------------
+class SparseGridSpec:
+    """SparseGridSpec (data_spec.hpp:18-52)."""
+
+    def __init__(self):
+        self.density_data: Optional[torch.Tensor] = None
+        self.sh_data: Optional[torch.Tensor] = None
+        self.links: Optional[torch.Tensor] = None
+        self._offset: Optional[torch.Tensor] = None
+        self._scaling: Optional[torch.Tensor] = None
+        self.background_links: Optional[torch.Tensor] = None
+        self.background_data: Optional[torch.Tensor] = None
+        self.basis_dim: int = 9
+        self.basis_type: int = BASIS_TYPE_SH
+        self.basis_data: Optional[torch.Tensor] = None
+
+
+class RaysSpec:
+    """RaysSpec (data_spec.hpp:104-113), plus an optional per-ray (t_near, t_far)
+    world-distance clip ``clip`` of shape (n, 2)."""
+
+    def __init__(self):
+        self.origins: Optional[torch.Tensor] = None
+        self.dirs: Optional[torch.Tensor] = None
+        self.clip: Optional[torch.Tensor] = None
+
+
+class CameraSpec:
+    """CameraSpec (data_spec.hpp:84-102)."""
+
+    def __init__(self):
+        self.c2w: Optional[torch.Tensor] = None
+        self.fx = self.fy = self.cx = self.cy = 0.0
+        self.width = self.height = 0
+        self.ndc_coeffx = self.ndc_coeffy = -1.0
+
+
+class RenderOptions:
+    """RenderOptions (data_spec.hpp:115-135)."""
+
+    def __init__(self):
+        self.background_brightness = 1.0
+        self.step_size = 0.5
+        self.sigma_thresh = 1e-10
+        self.stop_thresh = 1e-7
+        self.near_clip = 0.0
+        self.use_spheric_clip = False
+        self.last_sample_opaque = False
+
+
+class GridOutputGrads:
+    """GridOutputGrads (data_spec.hpp:54-82)."""
+
+    def __init__(self):
+        self.grad_density_out: Optional[torch.Tensor] = None
+        self.grad_sh_out: Optional[torch.Tensor] = None
+        self.grad_basis_out: Optional[torch.Tensor] = None
+        self.grad_background_out: Optional[torch.Tensor] = None
+        self.mask_out: Optional[torch.Tensor] = None
+        self.mask_background_out: Optional[torch.Tensor] = None
+
+
+# ---------------------------------------------------------------------------
+# Checks (CHECK_INPUT / TORCH_CHECK equivalents)
+# ---------------------------------------------------------------------------
+def _check(t: Optional[torch.Tensor], name: str, dtypes=None, ndim: Optional[int] = None):
+    if t is None:
+        raise RuntimeError(f"{name} must be defined")
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} must be a tensor")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+    if dtypes is not None and t.dtype not in dtypes:
+        raise RuntimeError(f"{name} must have dtype in {dtypes}, got {t.dtype}")
+    if ndim is not None and t.dim() != ndim:
+        raise RuntimeError(f"{name} must have {ndim} dims, got {t.dim()}")
+    return t
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(t: torch.Tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+class _on_device:
+    """Makes the tensor's device current for the duration of a call
+    (the reference's DEVICE_GUARD, include/cuda_util.cuh:10-11)."""
+
+    def __init__(self, t: torch.Tensor):
+        self.idx = t.device.index
+        self.prev = None
+
+    def __enter__(self):
+        cur = torch.cuda.current_device()
+        if self.idx is not None and cur != self.idx:
+            self.prev = cur
+            torch.cuda.set_device(self.idx)
+
+    def __exit__(self, *exc):
+        if self.prev is not None:
+            torch.cuda.set_device(self.prev)
+
+
+_DATA_DTYPES = (torch.float32, torch.bfloat16)
+
+
+def _storage_of(t: torch.Tensor) -> int:
+    return STORAGE_BF16 if t.dtype == torch.bfloat16 else STORAGE_F32
+
+
+def _vec3(v) -> Tuple[float, float, float]:
+    if isinstance(v, torch.Tensor):
+        if v.is_cuda:
+            raise RuntimeError("_offset/_scaling must be CPU tensors")
+        v = v.detach().double().flatten().tolist()
+    v = [float(x) for x in v]
+    if len(v) != 3:
+        raise RuntimeError("_offset/_scaling must have 3 entries")
+    return v[0], v[1], v[2]
+
+
+def _grid_struct(g: SparseGridSpec, need_sh: bool = True) -> _Grid:
+    links = _check(g.links, "links", (torch.int32,), 3)
+    dens = _check(g.density_data, "density_data", _DATA_DTYPES, 2)
+    sh = g.sh_data
+    if need_sh or sh is not None:
+        sh = _check(sh, "sh_data", _DATA_DTYPES, 2)
+        if sh.dtype != dens.dtype:
+            raise RuntimeError("density_data and sh_data must share a storage dtype")
+        if sh.size(0) != dens.size(0):
+            raise RuntimeError("density_data and sh_data row counts differ")
+    if dens.size(1) != 1:
+        raise RuntimeError("density_data must be (rows, 1)")
+    if g.background_links is not None and g.background_links.numel() > 0:
+        raise RuntimeError("background (MSI) models are not supported")
+    if g.basis_type != BASIS_TYPE_SH:
+        raise RuntimeError(f"basis_type {g.basis_type} not supported (SH only)")
+    s = _Grid()
+    s.links = links.data_ptr()
+    s.density_data = dens.data_ptr()
+    s.sh_data = sh.data_ptr() if sh is not None else None
+    s.rows = dens.size(0)
+    for i in range(3):
+        s.size[i] = links.size(i)
+    s.basis_dim = int(g.basis_dim)
+    s.sh_data_dim = sh.size(1) if sh is not None else 3 * int(g.basis_dim)
+    s.basis_type = int(g.basis_type)
+    s.storage = _storage_of(dens)
+    off, sc = _vec3(g._offset), _vec3(g._scaling)
+    for i in range(3):
+        s.offset[i] = off[i]
+        s.scaling[i] = sc[i]
+    return s
+
+
+def _opt_struct(o: RenderOptions) -> _Opt:
+    s = _Opt()
+    s.background_brightness = float(o.background_brightness)
+    s.step_size = float(o.step_size)
+    s.sigma_thresh = float(o.sigma_thresh)
+    s.stop_thresh = float(o.stop_thresh)
+    s.near_clip = float(o.near_clip)
+    s.use_spheric_clip = int(bool(o.use_spheric_clip))
+    s.last_sample_opaque = int(bool(o.last_sample_opaque))
+    return s
+
+
+def _rays(r: RaysSpec):
+    o = _check(r.origins, "origins", (torch.float32,), 2)
+    d = _check(r.dirs, "dirs", (torch.float32,), 2)
+    if o.size(1) != 3 or d.shape != o.shape:
+        raise RuntimeError("origins/dirs must both be (n, 3)")
+    clip = getattr(r, "clip", None)
+    if clip is not None:
+        _check(clip, "clip", (torch.float32,), 2)
+        if clip.shape != (o.size(0), 2):
+            raise RuntimeError("clip must be (n, 2)")
+    return o, d, clip
+
+
+def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
+    s = _Grads()
+    if gr.grad_density_out is not None:
+        t = _check(gr.grad_density_out, "grad_density_out", (torch.float32,))
+        if t.numel() != rows:
+            raise RuntimeError("grad_density_out must have one value per row")
+        s.grad_density_out = t.data_ptr()
+    if gr.grad_sh_out is not None:
+        t = _check(gr.grad_sh_out, "grad_sh_out", (torch.float32,))
+        if t.numel() != rows * sh_dim:
+            raise RuntimeError("grad_sh_out must be (rows, sh_data_dim)")
+        s.grad_sh_out = t.data_ptr()
+    if gr.mask_out is not None and gr.mask_out.numel() > 0:
+        t = _check(gr.mask_out, "mask_out", (torch.bool, torch.uint8))
+        if t.numel() != rows:
+            raise RuntimeError("mask_out must have one entry per row")
+        s.mask_out = t.data_ptr()
+    if gr.grad_basis_out is not None or gr.grad_background_out is not None:
+        raise RuntimeError("basis / background gradients are not supported")
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Entry points
+# ---------------------------------------------------------------------------
+def volume_render_cuvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                        log_transmit_out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """volume_render_cuvol (render_lerp_kernel_cuvol.cu:918-958): (n, 3) colours."""
+    g = _grid_struct(grid)
+    o, d, clip = _rays(rays)
+    out = torch.empty_like(o)
+    if log_transmit_out is not None:
+        _check(log_transmit_out, "log_transmit_out", (torch.float32,))
+        if log_transmit_out.numel() != o.size(0):
+            raise RuntimeError("log_transmit_out must have one value per ray")
+    with _on_device(o):
+        _call("svox2_volume_render_cuvol", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _ptr(log_transmit_out),
+              _stream(o))
+    return out
+
+
+def volume_render_cuvol_stats(grid: SparseGridSpec, rays: RaysSpec,
+                              opt: RenderOptions) -> torch.Tensor:
+    """Per-ray (iterations, evaluated, contributing, hash) int32 counters of the
+    forward march (no reference counterpart; used for measurement and bit-exact
+    index tests)."""
+    g = _grid_struct(grid)
+    o, d, clip = _rays(rays)
+    out = torch.empty((o.size(0), 4), dtype=torch.int32, device=o.device)
+    with _on_device(o):
+        _call("svox2_volume_render_cuvol_stats", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _stream(o))
+    return out
+
+
+def _cam_struct(cam: CameraSpec) -> _Cam:
+    c2w = cam.c2w
+    if not isinstance(c2w, torch.Tensor) or c2w.dim() != 2 or c2w.size(1) != 4 \
+            or c2w.size(0) < 3 or not c2w.is_floating_point():
+        raise RuntimeError("c2w must be a floating (3 or 4, 4) tensor")
+    vals = c2w[:3].detach().double().cpu().flatten().tolist()
+    s = _Cam()
+    for i in range(12):
+        s.c2w[i] = vals[i]
+    s.fx, s.fy, s.cx, s.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
+    s.width, s.height = int(cam.width), int(cam.height)
+    s.ndc_coeffx, s.ndc_coeffy = float(cam.ndc_coeffx), float(cam.ndc_coeffy)
+    return s
+
+
+def volume_render_cuvol_image(grid: SparseGridSpec, cam: CameraSpec, opt: RenderOptions,
+                              log_transmit_out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """volume_render_cuvol_image (render_lerp_kernel_cuvol.cu:960-1008): (H, W, 3)."""
+    g = _grid_struct(grid)
+    dev = grid.sh_data.device
+    out = torch.empty((int(cam.height), int(cam.width), 3), dtype=torch.float32, device=dev)
+    if log_transmit_out is not None:
+        _check(log_transmit_out, "log_transmit_out", (torch.float32,))
+    with _on_device(grid.sh_data):
+        _call("svox2_volume_render_cuvol_image", ctypes.byref(g), ctypes.byref(_cam_struct(cam)),
+              ctypes.byref(_opt_struct(opt)), _ptr(out), _ptr(log_transmit_out),
+              _stream(grid.sh_data))
+    return out
+
+
+def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                                 grad_out: torch.Tensor, color_cache: torch.Tensor,
+                                 grads: GridOutputGrads) -> None:
+    """volume_render_cuvol_backward (render_lerp_kernel_cuvol.cu:1010-1068)."""
+    g = _grid_struct(grid)
+    o, d, clip = _rays(rays)
+    _check(grad_out, "grad_out", (torch.float32,), 2)
+    _check(color_cache, "color_cache", (torch.float32,), 2)
+    if grad_out.shape != o.shape or color_cache.shape != o.shape:
+        raise RuntimeError("grad_out / color_cache must be (n, 3)")
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    with _on_device(o):
+        _call("svox2_volume_render_cuvol_backward", ctypes.byref(g), _ptr(o), _ptr(d),
+              _ptr(clip), o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(grad_out),
+              _ptr(color_cache), ctypes.byref(gs), _stream(o))
+
+
+def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                              rgb_gt: torch.Tensor, beta_loss: float, sparsity_loss: float,
+                              rgb_out: torch.Tensor, grads: GridOutputGrads) -> None:
+    """volume_render_cuvol_fused (render_lerp_kernel_cuvol.cu:1070-1152)."""
+    g = _grid_struct(grid)
+    o, d, clip = _rays(rays)
+    _check(rgb_gt, "rgb_gt", (torch.float32,), 2)
+    _check(rgb_out, "rgb_out", (torch.float32,), 2)
+    if rgb_gt.shape != o.shape or rgb_out.shape != o.shape:
+        raise RuntimeError("rgb_gt / rgb_out must be (n, 3)")
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    with _on_device(o):
+        _call("svox2_volume_render_cuvol_fused", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(rgb_gt), float(beta_loss),
+              float(sparsity_loss), _ptr(rgb_out), ctypes.byref(gs), _stream(o))
+
+
+def sample_grid(grid: SparseGridSpec, points: torch.Tensor,
+                want_colors: bool) -> Tuple[torch.Tensor, torch.Tensor]:
+    """sample_grid (svox2_kernel.cu:181-217): (density (n,1), sh (n,C) or (0,C))."""
+    g = _grid_struct(grid, need_sh=bool(want_colors))
+    _check(points, "points", (torch.float32,), 2)
+    if points.size(1) != 3:
+        raise RuntimeError("points must be (n, 3)")
+    n = points.size(0)
+    dens = torch.empty((n, 1), dtype=torch.float32, device=points.device)
+    sh = torch.empty((n if want_colors else 0, g.sh_data_dim), dtype=torch.float32,
+                     device=points.device)
+    with _on_device(points):
+        _call("svox2_sample_grid", ctypes.byref(g), _ptr(points), n, int(bool(want_colors)),
+              _ptr(dens), _ptr(sh) if want_colors else None, _stream(points))
+    return dens, sh
+
+
+def sample_grid_backward(grid: SparseGridSpec, points: torch.Tensor,
+                         grad_out_density: torch.Tensor, grad_out_sh: torch.Tensor,
+                         grad_density_out: torch.Tensor, grad_sh_out: torch.Tensor,
+                         want_colors: bool) -> None:
+    """sample_grid_backward (svox2_kernel.cu:219-267)."""
+    g = _grid_struct(grid, need_sh=False)
+    _check(points, "points", (torch.float32,), 2)
+    n = points.size(0)
+    _check(grad_out_density, "grad_out_density", (torch.float32,), 2)
+    _check(grad_density_out, "grad_density_out", (torch.float32,))
+    if grad_out_density.shape != (n, 1) or grad_density_out.numel() != g.rows:
+        raise RuntimeError("density gradient shapes do not match")
+    if want_colors:
+        _check(grad_out_sh, "grad_out_sh", (torch.float32,), 2)
+        _check(grad_sh_out, "grad_sh_out", (torch.float32,))
+        if grad_out_sh.shape != (n, g.sh_data_dim) or \
+                grad_sh_out.numel() != g.rows * g.sh_data_dim:
+            raise RuntimeError("sh gradient shapes do not match")
+    with _on_device(points):
+        _call("svox2_sample_grid_backward", ctypes.byref(g), _ptr(points), n,
+              _ptr(grad_out_density), _ptr(grad_out_sh) if want_colors else None,
+              _ptr(grad_density_out), _ptr(grad_sh_out) if want_colors else None,
+              int(bool(want_colors)), _stream(points))
+
+
+def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Tensor,
+                   mask_out: torch.Tensor, start_dim: int, end_dim: int, scale: float,
+                   use_logalpha: bool, logalpha_delta: float, ignore_edge: bool,
+                   ignore_last_z: bool, ndc_coeffx: float, ndc_coeffy: float,
+                   grad_data: torch.Tensor) -> None:
+    """tv_grad_sparse (loss_kernel.cu:558-604)."""
+    _check(links, "links", (torch.int32,), 3)
+    _check(data, "data", _DATA_DTYPES, 2)
+    _check(grad_data, "grad_data", (torch.float32,), 2)
+    _check(rand_cells, "rand_cells", (torch.int32,))
+    if grad_data.shape != data.shape:
+        raise RuntimeError("grad_data must match data's shape")
+    mask_ptr = None
+    if mask_out is not None and mask_out.dim() > 0 and mask_out.numel() > 0:
+        _check(mask_out, "mask_out", (torch.bool, torch.uint8))
+        if mask_out.numel() != data.size(0):
+            raise RuntimeError("mask_out must have one entry per row")
+        mask_ptr = _ptr(mask_out)
+    size = (ctypes.c_int32 * 3)(*links.shape)
+    with _on_device(data):
+        _call("svox2_tv_grad_sparse", _ptr(links), size, _ptr(data), _storage_of(data),
+              data.size(0), data.size(1), _ptr(rand_cells), rand_cells.numel(), mask_ptr,
+              int(start_dim), int(end_dim), float(scale), int(bool(use_logalpha)),
+              float(logalpha_delta), int(bool(ignore_edge)), int(bool(ignore_last_z)),
+              float(ndc_coeffx), float(ndc_coeffy), _ptr(grad_data), _stream(data))
+
+
+def rmsprop_step(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor,
+                 indexer: torch.Tensor, beta: float, lr: float, epsilon: float, minval: float,
+                 lr_last: float) -> None:
+    """rmsprop_step (optim_kernel.cu:154-218).  ``indexer``: 0-dim tensor = all
+    rows, bool/uint8 (rows,) mask, or int64 row indices; an empty 1-d indexer
+    skips the update."""
+    _check(data, "data", _DATA_DTYPES, 2)
+    _check(rms, "rms", (torch.float32,))
+    _check(grad, "grad", (torch.float32,))
+    if rms.shape != data.shape or grad.shape != data.shape:
+        raise RuntimeError("rms / grad must match data's shape")
+    rows, dim = data.shape
+    if indexer.dim() == 0:
+        kind, iptr, n = INDEXER_ALL, None, 0
+    elif indexer.numel() == 0:
+        return
+    elif indexer.dtype in (torch.bool, torch.uint8):
+        _check(indexer, "indexer", (torch.bool, torch.uint8), 1)
+        kind, iptr, n = INDEXER_MASK, _ptr(indexer), indexer.numel()
+    else:
+        _check(indexer, "indexer", (torch.int64,), 1)
+        kind, iptr, n = INDEXER_INDEX, _ptr(indexer), indexer.numel()
+    with _on_device(data):
+        _call("svox2_rmsprop_step", _ptr(data), _storage_of(data), _ptr(rms), _ptr(grad), rows,
+              dim, kind, iptr, n, float(beta), float(lr), float(epsilon), float(minval),
+              float(lr_last), _stream(data))
+
+
+def accel_dist_prop(links: torch.Tensor) -> None:
+    """accel_dist_prop (misc_kernel.cu:356-416), in place on ``links``."""
+    _check(links, "links", (torch.int32,), 3)
+    lib = load_library()
+    size = (ctypes.c_int32 * 3)(*links.shape)
+    nbytes = lib.svox2_accel_dist_prop_workspace_bytes(size)
+    ws = torch.empty(max(int(nbytes), 4), dtype=torch.uint8, device=links.device)
+    with _on_device(links):
+        _call("svox2_accel_dist_prop", _ptr(links), size, _ptr(ws), ctypes.c_size_t(nbytes),
+              _stream(links))

@@ -1,0 +1,143 @@
+// Shared device helpers and host-side argument checking for the gfx950
+// sparse-voxel kernels.  Every kernel here is built with -ffp-contract=off so
+// that the index arithmetic (sample positions, cell indices, skip distances,
+// trilinear density) rounds exactly as written and can be reproduced bit for
+// bit by a plain C restatement; fused multiply-adds are spelled out with fmaf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/svox2_amd.h"
+
+namespace svox2amd {
+
+// ---------------------------------------------------------------------------
+// Host-side error reporting (thread-local last error, returned as status).
+// ---------------------------------------------------------------------------
+std::string& last_error_slot();
+
+inline int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    last_error_slot() = buf;
+    return code;
+}
+
+inline int ok() {
+    last_error_slot().clear();
+    return SVOX2_OK;
+}
+
+// Checks the launch that was just enqueued.
+inline int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        return fail(SVOX2_ERR_HIP, "%s: kernel launch failed: %s", what,
+                    hipGetErrorString(e));
+    }
+    return ok();
+}
+
+#define SVOX2_REQUIRE(cond, ...)                                   \
+    do {                                                           \
+        if (!(cond)) return ::svox2amd::fail(SVOX2_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Storage element loads.  BF16 values are raw bit patterns (uint16_t); the
+// widening to float is exact.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ld(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float ld(const uint16_t* p, int64_t i) {
+    return __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
+}
+
+// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) {
+        return static_cast<uint16_t>((u >> 16) | 0x40u);
+    }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ void st(float* p, int64_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st(uint16_t* p, int64_t i, float v) {
+    p[i] = f32_to_bf16_bits(v);
+}
+
+// (1 - w) a + w b with one rounding for the product-add, the form the
+// reference's trilinear interpolation uses (include/cuda_util.cuh:72-75).
+__device__ __forceinline__ float lerpf(float a, float b, float w) {
+    return fmaf(w, b - a, a);
+}
+
+// No-return float atomic add; lowers to global_atomic_add_f32.
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) {
+    unsafeAtomicAdd(p, v);
+}
+
+// ---------------------------------------------------------------------------
+// Grid view passed by value to kernels.
+// ---------------------------------------------------------------------------
+template <typename DT>
+struct GridView {
+    const int32_t* __restrict__ links;
+    const DT* __restrict__ density;
+    const DT* __restrict__ sh;
+    int size[3];
+    int stride_x;  // size[1] * size[2]
+    int stride_y;  // size[2]
+    int rows;
+    int basis_dim;
+    int sh_dim;
+    float offset[3];
+    float scaling[3];
+    // A link addresses a data row iff 0 <= link < rows; negative links are
+    // empty cells and out-of-range links are treated as empty.
+    __device__ __forceinline__ bool valid(int link) const {
+        return static_cast<uint32_t>(link) < static_cast<uint32_t>(rows);
+    }
+};
+
+template <typename V>
+struct ViewDT;
+template <typename DT>
+struct ViewDT<GridView<DT>> {
+    using type = DT;
+};
+
+template <typename DT>
+inline GridView<DT> make_view(const svox2_grid_t& g) {
+    GridView<DT> v;
+    v.links = g.links;
+    v.density = static_cast<const DT*>(g.density_data);
+    v.sh = static_cast<const DT*>(g.sh_data);
+    for (int i = 0; i < 3; ++i) {
+        v.size[i] = g.size[i];
+        v.offset[i] = g.offset[i];
+        v.scaling[i] = g.scaling[i];
+    }
+    v.stride_x = g.size[1] * g.size[2];
+    v.stride_y = g.size[2];
+    v.rows = static_cast<int>(g.rows);
+    v.basis_dim = g.basis_dim;
+    v.sh_dim = g.sh_data_dim;
+    return v;
+}
+
+// Validates a grid descriptor; `need_sh` requires sh_data to be present.
+int check_grid(const svox2_grid_t* g, bool need_sh);
+
+inline unsigned blocks_for(int64_t n, int threads) {
+    return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+}  // namespace svox2amd

@@ -1,0 +1,132 @@
+"""Host-side logic of the SparseGrid facade (no kernel launches): options,
+rays, cameras, grid construction and layouts, coordinate maps, random TV
+cells, indexer conversion, checkpoints, and that every compute method refuses
+to run without a GPU."""
+import numpy as np
+import pytest
+import torch
+
+import svox2_amd
+from svox2_amd import scenes, utils
+
+
+def test_render_options_defaults_match_reference():
+    o = svox2_amd.RenderOptions()
+    # svox2/svox2.py:31-55
+    assert (o.backend, o.background_brightness, o.step_size, o.sigma_thresh, o.stop_thresh,
+            o.last_sample_opaque, o.near_clip, o.use_spheric_clip) == \
+        ("cuvol", 1.0, 0.5, 1e-10, 1e-7, False, 0.0, False)
+    c = o._to_cpp()
+    assert c.step_size == 0.5 and c.stop_thresh == 1e-7
+
+
+def test_rays_and_camera():
+    r = svox2_amd.Rays(torch.zeros(10, 3), torch.ones(10, 3), torch.zeros(10, 2))
+    s = r[2:5]
+    assert s.origins.shape == (3, 3) and s.clip.shape == (3, 2) and not r.is_cuda
+    cam = svox2_amd.Camera(torch.eye(4)[:3], fx=100.0, width=8, height=6)
+    assert cam.fy_val == 100.0 and cam.cx_val == 4.0 and cam.cy_val == 3.0 and not cam.using_ndc
+    rays = cam.gen_rays()
+    assert rays.origins.shape == (48, 3)
+    # centre-ish pixel looks along +z (identity rotation)
+    d = rays.dirs.view(6, 8, 3)
+    assert torch.allclose(d.norm(dim=-1), torch.ones(6, 8), atol=1e-6)
+    assert d[3, 4, 2] > 0.99
+
+
+def test_grid_construction_layouts():
+    g = svox2_amd.SparseGrid(reso=8, use_z_order=True)
+    assert g.links.shape == (8, 8, 8) and g.sh_data.shape == (512, 27)
+    assert torch.equal(g.links, utils.gen_morton(8, dtype=torch.int32))
+    g2 = svox2_amd.SparseGrid(reso=[4, 6, 5])
+    assert torch.equal(g2.links.flatten(), torch.arange(120, dtype=torch.int32))
+    g3 = svox2_amd.SparseGrid(reso=16, use_sphere_bound=True)
+    valid = g3.links[g3.links >= 0]
+    assert g3.capacity == valid.numel() < 16 ** 3
+    assert torch.equal(torch.sort(valid).values, torch.arange(g3.capacity, dtype=torch.int32))
+    gb = svox2_amd.SparseGrid(reso=4, dtype=torch.bfloat16)
+    assert gb.sh_data.dtype == torch.bfloat16 and gb.density_data.grad_dtype == torch.float32
+    with pytest.raises(NotImplementedError):
+        svox2_amd.SparseGrid(reso=4, basis_type=svox2_amd.BASIS_TYPE_MLP)
+    with pytest.raises(NotImplementedError):
+        svox2_amd.SparseGrid(reso=4, background_nlayers=4)
+
+
+def test_world_grid_maps_inverse():
+    g = svox2_amd.SparseGrid(reso=[10, 12, 14], radius=[1.5, 2.0, 0.5], center=[0.1, -0.2, 0.3])
+    p = torch.rand(100, 3) * 2 - 1
+    assert torch.allclose(g.grid2world(g.world2grid(p)), p, atol=1e-5)
+    # voxel centres at integers: the world corner maps to -0.5
+    corner = (g.center - g.radius)[None]
+    assert torch.allclose(g.world2grid(corner), torch.full((1, 3), -0.5), atol=1e-5)
+
+
+def test_rand_cells_and_indexer_conversion():
+    g = svox2_amd.SparseGrid(reso=10)
+    np.random.seed(0)
+    cells = g._get_rand_cells(0.05)
+    assert cells.numel() == 50 and cells.dtype == torch.int32
+    assert int(cells.min()) >= 0 and int(cells.max()) < 1000
+    # wrap-around keeps a contiguous run modulo the cell count
+    d = torch.diff(cells.long()) % 1000
+    assert torch.all(d == 1)
+    assert g._get_rand_cells(1.0) is None
+    assert g._maybe_convert_sparse_grad_indexer().dim() == 0
+    g.sparse_grad_indexer = torch.zeros(1000, dtype=torch.bool)
+    g.sparse_grad_indexer[[3, 7]] = True
+    idx = g._maybe_convert_sparse_grad_indexer()
+    assert idx.dtype == torch.int64 and idx.tolist() == [3, 7]
+    g.sparse_grad_indexer[:500] = True
+    assert g._maybe_convert_sparse_grad_indexer().dtype == torch.bool
+
+
+def test_compute_methods_refuse_cpu():
+    g = svox2_amd.SparseGrid(reso=8)
+    rays = svox2_amd.Rays(torch.zeros(4, 3), torch.ones(4, 3))
+    with pytest.raises(RuntimeError, match="GPU"):
+        g.volume_render(rays)
+    with pytest.raises(NotImplementedError):
+        g.volume_render(rays, use_kernel=False)
+    with pytest.raises(RuntimeError, match="GPU"):
+        g.volume_render_fused(rays, torch.zeros(4, 3))
+    with pytest.raises(RuntimeError, match="GPU"):
+        g.sample(torch.zeros(4, 3))
+    with pytest.raises(RuntimeError, match="GPU"):
+        g.accelerate()
+    with pytest.raises(RuntimeError, match="GPU"):
+        g.optim_density_step(0.1)
+
+
+def test_ray_lengths():
+    g = svox2_amd.SparseGrid(reso=8)
+    rays = svox2_amd.Rays(torch.tensor([[-2.0, 0.1, 0.2]]), torch.tensor([[1.0, 0.0, 0.0]]))
+    assert float(g.volume_render(rays, return_raylen=True)[0]) == pytest.approx(8.0)
+
+
+def test_save_load_roundtrip(tmp_path):
+    gen = torch.Generator().manual_seed(0)
+    sc = scenes.make_blob_grid(16, scenes.BlobSpec(2, (0.2, 0.2), (10.0, 10.0)), gen)
+    g = scenes.to_sparse_grid(sc, device="cpu", accelerate=False)
+    path = str(tmp_path / "ckpt.npz")
+    g.save(path)
+    z = np.load(path)
+    assert z["sh_data"].dtype == np.float16 and z["links"].dtype == np.int32
+    assert set(z.files) == {"radius", "center", "links", "density_data", "sh_data", "basis_type"}
+    g2 = svox2_amd.SparseGrid.load(path)
+    assert torch.equal(g2.links, g.links)
+    assert torch.equal(g2.density_data.data, g.density_data.data)
+    assert torch.allclose(g2.sh_data.data, g.sh_data.data.half().float())
+
+
+def test_scene_generators_deterministic():
+    a = scenes.workload_dense64(seed=0, n_rays=16)
+    b = scenes.workload_dense64(seed=0, n_rays=16)
+    assert torch.equal(a[0].density, b[0].density) and torch.equal(a[1], b[1])
+    assert torch.allclose(a[2].norm(dim=1), torch.ones(16), atol=1e-6)
+    gen = torch.Generator().manual_seed(0)
+    cams = scenes.CameraSet(scenes.hemisphere_cameras(4, gen))
+    o, d = cams.random_batch(100, gen)
+    assert torch.allclose(o.norm(dim=1), torch.full((100,), 4.0), atol=1e-5)
+    assert torch.all(o[:, 2] >= 0)
+    # rays point towards the scene
+    assert torch.all((-o * d).sum(1) > 0)

@@ -1,0 +1,155 @@
+"""GPU gradients (fused MSE kernel and the autograd backward) against the FP64
+oracle's autograd gradient and FP64 finite differences.
+
+Tolerances: colours 2e-5 absolute; gradients max|g - g64| <= 1e-3 * max|g64|
+(per tensor; float32 accumulation through atomics in arbitrary order, so the
+results are not bit-reproducible); the touched-row mask is bit-exact against
+the rows that are corners of a contributing sample of the FP32 march.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle.oracle as O
+import svox2_amd
+from svox2_amd import scenes
+from tests.conftest import host_grid
+
+pytestmark = pytest.mark.gpu
+
+OPT = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+GRAD_RTOL = 1e-3
+
+
+def _grid(sc, dev, dtype=torch.float32):
+    g = scenes.to_sparse_grid(sc, device=dev, dtype=dtype)
+    g.opt.sigma_thresh = OPT.sigma_thresh
+    g.opt.stop_thresh = OPT.stop_thresh
+    return g
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _fused(grid, o, d, gt, dev, **kw):
+    rays = svox2_amd.Rays(o.to(dev), d.to(dev))
+    rgb = grid.volume_render_fused(rays, gt.to(dev), **kw)
+    return (rgb.cpu().numpy(), grid.density_data.grad.cpu().numpy().reshape(-1),
+            grid.sh_data.grad.cpu().numpy(), grid.sparse_grad_indexer.cpu().numpy())
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_fused_dense64_vs_fp64(gpu, dense64, march, dtype):
+    _, o, d, gt, sc = dense64
+    grid = _grid(sc, gpu, dtype)
+    g = host_grid(sc, dtype)
+    rgb, gd, gs, mask = _fused(grid, torch.from_numpy(o), torch.from_numpy(d),
+                               torch.from_numpy(gt), gpu)
+    smp = march.samples(g, o, d, OPT)
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o, d, OPT, gt, samples=smp)
+    e = (np.abs(rgb - rgb64).max(), _rel(gd, gd64), _rel(gs, gs64))
+    print(f"dense64 fused {dtype}: colour {e[0]:.3e}, grad_density rel {e[1]:.3e}, "
+          f"grad_sh rel {e[2]:.3e}")
+    assert e[0] < 2e-5 and e[1] < GRAD_RTOL and e[2] < GRAD_RTOL
+    assert np.array_equal(mask, O.touched_rows(g, smp))
+
+
+def test_fused_beta_and_sparsity(gpu, dense64, march):
+    _, o, d, gt, sc = dense64
+    grid = _grid(sc, gpu)
+    g = host_grid(sc)
+    kw = dict(beta_loss=0.3, sparsity_loss=0.05)
+    rgb, gd, gs, _ = _fused(grid, torch.from_numpy(o[:1024]), torch.from_numpy(d[:1024]),
+                            torch.from_numpy(gt[:1024]), gpu, **kw)
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:1024], d[:1024], OPT, gt[:1024], **kw)
+    assert np.abs(rgb - rgb64).max() < 2e-5
+    assert _rel(gd, gd64) < GRAD_RTOL and _rel(gs, gs64) < GRAD_RTOL
+
+
+def test_autograd_backward_matches_oracle(gpu, dense64, march):
+    _, o, d, gt, sc = dense64
+    grid = _grid(sc, gpu)
+    g = host_grid(sc)
+    n = 2048
+    rays = svox2_amd.Rays(torch.from_numpy(o[:n]).to(gpu), torch.from_numpy(d[:n]).to(gpu))
+    rgb = grid.volume_render(rays)
+    loss = ((rgb - torch.from_numpy(gt[:n]).to(gpu)) ** 2).mean()
+    loss.backward()
+    gd = grid.density_data.grad.cpu().numpy().reshape(-1)
+    gs = grid.sh_data.grad.cpu().numpy()
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:n], d[:n], OPT, gt[:n])
+    assert np.abs(rgb.detach().cpu().numpy() - rgb64).max() < 2e-5
+    assert _rel(gd, gd64) < GRAD_RTOL and _rel(gs, gs64) < GRAD_RTOL
+
+
+def test_finite_differences_with_ray_clip(gpu, march):
+    """Per-ray clipped rays on a 64^3 instance: the GPU gradient (autograd
+    Function, transmittance recomputed in the backward) against FP64 central
+    differences (h = 1e-4) of the oracle loss on a random 1% of rows, along
+    random directions supported on that subset and per element."""
+    gen = torch.Generator().manual_seed(11)
+    sc = scenes.make_blob_grid(64, scenes.DENSE64, gen, dense=True)
+    o, d = scenes.sphere_rays(2048, gen)
+    gt = torch.rand(2048, 3, generator=gen, dtype=torch.float64).float()
+    clip = scenes.config_clip(2048, gen) * 0.6
+    grid = _grid(sc, gpu)
+    rays = svox2_amd.Rays(o.to(gpu), d.to(gpu), clip.to(gpu))
+    loss = ((grid.volume_render(rays) - gt.to(gpu)) ** 2).mean()
+    loss.backward()
+    gd = grid.density_data.grad.cpu().double().numpy().reshape(-1)
+    gs = grid.sh_data.grad.cpu().double().numpy()
+    g = host_grid(sc)
+    smp = march.samples(g, o.numpy(), d.numpy(), OPT, clip=clip.numpy())
+    R = g.density.shape[0]
+    rng = np.random.default_rng(0)
+    subset = rng.choice(R, R // 100, replace=False)
+
+    def L(dens, sh):
+        return float(((O.render_fp64(g, o.numpy(), d.numpy(), OPT, torch.from_numpy(dens),
+                                     torch.from_numpy(sh), samples=smp)
+                       - gt.double()) ** 2).mean())
+
+    h = 1e-4
+    for k in range(4):
+        vd = np.zeros(R)
+        vd[subset] = rng.standard_normal(subset.size)
+        vs = np.zeros_like(g.sh)
+        vs[subset] = rng.standard_normal((subset.size, g.sh.shape[1]))
+        fd = (L(g.density + h * vd, g.sh + h * vs) - L(g.density - h * vd, g.sh - h * vs)) / (2 * h)
+        an = float(gd @ vd + (gs * vs).sum())
+        assert an == pytest.approx(fd, rel=2e-3, abs=1e-9), k
+    top = subset[np.argsort(-np.abs(gd[subset]))[:4]]
+    for r in top:
+        e = np.zeros(R)
+        e[r] = 1
+        fd = (L(g.density + h * e, g.sh) - L(g.density - h * e, g.sh)) / (2 * h)
+        assert gd[r] == pytest.approx(fd, rel=2e-3, abs=1e-9)
+
+
+def test_training_step_facade(gpu):
+    """One training step of the reference loop (opt/opt.py:492-603): fused
+    render, TV on density and SH, RMSprop on both; gradients are zeroed and only
+    touched rows move."""
+    gen = torch.Generator().manual_seed(2)
+    sc = scenes.make_blob_grid(64, scenes.SPARSE256, gen, device=gpu)
+    cams = scenes.CameraSet(scenes.hemisphere_cameras(10, gen))
+    o, d = cams.random_batch(5000, gen)
+    gt = torch.rand(5000, 3, generator=gen)
+    grid = _grid(sc, gpu)
+    before_d = grid.density_data.data.clone()
+    before_s = grid.sh_data.data.clone()
+    rays = svox2_amd.Rays(o.to(gpu), d.to(gpu))
+    grid.volume_render_fused(rays, gt.to(gpu))
+    np.random.seed(0)
+    grid.inplace_tv_grad(grid.density_data.grad, scaling=1e-5, sparse_frac=0.01)
+    grid.inplace_tv_color_grad(grid.sh_data.grad, scaling=1e-3, sparse_frac=0.01)
+    touched = grid.sparse_grad_indexer.clone()
+    grid.optim_density_step(30.0, beta=0.95)
+    grid.optim_sh_step(1e-2, beta=0.95)
+    assert torch.count_nonzero(grid.density_data.grad) == 0
+    assert torch.count_nonzero(grid.sh_data.grad) == 0
+    moved = (grid.density_data.data != before_d).flatten()
+    assert moved.any() and not (moved & ~touched).any()
+    moved_s = (grid.sh_data.data != before_s).any(1)
+    assert moved_s.any() and not (moved_s & ~grid.sparse_sh_grad_indexer).any()

@@ -1,0 +1,75 @@
+"""The 512^3 (about 3% occupied, SH-2) grid of the benchmark: bit-exact sample
+indexing for a full 65,536-ray batch, colours and gradients against the FP64
+oracle on a 2,048-ray subset, for float32 and bfloat16 storage.  Tolerances as
+in the smaller tests (colours 2e-5 vs the oracle on the stored values, 2e-3
+including BF16 storage rounding; gradients 1e-3 of their max)."""
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.oracle as O
+import svox2_amd
+from svox2_amd import _C, scenes
+from tests.conftest import host_grid
+
+pytestmark = pytest.mark.gpu
+
+OPT = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+
+
+@pytest.fixture(scope="module")
+def big(gpu):
+    t = time.time()
+    sc, cams, gen = scenes.workload_blobs(512, scenes.SPARSE512, seed=0, device=gpu)
+    grid = scenes.to_sparse_grid(sc, device=gpu)
+    grid.opt.sigma_thresh, grid.opt.stop_thresh = OPT.sigma_thresh, OPT.stop_thresh
+    links = grid.links.cpu().numpy()
+    o, d = cams.random_batch(65536, gen)
+    gt = torch.rand(65536, 3, generator=gen)
+    print(f"512^3 grid: {sc.density.shape[0]} rows ({sc.occupancy:.2%}), built in "
+          f"{time.time() - t:.1f}s")
+    return sc, grid, links, o, d, gt
+
+
+def test_512_stats_bit_exact(gpu, big, march):
+    sc, grid, links, o, d, gt = big
+    rays = svox2_amd.Rays(o.to(gpu), d.to(gpu))
+    st = _C.volume_render_cuvol_stats(grid._to_cpp(), rays._to_cpp(),
+                                      grid.opt._to_cpp()).cpu().numpy()
+    g = host_grid(sc, links=links)
+    t = time.time()
+    _, _, st_ref = march.render(g, o.numpy(), d.numpy(), OPT)
+    print(f"C restatement: {time.time() - t:.1f}s; I={st[:, 0].sum()} E={st[:, 1].sum()} "
+          f"H={st[:, 2].sum()}")
+    assert np.array_equal(st, st_ref)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_512_colours_and_grads(gpu, big, march, dtype):
+    sc, grid32, links, o, d, gt = big
+    grid = grid32 if dtype == torch.float32 else scenes.to_sparse_grid(sc, device=gpu,
+                                                                       dtype=dtype)
+    grid.opt.sigma_thresh, grid.opt.stop_thresh = OPT.sigma_thresh, OPT.stop_thresh
+    n = 2048
+    rays = svox2_amd.Rays(o[:n].to(gpu), d[:n].to(gpu))
+    grid.density_data.grad = None
+    grid.sh_data.grad = None
+    rgb = grid.volume_render_fused(rays, gt[:n].to(gpu)).cpu().numpy()
+    gd = grid.density_data.grad.cpu().numpy().reshape(-1)
+    gs = grid.sh_data.grad.cpu().numpy()
+    g = host_grid(sc, dtype, links=links)
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:n].numpy(), d[:n].numpy(), OPT, gt[:n].numpy())
+    ed = np.abs(gd - gd64).max() / np.abs(gd64).max()
+    es = np.abs(gs - gs64).max() / np.abs(gs64).max()
+    ec = np.abs(rgb - rgb64).max()
+    print(f"512^3 {dtype}: colour {ec:.3e}, grad_density rel {ed:.3e}, grad_sh rel {es:.3e}")
+    assert ec < 2e-5 and ed < 1e-3 and es < 1e-3
+    if dtype == torch.bfloat16:
+        g32 = host_grid(sc, torch.float64, links=links)
+        rgb_s = O.render_fp64(g32, o[:n].numpy(), d[:n].numpy(), OPT,
+                              torch.from_numpy(g32.density), torch.from_numpy(g32.sh)).numpy()
+        es_ = np.abs(rgb - rgb_s).max()
+        print(f"512^3 bf16 colour error including storage rounding: {es_:.3e}")
+        assert es_ < 2e-3

@@ -1,0 +1,172 @@
+"""Pins of the oracle's small restatements against closed forms, library
+routines and brute force: SH basis, Morton codes, trilinear sampling,
+accel_dist_prop, RMSprop."""
+import math
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+import oracle.oracle as O
+from svox2_amd import utils as U
+
+
+# --------------------------------------------------------------------------- SH
+def test_sh_constants_closed_form():
+    pi = math.pi
+    assert O.SH_C0 == pytest.approx(1 / (2 * math.sqrt(pi)), abs=1e-15)
+    assert O.SH_C1 == pytest.approx(math.sqrt(3) / (2 * math.sqrt(pi)), abs=1e-15)
+    c2 = [math.sqrt(15) / (2 * math.sqrt(pi)), -math.sqrt(15) / (2 * math.sqrt(pi)),
+          math.sqrt(5) / (4 * math.sqrt(pi)), -math.sqrt(15) / (2 * math.sqrt(pi)),
+          math.sqrt(15) / (4 * math.sqrt(pi))]
+    for a, b in zip(O.SH_C2, c2):
+        assert a == pytest.approx(b, abs=1e-15)
+
+
+def _sphere_quadrature(n_theta=16, n_phi=32):
+    """Gauss-Legendre in cos(theta) x uniform phi: exact for polynomials of
+    degree < 2*n_theta on the sphere."""
+    x, w = np.polynomial.legendre.leggauss(n_theta)
+    phi = (np.arange(n_phi) + 0.5) * 2 * np.pi / n_phi
+    ct, ph = np.meshgrid(x, phi, indexing="ij")
+    st = np.sqrt(1 - ct ** 2)
+    dirs = np.stack([st * np.cos(ph), st * np.sin(ph), ct], -1).reshape(-1, 3)
+    wts = (w[:, None] * np.full(n_phi, 2 * np.pi / n_phi)[None, :]).reshape(-1)
+    return torch.from_numpy(dirs), torch.from_numpy(wts)
+
+
+def test_sh_orthonormal():
+    dirs, w = _sphere_quadrature()
+    Y = O.eval_sh_bases(9, dirs)
+    G = (Y * w[:, None]).T @ Y
+    assert torch.allclose(G, torch.eye(9, dtype=torch.float64), atol=1e-12)
+
+
+def test_sh_specific_values():
+    d = torch.tensor([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], dtype=torch.float64)
+    Y = O.eval_sh_bases(9, d)
+    # Y_2^0 = sqrt(5/(16 pi)) (3 z^2 - 1) appears with the reference's sign/normalisation
+    assert Y[0, 6] == pytest.approx(math.sqrt(5 / (16 * math.pi)) * 2, abs=1e-14)
+    assert Y[1, 6] == pytest.approx(-math.sqrt(5 / (16 * math.pi)), abs=1e-14)
+    assert Y[0, 2] == pytest.approx(O.SH_C1, abs=1e-15)
+    assert Y[1, 3] == pytest.approx(-O.SH_C1, abs=1e-15)
+    assert Y[2, 1] == pytest.approx(-O.SH_C1, abs=1e-15)
+    assert Y[1, 8] == pytest.approx(O.SH_C2[4], abs=1e-15)
+
+
+@pytest.mark.parametrize("bd", [1, 4, 9])
+def test_sh_product_matches_oracle(bd):
+    g = torch.Generator().manual_seed(1)
+    d = torch.randn(100, 3, generator=g, dtype=torch.float64)
+    d /= d.norm(dim=1, keepdim=True)
+    assert torch.allclose(U.eval_sh_bases(bd, d), O.eval_sh_bases(bd, d), atol=1e-15)
+
+
+# ----------------------------------------------------------------------- Morton
+def test_morton_bijective_and_matches():
+    D = 16
+    ref = O.gen_morton(D)
+    assert sorted(ref.flatten().tolist()) == list(range(D ** 3))
+    prod = U.gen_morton(D).numpy()
+    assert np.array_equal(ref, prod)
+    for x, y, z in [(0, 0, 1), (0, 1, 0), (1, 0, 0), (3, 5, 7), (1023, 0, 511)]:
+        c = O.morton_code_3(x, y, z)
+        assert int(U.morton_code_3(torch.tensor(x), torch.tensor(y), torch.tensor(z))) == c
+        X, Y, Z = U.inv_morton_code_3(torch.tensor(c))
+        assert (int(X), int(Y), int(Z)) == (x, y, z)
+    assert O.morton_code_3(1, 0, 0) == 4 and O.morton_code_3(0, 1, 0) == 2
+
+
+# ---------------------------------------------------------------------- trilerp
+def _dense_grid(N, C, seed):
+    g = torch.Generator().manual_seed(seed)
+    links = np.arange(N ** 3, dtype=np.int32).reshape(N, N, N)
+    dens = torch.randn(N ** 3, generator=g, dtype=torch.float64)
+    sh = torch.randn(N ** 3, C, generator=g, dtype=torch.float64)
+    return links, dens, sh
+
+
+def test_trilerp_matches_grid_sample():
+    N, C = 9, 4
+    links, dens, sh = _dense_grid(N, C, 0)
+    g = torch.Generator().manual_seed(3)
+    world = torch.rand(500, 3, generator=g, dtype=torch.float64) * 2.4 - 1.2  # includes border
+    grid_pts = (world * (N / 2) + N / 2 - 0.5).numpy()
+    d, s = O.sample_trilerp(links, dens, sh, grid_pts)
+    vol = torch.cat([dens.view(N, N, N, 1), sh.view(N, N, N, C)], -1).permute(3, 0, 1, 2)[None]
+    ref = F.grid_sample(vol, world[:, [2, 1, 0]].view(1, -1, 1, 1, 3), mode="bilinear",
+                        padding_mode="border", align_corners=False)
+    ref = ref.view(C + 1, -1).T
+    assert torch.allclose(d[:, 0], ref[:, 0], atol=1e-12)
+    assert torch.allclose(s, ref[:, 1:], atol=1e-12)
+
+
+def test_trilerp_exact_on_linear_field_and_empty_corners():
+    N = 6
+    links = np.arange(N ** 3, dtype=np.int32).reshape(N, N, N)
+    X, Y, Z = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    lin = torch.from_numpy((0.5 + 2 * X - 3 * Y + 0.25 * Z).reshape(-1).astype(np.float64))
+    pts = np.random.default_rng(0).uniform(0, N - 1, size=(200, 3))
+    d, _ = O.sample_trilerp(links, lin, lin[:, None].clone(), pts)
+    expect = 0.5 + 2 * pts[:, 0] - 3 * pts[:, 1] + 0.25 * pts[:, 2]
+    assert np.allclose(d[:, 0].numpy(), expect, atol=1e-12)
+    # an empty corner reads 0: at a corner's own position the sample is 0
+    links2 = links.copy()
+    links2[2, 3, 4] = -1
+    d2, _ = O.sample_trilerp(links2, lin, lin[:, None].clone(), np.array([[2.0, 3.0, 4.0]]))
+    assert float(d2[0, 0]) == 0.0
+
+
+# ------------------------------------------------------------------ accel_dist
+def _accel_brute(links):
+    """Per cell: -1 - (number of consecutive levels i >= 1 whose aligned block
+    of 2^i cells around the cell is empty), by slicing."""
+    occ = links >= 0
+    S = occ.shape
+    out = links.copy()
+    nlev = 0
+    s = list(S)
+    while all(v > 1 for v in s):
+        s = [(v + 1) // 2 for v in s]
+        nlev += 1
+    for x, y, z in zip(*np.nonzero(~occ)):
+        k = 0
+        for i in range(1, nlev + 1):
+            b = 1 << i
+            x0, y0, z0 = (x >> i) << i, (y >> i) << i, (z >> i) << i
+            if occ[x0:x0 + b, y0:y0 + b, z0:z0 + b].any():
+                break
+            k += 1
+        out[x, y, z] = -1 - k
+    return out
+
+
+@pytest.mark.parametrize("shape,frac,seed", [((12, 10, 9), 0.05, 0), ((16, 16, 16), 0.01, 1),
+                                             ((7, 5, 3), 0.2, 2), ((8, 8, 8), 0.0, 3)])
+def test_accel_dist_prop_brute_force(shape, frac, seed):
+    rng = np.random.default_rng(seed)
+    occ = rng.random(shape) < frac
+    links = np.where(occ, np.arange(np.prod(shape)).reshape(shape), -1).astype(np.int32)
+    a = O.accel_dist_prop(links)
+    b = _accel_brute(links)
+    assert np.array_equal(a, b)
+    # idempotent on already-coded links
+    assert np.array_equal(O.accel_dist_prop(a), a)
+
+
+# ---------------------------------------------------------------------- rmsprop
+def test_rmsprop_closed_form():
+    data = np.array([[1.0, 2.0], [3.0, -4.0], [0.5, 0.5]])
+    rms = np.array([[0.0, 0.25], [1.0, 0.0], [9.0, 9.0]])
+    grad = np.array([[0.5, -1.0], [2.0, 0.1], [3.0, 3.0]])
+    d, r, g = O.rmsprop_step(data, rms, grad, rows_sel=[0, 1], beta=0.9, lr=0.1, eps=1e-8,
+                             minval=-3.5, lr_last=0.2)
+    # row 0 col 0: rms==0 -> g^2 = 0.25; x = 1 - 0.1*0.5/(0.5+eps)
+    assert r[0, 0] == 0.25 and d[0, 0] == pytest.approx(1 - 0.1 * 0.5 / (0.5 + 1e-8))
+    # row 0 col 1 (last channel uses lr_last): rms = 0.9*0.25 + 0.1*1
+    assert r[0, 1] == pytest.approx(0.325)
+    assert d[0, 1] == pytest.approx(2 + 0.2 / (math.sqrt(0.325) + 1e-8))
+    # clamp to minval
+    assert d[1, 1] == pytest.approx(max(-4 - 0.2 * 0.1 / (0.1 + 1e-8), -3.5))
+    assert (g[:2] == 0).all() and (g[2] == grad[2]).all() and (d[2] == data[2]).all()
