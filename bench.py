@@ -8,9 +8,9 @@ camera rays already resident in HBM.  Prints one JSON line with the
 throughput, a roofline entry for the fused kernel (algorithmic bytes per
 launch / average launch time from HIP events on the launch stream, against the
 8 TB/s HBM peak), side measurements (BF16 storage, 5,000-ray batches, an
-800x800 forward image) and a CPU baseline: the reference's pure-PyTorch
-renderer with autograd backward (restated in oracle/) timed on this host on a
-bounded sample of the same batch.
+800x800 forward image) and a CPU baseline: the scalar C port of the
+reference's fused forward+backward (oracle/march_ref.c) timed on one host core
+on a bounded sample of the same batch.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -174,39 +174,37 @@ def image_forward(grid, cams, reps=5):
     return dict(ms=ms, rays_per_s=cam.width * cam.height / (ms * 1e-3))
 
 
-def cpu_baseline(wl: Workload, budget_s: float = 20.0):
-    """The reference's pure-PyTorch renderer + autograd MSE backward (float32,
-    restated in oracle/oracle.py) on this host, on the first rays of the first
-    benchmark batch."""
+def cpu_baseline(wl: Workload, budget_s: float = 12.0):
+    """The scalar C port of the reference's fused forward + MSE backward
+    (oracle/march_ref.c: trace_ray_cuvol and trace_ray_cuvol_backward, one
+    thread, every contributing sample scattered into its 8 corner rows) on
+    this host, over the first rays of a benchmark batch of the same grid."""
     import oracle.oracle as O
-    threads = os.cpu_count() or 1
-    torch.set_num_threads(threads)
-    sc = wl.scene
+    O.build(force=False)
     grid = wl.grid(torch.float32)
-    links = grid.links.cpu()
-    dens = grid.density_data.data.cpu()
-    sh = grid.sh_data.data.cpu()
-    o, d = wl.cams.random_batch(4096, torch.Generator().manual_seed(123))
-    gt = torch.rand(4096, 3, generator=torch.Generator().manual_seed(124))
+    N = np.array(grid.links.shape, np.float64)
+    hg = O.Grid(grid.links.cpu().numpy(), grid.density_data.data.cpu().double().numpy().reshape(-1),
+                grid.sh_data.data.cpu().double().numpy(), 9, offset=0.5 * N - 0.5,
+                scaling=0.5 * N)
+    opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+    o, d = wl.cams.random_batch(BATCH, torch.Generator().manual_seed(123))
+    gt = torch.rand(BATCH, 3, generator=torch.Generator().manual_seed(124))
+    march = O.march()
 
     def run(n):
-        dd = dens.clone().requires_grad_(True)
-        ss = sh.clone().requires_grad_(True)
         t = time.perf_counter()
-        rgb = O.volume_render_gradcheck_lerp(links, dd, ss, 9, 1.0, 0.0, o[:n], d[:n],
-                                             step_size=0.5, dtype=torch.float32)
-        ((rgb - gt[:n]) ** 2).mean().backward()
+        march.fused(hg, o[:n].numpy(), d[:n].numpy(), opt, gt[:n].numpy())
         return time.perf_counter() - t
 
-    n = 64
-    dt = run(n)  # warm-up and size probe
-    n = int(min(4096, max(64, n * (budget_s / 4) / max(dt, 1e-3))))
-    times = [run(n) for _ in range(3)]
-    med = float(np.median(times))
-    return dict(value=n / med, unit="rays/s", cores=threads, kind="port",
-                sample=f"{n} rays of a 512^3 fused fwd+bwd batch (reference pure-PyTorch "
-                       f"renderer + autograd backward, float32), median of 3 runs "
-                       f"({med:.2f}s each)")
+    n = 256
+    dt = run(n)
+    n = int(min(BATCH, max(256, n * budget_s / max(dt, 1e-3))))
+    dt = run(n)
+    log(f"CPU baseline: {n} rays in {dt:.2f}s")
+    return dict(value=n / dt, unit="rays/s", cores=1, kind="port",
+                sample=f"first {n} rays of a 65536-ray batch on the same 512^3 grid: scalar C "
+                       f"port of the reference fused forward+backward (one thread, float32), "
+                       f"{dt:.2f}s")
 
 
 def main():
@@ -235,20 +233,26 @@ def main():
     log("fp32 65536-ray fused steps")
     head = measure(g32, batches, args.steps, args.warmup, 4, 4)
     log(f"fp32: {head['rays_per_s']:.4g} rays/s, kernel {head['kernel_ms']:.3f} ms, "
-        f"{head['alg_gbps']:.1f} GB/s algorithmic")
+        f"{head['alg_gbps']:.1f} GB/s algorithmic, counters {head['counters']}")
     extra = {}
     if not args.quick:
         gb = wl.grid(torch.bfloat16)
         log("bf16 65536-ray fused steps")
         extra["bf16_65536"] = measure(gb, batches, args.steps, args.warmup, 2, 2)
         extra["bf16_speedup"] = extra["bf16_65536"]["rays_per_s"] / head["rays_per_s"]
+        log(f"bf16: {extra['bf16_65536']['rays_per_s']:.4g} rays/s "
+            f"({extra['bf16_speedup']:.2f}x fp32)")
         small = wl.batches(5000, POOL)
         log("5000-ray fused steps")
         extra["fp32_5000"] = measure(g32, small, args.steps, args.warmup, 4, 4)
         extra["bf16_5000"] = measure(gb, small, args.steps, args.warmup, 2, 2)
+        log(f"5000-ray batches: fp32 {extra['fp32_5000']['rays_per_s']:.4g}, bf16 "
+            f"{extra['bf16_5000']['rays_per_s']:.4g} rays/s")
         log("800x800 image forward")
         extra["image_800_fp32"] = image_forward(g32, wl.cams)
         extra["image_800_bf16"] = image_forward(gb, wl.cams)
+        log(f"800x800 image: fp32 {extra['image_800_fp32']['ms']:.3f} ms, bf16 "
+            f"{extra['image_800_bf16']['ms']:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline")

@@ -22,7 +22,13 @@
  *                (iterations, evaluated, contributing, FNV-1a hash) counters
  *                the GPU stats kernel produces;
  *   ref_samples  every evaluated (not skipped) sample of every ray, ignoring
- *                the stop test, for the FP64 oracle to integrate.
+ *                the stop test, for the FP64 oracle to integrate;
+ *   ref_fused    forward colour plus the MSE gradient (render_ray_backward_kernel
+ *                with grad_out_is_rgb, render_lerp_kernel_cuvol.cu:692-773, and
+ *                trace_ray_cuvol_backward :227-383), scattering every
+ *                contributing sample into the 8 corner rows as the reference
+ *                does (render_util.cuh:93-153): the scalar CPU port timed as
+ *                the CPU baseline.
  */
 #include <math.h>
 #include <stdint.h>
@@ -370,6 +376,94 @@ int ref_camera_rays(const float* c2w, float fx, float fy, float cx, float cy, in
             origins[3 * i + 0] = c2w[3];
             origins[3 * i + 1] = c2w[7];
             origins[3 * i + 2] = c2w[11];
+        }
+    }
+    return 0;
+}
+
+/* Fused forward + MSE backward of rays [0, n): rgb_out (n,3), gradients
+ * accumulated into grad_density (rows) / grad_sh (rows x C), mask (rows). */
+int ref_fused(const ref_grid_t* g, const float* origins, const float* dirs, const float* clip,
+              int64_t n, const ref_opt_t* o, const float* rgb_gt, float beta_loss,
+              float sparsity_loss, float* rgb_out, float* grad_density, float* grad_sh,
+              uint8_t* mask) {
+    const float log_stop = log_stop_threshold(o->stop_thresh);
+    const int B = g->basis_dim, C = 3 * B;
+    const float norm = 2.f / (float)(3 * n);
+    const float beta = beta_loss / (float)n;
+    for (int64_t i = 0; i < n; ++i) {
+        const float* wo = origins + 3 * i;
+        const float* wd = dirs + 3 * i;
+        const float* cl = clip ? clip + 2 * i : NULL;
+        float rgb[3], log_t_fwd;
+        render_one(g, o, log_stop, wo, wd, cl, rgb, &log_t_fwd, NULL);
+        memcpy(rgb_out + 3 * i, rgb, sizeof(rgb));
+        float gq[3];
+        for (int c = 0; c < 3; ++c) gq[c] = (rgb[c] - rgb_gt[3 * i + c]) * norm;
+        float accum = fmaf(rgb[0], gq[0], fmaf(rgb[1], gq[1], rgb[2] * gq[2]));
+        ray_t r;
+        setup_ray(g, o, wo, wd, cl, &r);
+        if (r.tmin > r.tmax) continue;
+        if (beta > 0.f) {
+            const float tin = expf(log_t_fwd);
+            accum += beta * (1.f - tin / (1.f - tin + 1e-3f));
+        }
+        float basis[9];
+        for (int j = 0; j < B; ++j) basis[j] = sh_basis(j, wd[0], wd[1], wd[2]);
+        float t = r.tmin, ws = r.ws, log_t = 0.f;
+        int guard = 0;
+        while (t <= r.tmax && ++guard < (1 << 22)) {
+            float p[3];
+            int l[3], lk[8];
+            const int base = sample_cell(g, &r, t, p, l);
+            const int link0 = g->links[base];
+            if (o->skip && link0 < -1) {
+                const float skip = skip_dist(link0, l, p, &r);
+                if (skip >= o->step) {
+                    t += ceilf(skip / o->step) * o->step;
+                    continue;
+                }
+            }
+            corner_links(g, base, lk);
+            float dn[8];
+            for (int k = 0; k < 8; ++k) dn[k] = valid_link(g, lk[k]) ? g->density[lk[k]] : 0.f;
+            const float sigma = trilerp8(dn, p);
+            if (o->last_opaque && t + o->step > r.tmax) ws = 1e9f;
+            if (sigma > o->sigma_thresh) {
+                float col[3] = {0.f, 0.f, 0.f};
+                for (int ch = 0; ch < C; ++ch) {
+                    float v[8];
+                    for (int k = 0; k < 8; ++k)
+                        v[k] = valid_link(g, lk[k]) ? g->sh[(int64_t)lk[k] * C + ch] : 0.f;
+                    col[ch / B] += trilerp8(v, p) * basis[ch % B];
+                }
+                const float pcnt = ws * sigma;
+                const float weight = expf(log_t) * (1.f - expf(-pcnt));
+                log_t -= pcnt;
+                float total = 0.f, gc[3];
+                for (int c = 0; c < 3; ++c) {
+                    const float x = col[c] + 0.5f;
+                    total += fmaxf(x, 0.f) * gq[c];
+                    gc[c] = weight * (x >= 0.f ? 1.f : 0.f) * gq[c];
+                }
+                accum -= weight * total;
+                float gs = ws * (total * expf(log_t) - accum);
+                if (sparsity_loss > 0.f)
+                    gs += sparsity_loss * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
+                const float ax = 1.f - p[0], ay = 1.f - p[1], az = 1.f - p[2];
+                for (int k = 0; k < 8; ++k) {
+                    if (!valid_link(g, lk[k])) continue;
+                    const float wk = ((k & 2) ? p[1] : ay) * ((k & 1) ? p[2] : az) *
+                                     ((k & 4) ? p[0] : ax);
+                    if (grad_density) grad_density[lk[k]] += wk * gs;
+                    if (mask) mask[lk[k]] = 1;
+                    if (grad_sh)
+                        for (int ch = 0; ch < C; ++ch)
+                            grad_sh[(int64_t)lk[k] * C + ch] += wk * basis[ch % B] * gc[ch / B];
+                }
+                if (log_t < log_stop) break;
+            }
+            t += o->step;
         }
     }
     return 0;

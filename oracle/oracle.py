@@ -134,6 +134,9 @@ class MarchRef:
         lib.ref_camera_rays.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_int32, ctypes.c_int32, vp, vp]
         lib.ref_camera_rays.restype = ctypes.c_int
+        lib.ref_fused.argtypes = [G, vp, vp, vp, i64, O, vp, ctypes.c_float, ctypes.c_float,
+                                  vp, vp, vp, vp]
+        lib.ref_fused.restype = ctypes.c_int
         self.lib = lib
 
     @staticmethod
@@ -216,6 +219,26 @@ class MarchRef:
         self.lib.ref_ray_setup(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), o.shape[0],
                                ctypes.byref(self._opt(opt)), _c_ptr(out))
         return out
+
+    def fused(self, g: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, beta_loss=0.0,
+              sparsity_loss=0.0):
+        """FP32 scalar port of the fused forward + MSE backward: rgb (n,3),
+        grad_density (R,), grad_sh (R,C), touched mask (R,)."""
+        keep: list = []
+        gs = self._grid(g, keep)
+        o, d, c = self._rays(origins, dirs, clip, keep)
+        gt = np.ascontiguousarray(rgb_gt, dtype=np.float32)
+        n = o.shape[0]
+        R = np.asarray(g.density).reshape(-1).shape[0]
+        C = 3 * g.basis_dim
+        rgb = np.zeros((n, 3), np.float32)
+        gd = np.zeros(R, np.float32)
+        gsh = np.zeros((R, C), np.float32)
+        m = np.zeros(R, np.uint8)
+        self.lib.ref_fused(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), n,
+                           ctypes.byref(self._opt(opt)), _c_ptr(gt), beta_loss, sparsity_loss,
+                           _c_ptr(rgb), _c_ptr(gd), _c_ptr(gsh), _c_ptr(m))
+        return rgb, gd, gsh, m.astype(bool)
 
     def camera_rays(self, c2w, fx, fy, cx, cy, width, height):
         m = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32)[:3, :4])

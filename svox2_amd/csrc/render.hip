@@ -2,17 +2,18 @@
 // for gfx950.
 //
 // Work mapping: one 32-lane half of a 64-wide wavefront per ray, two rays per
-// wavefront, lane c holding SH channel c (27 of 32 lanes busy for SH-2).  All
-// per-ray scalar state (sample position, cell, density, transmittance) is
-// computed redundantly and identically by the 32 lanes, so control flow stays
-// uniform inside a half; the only cross-lane traffic is the per-colour sum of
-// the 9 basis-weighted channels.
+// wavefront.  The march is sample-parallel inside the ray: a chunk puts up to
+// 32 consecutive samples on the 32 lanes, so the corner links and densities of
+// all of them are fetched by two waves of independent loads instead of one
+// dependent round trip per sample, and the transmittance chain is replayed in
+// registers.  Colours of the live samples are then gathered four samples at a
+// time with lane = (sample, corner): each lane reads one contiguous SH row and
+// projects it on the SH basis; the 8 corners are summed with lane shuffles.
 //
-// The eight corners of the current cell (links, densities and this lane's SH
-// channel) are held in registers and re-used by every sample that falls in the
-// same cell; the backward pass accumulates each corner's gradient in registers
-// and issues one no-return atomic per (corner, channel) when the ray leaves the
-// cell, instead of one per sample.
+// The backward pass rematerialises the same chunks and applies each live
+// sample in order with lane = SH channel, accumulating every corner's
+// gradient in registers while the ray stays in a cell and issuing one
+// coalesced no-return atomic per (corner, channel) when it leaves.
 //
 // Semantics follow the reference kernels: trace_ray_cuvol
 // (svox2/csrc/render_lerp_kernel_cuvol.cu:30-120), trace_ray_cuvol_backward
@@ -166,19 +167,6 @@ __device__ __forceinline__ float trilerp8(const float* v, const float* p) {
     return lerpf(ix0, ix1, p[0]);
 }
 
-// Sums each colour's B basis-weighted channels inside the 32-lane group and
-// broadcasts the three sums to every lane.  j = lane % B.
-template <int B>
-__device__ __forceinline__ void color_sums(float v, int j, float* col) {
-    if (B > 1) { const float u = __shfl_down(v, 1, kGroup); if (j + 1 < B) v += u; }
-    if (B > 2) { const float u = __shfl_down(v, 2, kGroup); if (j + 2 < B) v += u; }
-    if (B > 4) { const float u = __shfl_down(v, 4, kGroup); if (j + 4 < B) v += u; }
-    if (B > 8) { const float u = __shfl_down(v, 8, kGroup); if (j + 8 < B) v += u; }
-    col[0] = __shfl(v, 0, kGroup);
-    col[1] = __shfl(v, B, kGroup);
-    col[2] = __shfl(v, 2 * B, kGroup);
-}
-
 __device__ __forceinline__ float pick3(const float* a, int i) {
     return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
 }
@@ -188,65 +176,38 @@ struct Fnv {
     __device__ __forceinline__ void mix(uint32_t w) { h = (h ^ w) * 16777619u; }
 };
 
-// Per-ray lane context.
+// 32-lane group helpers.  A ray owns one half of the wavefront; group-local
+// lane ids are 0..31 and masks are the half's 32 ballot bits.
+__device__ __forceinline__ uint32_t group_ballot(bool pred) {
+    const unsigned long long m = __ballot(pred);
+    return static_cast<uint32_t>(m >> (threadIdx.x & 32u));
+}
+
+// Position (0-based) of the n-th set bit of m; m must have more than n bits set.
+__device__ __forceinline__ int nth_set_bit(uint32_t m, int n) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        const uint32_t lo = m & ((1u << w) - 1u);
+        const int c = __popc(lo);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = lo;
+        }
+    }
+    return pos;
+}
+
+// Per-ray constants every lane keeps: SH basis at the view direction.
 template <int B>
-struct Lane {
-    int lane;    // 0..31
-    int j;       // basis index lane % B
-    int color;   // lane / B (clamped to 2)
-    bool act;    // lane < 3B
-    int ch;      // channel loaded by this lane
-    float basis; // SH basis j at the view direction
-    __device__ __forceinline__ Lane(int lane_, const float* vdir) {
-        lane = lane_;
-        act = lane < 3 * B;
-        j = lane % B;
-        color = act ? lane / B : 2;
-        ch = act ? lane : 0;
-        basis = act ? sh_basis(j, vdir[0], vdir[1], vdir[2]) : 0.f;
-    }
-};
-
-// The register window: links, densities and this lane's SH channel at the
-// eight corners of cell (c0, c1, c2).
-template <typename DT, int B>
-struct Window {
-    int c0 = INT_MIN, c1 = 0, c2 = 0;
-    int lk[8];
-    float dn[8];
-    float sv[8];
-    bool full = false;  // links 1..7 and densities loaded
-    bool shv = false;   // SH channel loaded
-
-    __device__ __forceinline__ bool enter(const GridView<DT>& g, const int* l, int base) {
-        if (l[0] == c0 && l[1] == c1 && l[2] == c2) return false;
-        c0 = l[0]; c1 = l[1]; c2 = l[2];
-        lk[0] = g.links[base];
-        full = false;
-        shv = false;
-        return true;
-    }
-    __device__ __forceinline__ void load_density(const GridView<DT>& g, int base) {
-        if (full) return;
-        const int sx = g.stride_x, sy = g.stride_y;
-        lk[1] = g.links[base + 1];
-        lk[2] = g.links[base + sy];
-        lk[3] = g.links[base + sy + 1];
-        lk[4] = g.links[base + sx];
-        lk[5] = g.links[base + sx + 1];
-        lk[6] = g.links[base + sx + sy];
-        lk[7] = g.links[base + sx + sy + 1];
+struct RayCtx {
+    float basis[B];
+    __device__ __forceinline__ explicit RayCtx(const float* vdir) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) dn[k] = g.valid(lk[k]) ? ld(g.density, lk[k]) : 0.f;
-        full = true;
-    }
-    __device__ __forceinline__ void load_sh(const GridView<DT>& g, int ch) {
-        if (shv) return;
-        const int C = 3 * B;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            sv[k] = g.valid(lk[k]) ? ld(g.sh, static_cast<int64_t>(lk[k]) * C + ch) : 0.f;
-        shv = true;
+        for (int j = 0; j < B; ++j) basis[j] = sh_basis(j, vdir[0], vdir[1], vdir[2]);
     }
 };
 
@@ -265,67 +226,238 @@ __device__ __forceinline__ int sample_cell(const GridView<DT>& g, const Ray& r, 
     return l[0] * g.stride_x + l[1] * g.stride_y + l[2];
 }
 
+// Trilinear weight of corner k (bits dx dy dz) at fractional offset p.
+__device__ __forceinline__ float corner_weight(int k, const float* p) {
+    const float wx = (k & 4) ? p[0] : 1.f - p[0];
+    const float wy = (k & 2) ? p[1] : 1.f - p[1];
+    const float wz = (k & 1) ? p[2] : 1.f - p[2];
+    return wy * wz * wx;
+}
+
+// One chunk of up to 32 consecutive samples of a ray, one sample per lane.
+//
+// Lane i holds the sample at the i-th step from the chunk start t0, with t
+// advanced by the same sequence of float additions the sequential march
+// performs, so positions, cells, skips and densities are those of the
+// reference march (render_lerp_kernel_cuvol.cu:54-86).  The chunk ends at the
+// first lane whose cell is skipped (that lane's skip sets the next chunk start)
+// or at tmax.  The log transmittance before each sample is replayed in sample
+// order with the same float subtractions, so the stop decision (:103-106) and
+// every later sample agree bit for bit with the sequential march.
+struct Chunk {
+    float t;          // this lane's sample t
+    float p[3];       // fractional position in the cell
+    int l[3];         // lower cell corner
+    int base;         // flat index of l
+    int lk[8];        // corner links
+    float sigma;      // trilinear density
+    float ws;         // world step of this sample (1e9 for a last-opaque sample)
+    float lt_before;  // log transmittance before the sample
+    float lt_after;   // log transmittance after the sample
+    float weight;     // compositing weight T (1 - exp(-ws sigma)) of a live sample
+    bool contrib;     // evaluated and sigma > sigma_thresh
+    bool live;        // contributes and lies at or before the stop sample
+    uint32_t live_mask;
+    int n_eval;       // evaluated samples (before the first skip / tmax / stop)
+    int n_iter;       // march iterations consumed (evaluated + one per skip)
+    bool stopped;     // the stop test fired in this chunk
+    bool more;        // the march continues after this chunk
+    float next_t;     // start of the next chunk
+    float log_t_end;  // log transmittance after the chunk
+};
+
+template <typename DT>
+__device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView& o,
+                                            const Ray& r, int lane, float t0, float log_t,
+                                            Chunk& c) {
+    // t of the lane-th step, by sequential float additions.
+    float ti = t0;
+#pragma unroll
+    for (int k = 1; k < 32; ++k) {
+        const float nx = ti + o.step;
+        ti = (k <= lane) ? nx : ti;
+    }
+    c.t = ti;
+    const bool in_range = ti <= r.tmax;
+    c.base = sample_cell(g, r, ti, c.p, c.l);
+    const int sx = g.stride_x, sy = g.stride_y;
+    if (in_range) {
+        c.lk[0] = g.links[c.base];
+        c.lk[1] = g.links[c.base + 1];
+        c.lk[2] = g.links[c.base + sy];
+        c.lk[3] = g.links[c.base + sy + 1];
+        c.lk[4] = g.links[c.base + sx];
+        c.lk[5] = g.links[c.base + sx + 1];
+        c.lk[6] = g.links[c.base + sx + sy];
+        c.lk[7] = g.links[c.base + sx + sy + 1];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c.lk[k] = -1;
+    }
+    float skip = 0.f;
+    bool is_skip = false;
+    if (in_range && c.lk[0] < -1) {
+        skip = skip_dist(c.lk[0], c.l, c.p, r);
+        is_skip = skip >= o.step;
+    }
+    const uint32_t skip_mask = group_ballot(is_skip);
+    const int n_range = __popc(group_ballot(in_range));
+    const int first_skip = skip_mask ? (__ffs(skip_mask) - 1) : 32;
+    const int n_eval = min(first_skip, n_range);
+    const bool eval = lane < n_eval;
+
+    float dn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dn[k] = (eval && g.valid(c.lk[k])) ? ld(g.density, c.lk[k]) : 0.f;
+    c.sigma = trilerp8(dn, c.p);
+    c.ws = (o.last_opaque && ti + o.step > r.tmax) ? 1e9f : r.ws;
+    c.contrib = eval && c.sigma > o.sigma_thresh;
+    const float pcnt = c.contrib ? c.ws * c.sigma : 0.f;
+
+    // Log transmittance before this lane's sample, subtracting the earlier
+    // samples' optical depths one by one in sample order.
+    float lt = log_t;
+    for (int j = 0; j < 31; ++j) {
+        const float pj = __shfl(pcnt, j, kGroup);
+        if (j < lane) lt -= pj;
+    }
+    c.lt_before = lt;
+    c.lt_after = lt - pcnt;
+    const uint32_t stop_mask = group_ballot(c.contrib && c.lt_after < o.log_stop);
+    const int first_stop = stop_mask ? (__ffs(stop_mask) - 1) : 32;
+    c.stopped = first_stop < 32;
+    c.live = c.contrib && lane <= first_stop;
+    c.live_mask = group_ballot(c.live);
+    c.weight = c.live ? __expf(c.lt_before) * (1.f - __expf(-pcnt)) : 0.f;
+
+    if (c.stopped) {
+        c.n_eval = first_stop + 1;
+        c.n_iter = first_stop + 1;
+        c.more = false;
+        c.log_t_end = -1e3f;
+        c.next_t = 0.f;
+        return;
+    }
+    c.n_eval = n_eval;
+    c.log_t_end = n_eval > 0 ? __shfl(c.lt_after, n_eval - 1, kGroup) : log_t;
+    if (first_skip < n_range) {
+        c.n_iter = n_eval + 1;
+        const float ts = __shfl(ti, first_skip, kGroup);
+        const float sk = __shfl(skip, first_skip, kGroup);
+        c.next_t = ts + ceilf(sk / o.step) * o.step;
+        c.more = true;
+    } else {
+        c.n_iter = n_eval;
+        c.more = n_range == 32;
+        c.next_t = __shfl(ti, 31, kGroup) + o.step;
+    }
+}
+
+// Colours of up to four live samples of a chunk at once: lane = (sample s, corner
+// k) with s = lane / 8.  Each lane reads its corner's SH row, projects it on
+// the SH basis and weights it trilinearly; the eight corners are summed within
+// the lane's group of 8.  Returns the raw colours (before +0.5 and clamping) of
+// sample pass*4 + lane/8 (identical in its 8 lanes), and that sample's source
+// lane in `src`.
+template <typename DT, int B>
+__device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx<B>& rc,
+                                             const Chunk& c, int lane, int pass, int n_live,
+                                             float* col, int& src, bool& has) {
+    const int q = pass * 4 + (lane >> 3);
+    const int k = lane & 7;
+    has = q < n_live;
+    src = has ? nth_set_bit(c.live_mask, q) : 0;
+    int lk = __shfl(c.lk[0], src, kGroup);
+#pragma unroll
+    for (int kk = 1; kk < 8; ++kk) {
+        const int v = __shfl(c.lk[kk], src, kGroup);
+        lk = (k == kk) ? v : lk;
+    }
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = __shfl(c.p[a], src, kGroup);
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+    if (has && g.valid(lk)) {
+        constexpr int C = 3 * B;
+        const DT* row = g.sh + static_cast<int64_t>(lk) * C;
+        float x[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) x[ch] = ld(row, ch);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            v0 = fmaf(rc.basis[j], x[j], v0);
+            v1 = fmaf(rc.basis[j], x[B + j], v1);
+            v2 = fmaf(rc.basis[j], x[2 * B + j], v2);
+        }
+        const float w = corner_weight(k, p);
+        v0 *= w;
+        v1 *= w;
+        v2 *= w;
+    }
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        v0 += __shfl_xor(v0, m, kGroup);
+        v1 += __shfl_xor(v1, m, kGroup);
+        v2 += __shfl_xor(v2, m, kGroup);
+    }
+    col[0] = v0;
+    col[1] = v1;
+    col[2] = v2;
+}
+
 // Forward march of one ray.  out[3] = composited colour (all lanes),
 // returns the final log transmittance.
 template <typename DT, int B, bool STATS>
 __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptView& o,
-                                               const Ray& r, const Lane<B>& ln,
+                                               const Ray& r, const RayCtx<B>& rc, int lane,
                                                float* out, int32_t* stats) {
-    out[0] = out[1] = out[2] = 0.f;
+    float acc[3] = {0.f, 0.f, 0.f};  // this lane's share of the composited colour
     uint32_t n_iter = 0, n_eval = 0, n_hit = 0;
     Fnv fnv;
     float log_t = 0.f;
     if (r.tmin <= r.tmax) {
-        Window<DT, B> w;
         float t = r.tmin;
-        float ws = r.ws;
         int guard = 0;
-        while (t <= r.tmax && ++guard < kMaxIters) {
-            if (STATS) ++n_iter;
-            float p[3];
-            int l[3];
-            const int base = sample_cell(g, r, t, p, l);
-            w.enter(g, l, base);
-            if (w.lk[0] < -1) {
-                const float skip = skip_dist(w.lk[0], l, p, r);
-                if (skip >= o.step) {
-                    t += ceilf(skip / o.step) * o.step;
-                    continue;
+        bool more = true;
+        while (more && ++guard < kMaxIters) {
+            Chunk c;
+            march_chunk(g, o, r, lane, t, log_t, c);
+            const int n_live = __popc(c.live_mask);
+            for (int pass = 0; pass * 4 < n_live; ++pass) {
+                float col[3];
+                int src;
+                bool has;
+                chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
+                const float w = __shfl(c.weight, src, kGroup);
+                if (has && (lane & 7) == 0) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
                 }
             }
             if (STATS) {
-                ++n_eval;
-                fnv.mix(static_cast<uint32_t>(base));
-                fnv.mix(__float_as_uint(t));
-            }
-            w.load_density(g, base);
-            const float sigma = trilerp8(w.dn, p);
-            if (o.last_opaque && t + o.step > r.tmax) ws = 1e9f;
-            if (sigma > o.sigma_thresh) {
-                if (STATS) { ++n_hit; fnv.mix(1u); }
-                w.load_sh(g, ln.ch);
-                const float v = ln.act ? trilerp8(w.sv, p) * ln.basis : 0.f;
-                float col[3];
-                color_sums<B>(v, ln.j, col);
-                const float pcnt = ws * sigma;
-                const float weight = __expf(log_t) * (1.f - __expf(-pcnt));
-                log_t -= pcnt;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) out[c] += weight * fmaxf(col[c] + 0.5f, 0.f);
-                if (log_t < o.log_stop) {
-                    log_t = -1e3f;
-                    break;
+                n_iter += c.n_iter;
+                n_eval += c.n_eval;
+                n_hit += n_live;
+                for (int j = 0; j < c.n_eval; ++j) {
+                    fnv.mix(static_cast<uint32_t>(__shfl(c.base, j, kGroup)));
+                    fnv.mix(__float_as_uint(__shfl(c.t, j, kGroup)));
+                    fnv.mix(__shfl(static_cast<int>(c.contrib), j, kGroup) ? 1u : 0u);
                 }
-            } else if (STATS) {
-                fnv.mix(0u);
             }
-            t += o.step;
+            log_t = c.log_t_end;
+            t = c.next_t;
+            more = c.more;
         }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        acc[a] += __shfl_xor(acc[a], 8, kGroup);
+        acc[a] += __shfl_xor(acc[a], 16, kGroup);
     }
     const float tb = __expf(log_t) * o.bg;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] += tb;
-    if (STATS && ln.lane == 0) {
+    for (int a = 0; a < 3; ++a) out[a] = __shfl(acc[a], 0, kGroup) + tb;
+    if (STATS && lane == 0) {
         fnv.mix(__float_as_uint(log_t));
         stats[0] = static_cast<int32_t>(n_iter);
         stats[1] = static_cast<int32_t>(n_eval);
@@ -343,30 +475,42 @@ struct GradOut {
 
 // Backward march of one ray given dL/drgb (gq) and accum = rgb . gq (+ beta
 // term); accumulates into the gradient buffers (trace_ray_cuvol_backward,
-// render_lerp_kernel_cuvol.cu:227-383).
+// render_lerp_kernel_cuvol.cu:227-383).  Samples are rematerialised chunk by
+// chunk exactly as in the forward; each live sample's gradient is then added,
+// in sample order and with lane = SH channel, into per-corner register
+// accumulators of the current cell, which are flushed with one coalesced
+// no-return atomic per (corner, channel) when the ray leaves the cell.
 template <typename DT, int B>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
-                                               const Ray& r, const Lane<B>& ln,
-                                               const float* gq, float accum,
-                                               float sparsity, const GradOut& go) {
+                                               const Ray& r, const RayCtx<B>& rc, int lane,
+                                               const float* gq, float accum, float sparsity,
+                                               const GradOut& go) {
     if (r.tmin > r.tmax) return;
-    const int C = 3 * B;
-    const float gout = pick3(gq, ln.color);
-    Window<DT, B> w;
+    constexpr int C = 3 * B;
+    const bool act = lane < C;
+    const int color = act ? lane / B : 2;
+    const float basis_j = act ? rc.basis[lane % B] : 0.f;
+    const float gout = pick3(gq, color);
+    int wl0 = INT_MIN, wl1 = 0, wl2 = 0;
+    int wlk[8];
     float acc_c[8], acc_d[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { acc_c[k] = 0.f; acc_d[k] = 0.f; }
+    for (int k = 0; k < 8; ++k) {
+        wlk[k] = -1;
+        acc_c[k] = 0.f;
+        acc_d[k] = 0.f;
+    }
     bool touched = false;
 
     auto flush = [&]() {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int lk = w.lk[k];
+            const int lk = wlk[k];
             if (g.valid(lk)) {
-                if (go.gs != nullptr && ln.act)
-                    atomic_add_f32(go.gs + static_cast<int64_t>(lk) * C + ln.lane,
-                                   ln.basis * acc_c[k]);
-                if (ln.lane == 0) {
+                if (go.gs != nullptr && act)
+                    atomic_add_f32(go.gs + static_cast<int64_t>(lk) * C + lane,
+                                   basis_j * acc_c[k]);
+                if (lane == 0) {
                     if (go.gd != nullptr) atomic_add_f32(go.gd + lk, acc_d[k]);
                     if (go.mask != nullptr) go.mask[lk] = 1;
                 }
@@ -377,61 +521,67 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     };
 
     float t = r.tmin;
-    float ws = r.ws;
     float log_t = 0.f;
     int guard = 0;
-    while (t <= r.tmax && ++guard < kMaxIters) {
-        float p[3];
-        int l[3];
-        const int base = sample_cell(g, r, t, p, l);
-        if (!(l[0] == w.c0 && l[1] == w.c1 && l[2] == w.c2)) {
-            if (touched) { flush(); touched = false; }
-            w.enter(g, l, base);
-        }
-        if (w.lk[0] < -1) {
-            const float skip = skip_dist(w.lk[0], l, p, r);
-            if (skip >= o.step) {
-                t += ceilf(skip / o.step) * o.step;
-                continue;
-            }
-        }
-        w.load_density(g, base);
-        const float sigma = trilerp8(w.dn, p);
-        if (o.last_opaque && t + o.step > r.tmax) ws = 1e9f;
-        if (sigma > o.sigma_thresh) {
-            w.load_sh(g, ln.ch);
-            const float v = ln.act ? trilerp8(w.sv, p) * ln.basis : 0.f;
+    bool more = true;
+    while (more && ++guard < kMaxIters) {
+        Chunk c;
+        march_chunk(g, o, r, lane, t, log_t, c);
+        const int n_live = __popc(c.live_mask);
+        for (int pass = 0; pass * 4 < n_live; ++pass) {
             float col[3];
-            color_sums<B>(v, ln.j, col);
-            const float pcnt = ws * sigma;
-            const float weight = __expf(log_t) * (1.f - __expf(-pcnt));
-            log_t -= pcnt;
-            float total = 0.f;
+            int src;
+            bool has;
+            chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
+            // Apply the pass's samples in sample order, lane = channel.
+            const int n_here = min(4, n_live - pass * 4);
+            for (int qq = 0; qq < n_here; ++qq) {
+                const int s = nth_set_bit(c.live_mask, pass * 4 + qq);
+                float cs[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                col[c] += 0.5f;
-                total += fmaxf(col[c], 0.f) * gq[c];
-            }
-            const float mine = pick3(col, ln.color);
-            const float in01 = (fmaxf(mine, 0.f) == mine) ? 1.f : 0.f;
-            const float grad_common = weight * in01 * gout;
-            accum -= weight * total;
-            float grad_sigma = ws * (total * __expf(log_t) - accum);
-            if (sparsity > 0.f)
-                grad_sigma += sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
-            // Trilinear weights of the eight corners.
-            const float ax = 1.f - p[0], ay = 1.f - p[1], az = 1.f - p[2];
-            const float wyz[4] = {ay * az, ay * p[2], p[1] * az, p[1] * p[2]};
+                for (int a = 0; a < 3; ++a) cs[a] = __shfl(col[a], qq * 8, kGroup) + 0.5f;
+                const float weight = __shfl(c.weight, s, kGroup);
+                const float lt_after = __shfl(c.lt_after, s, kGroup);
+                const float ws = __shfl(c.ws, s, kGroup);
+                const float sigma = __shfl(c.sigma, s, kGroup);
+                float p[3];
+                int l[3];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float wk = wyz[k & 3] * ((k & 4) ? p[0] : ax);
-                acc_c[k] = fmaf(wk, grad_common, acc_c[k]);
-                acc_d[k] = fmaf(wk, grad_sigma, acc_d[k]);
+                for (int a = 0; a < 3; ++a) {
+                    p[a] = __shfl(c.p[a], s, kGroup);
+                    l[a] = __shfl(c.l[a], s, kGroup);
+                }
+                float total = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * gq[a];
+                const float mine = pick3(cs, color);
+                const float in01 = (fmaxf(mine, 0.f) == mine) ? 1.f : 0.f;
+                const float grad_common = weight * in01 * gout;
+                accum -= weight * total;
+                float grad_sigma = ws * (total * __expf(lt_after) - accum);
+                if (sparsity > 0.f)
+                    grad_sigma += sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
+                if (!(l[0] == wl0 && l[1] == wl1 && l[2] == wl2)) {
+                    if (touched) flush();
+                    touched = false;
+                    wl0 = l[0];
+                    wl1 = l[1];
+                    wl2 = l[2];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) wlk[k] = __shfl(c.lk[k], s, kGroup);
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const float wk = corner_weight(k, p);
+                    acc_c[k] = fmaf(wk, grad_common, acc_c[k]);
+                    acc_d[k] = fmaf(wk, grad_sigma, acc_d[k]);
+                }
+                touched = true;
             }
-            touched = true;
-            if (log_t < o.log_stop) break;
         }
-        t += o.step;
+        log_t = c.log_t_end;
+        t = c.next_t;
+        more = c.more;
     }
     if (touched) flush();
 }
@@ -451,11 +601,11 @@ __global__ __launch_bounds__(kBlock) void k_render_rays(
     if (ray >= n) return;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
-    const Lane<B> ln(lane, wd);
+    const RayCtx<B> rc(wd);
     Ray r;
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float out[3];
-    const float log_t = march_forward<DT, B, STATS>(g, o, r, ln, out,
+    const float log_t = march_forward<DT, B, STATS>(g, o, r, rc, lane, out,
                                                     STATS ? stats + 4 * ray : nullptr);
     if (!STATS) {
         if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
@@ -510,11 +660,11 @@ __global__ __launch_bounds__(kBlock) void k_render_image(
     const int iy = static_cast<int>(ray / cam.width);
     float wd[3], wo[3];
     camera_ray(cam, ix, iy, wd, wo);
-    const Lane<B> ln(lane, wd);
+    const RayCtx<B> rc(wd);
     Ray r;
     setup_ray(g, o, wo, wd, nullptr, r);
     float out[3];
-    const float log_t = march_forward<DT, B, false>(g, o, r, ln, out, nullptr);
+    const float log_t = march_forward<DT, B, false>(g, o, r, rc, lane, out, nullptr);
     if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
     if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
 }
@@ -530,13 +680,13 @@ __global__ __launch_bounds__(kBlock) void k_render_backward(
     if (ray >= n) return;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
-    const Lane<B> ln(lane, wd);
+    const RayCtx<B> rc(wd);
     Ray r;
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     const float gq[3] = {grad_out[3 * ray], grad_out[3 * ray + 1], grad_out[3 * ray + 2]};
     const float* cc = color_cache + 3 * ray;
     const float accum = fmaf(cc[0], gq[0], fmaf(cc[1], gq[1], cc[2] * gq[2]));
-    march_backward<DT, B>(g, o, r, ln, gq, accum, 0.f, go);
+    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, 0.f, go);
 }
 
 // Forward then backward of the same ray in one pass of the workgroup: the
@@ -553,11 +703,11 @@ __global__ __launch_bounds__(kBlock) void k_render_fused(
     if (ray >= n) return;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
-    const Lane<B> ln(lane, wd);
+    const RayCtx<B> rc(wd);
     Ray r;
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float rgb[3];
-    const float log_t = march_forward<DT, B, false>(g, o, r, ln, rgb, nullptr);
+    const float log_t = march_forward<DT, B, false>(g, o, r, rc, lane, rgb, nullptr);
     if (lane < 3) rgb_out[3 * ray + lane] = pick3(rgb, lane);
     float gq[3];
 #pragma unroll
@@ -567,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_render_fused(
         const float tin = __expf(log_t);
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
-    march_backward<DT, B>(g, o, r, ln, gq, accum, sparsity, go);
+    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go);
 }
 
 // ---------------------------------------------------------------------------

@@ -104,6 +104,10 @@ def test_finite_differences_with_ray_clip(gpu, march):
     R = g.density.shape[0]
     rng = np.random.default_rng(0)
     subset = rng.choice(R, R // 100, replace=False)
+    # Directions move only rows whose density stays far above sigma_thresh under
+    # the perturbation: near-zero densities sit on the sigma_thresh kink, where
+    # the loss has no derivative.
+    smooth = subset[g.density[subset] > 1e-2]
 
     def L(dens, sh):
         return float(((O.render_fp64(g, o.numpy(), d.numpy(), OPT, torch.from_numpy(dens),
@@ -113,13 +117,13 @@ def test_finite_differences_with_ray_clip(gpu, march):
     h = 1e-4
     for k in range(4):
         vd = np.zeros(R)
-        vd[subset] = rng.standard_normal(subset.size)
+        vd[smooth] = rng.standard_normal(smooth.size)
         vs = np.zeros_like(g.sh)
-        vs[subset] = rng.standard_normal((subset.size, g.sh.shape[1]))
+        vs[smooth] = rng.standard_normal((smooth.size, g.sh.shape[1]))
         fd = (L(g.density + h * vd, g.sh + h * vs) - L(g.density - h * vd, g.sh - h * vs)) / (2 * h)
         an = float(gd @ vd + (gs * vs).sum())
         assert an == pytest.approx(fd, rel=2e-3, abs=1e-9), k
-    top = subset[np.argsort(-np.abs(gd[subset]))[:4]]
+    top = smooth[np.argsort(-np.abs(gd[smooth]))[:4]]
     for r in top:
         e = np.zeros(R)
         e[r] = 1

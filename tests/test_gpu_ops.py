@@ -1,7 +1,8 @@
 """GPU sampling, total variation, RMSprop and accel_dist_prop against the
-oracle.  Tolerances: trilinear samples 1e-6 absolute (float32 lerps of O(10)
-values); gradients 1e-5 relative to their max; RMSprop 1e-6 relative (float32
-arithmetic of the closed form); accel_dist_prop and masks bit-exact."""
+oracle.  Tolerances: trilinear samples 1e-5 absolute (float32 lerps of O(10)
+values); gradients 1e-5 relative to their max; RMSprop data 1e-6 and rms 1e-5
+relative (float32 arithmetic of the closed form; BF16 data within its rounding);
+accel_dist_prop and masks bit-exact."""
 import numpy as np
 import pytest
 import torch
@@ -97,7 +98,9 @@ def test_rmsprop_all_mask_index(gpu, dtype):
                                      grad0.double().numpy(), rows, 0.95, 0.1, 1e-8, -1.5, 0.05)
         tol = 1e-6 if dtype == torch.float32 else 8e-3
         assert np.allclose(data.double().cpu().numpy(), rd, rtol=tol, atol=tol), kind
-        assert np.allclose(rms.cpu().numpy(), rr, rtol=1e-6, atol=1e-12), kind
+        # rms = fma(beta, rms - g^2, g^2) cancels when g^2 >> rms: float32 keeps
+        # about 1e-6 relative there
+        assert np.allclose(rms.cpu().numpy(), rr, rtol=1e-5, atol=1e-12), kind
         assert np.array_equal(grad.cpu().numpy(), rgr.astype(np.float32)), kind
     # an empty 1-d indexer skips the update
     data = data0.clone().to(gpu)

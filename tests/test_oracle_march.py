@@ -200,3 +200,24 @@ def test_touched_rows_match_contributions(dense64, march):
     # every row with a nonzero gradient is touched
     assert not np.any((np.abs(gd) > 0) & ~m)
     assert m.sum() > 0
+
+
+def test_c_fused_port_matches_fp64(march):
+    """The scalar C port of the fused forward+backward (the CPU baseline)
+    against the FP64 autograd oracle, with per-ray clips, beta and sparsity."""
+    gen = torch.Generator().manual_seed(21)
+    sc = scenes.make_blob_grid(32, scenes.BlobSpec(4, (0.15, 0.25), (10.0, 40.0)), gen)
+    links = O.accel_dist_prop(sc.links.numpy())
+    g = host_grid(sc, links=links)
+    o, d = scenes.sphere_rays(512, gen, radius=1.6, target=0.4)
+    gt = torch.rand(512, 3, generator=gen, dtype=torch.float64).numpy()
+    clip = scenes.config_clip(512, gen).numpy() * 0.7
+    opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+    kw = dict(beta_loss=0.2, sparsity_loss=0.01)
+    rgb, gd, gs, m = march.fused(g, o.numpy(), d.numpy(), opt, gt, clip=clip, **kw)
+    smp = march.samples(g, o.numpy(), d.numpy(), opt, clip=clip)
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o.numpy(), d.numpy(), opt, gt, samples=smp, **kw)
+    assert np.abs(rgb - rgb64).max() < 1e-5
+    assert np.abs(gd - gd64).max() < 1e-4 * np.abs(gd64).max()
+    assert np.abs(gs - gs64).max() < 1e-4 * np.abs(gs64).max()
+    assert np.array_equal(m, O.touched_rows(g, smp))
