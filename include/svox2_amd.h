@@ -18,6 +18,11 @@
  *                        [R0..R(B-1), G0..G(B-1), B0..B(B-1)], B = basis_dim
  *   gradients    always float32, same row count as the data
  *   rays         origins/dirs float32 [n][3]
+ * Per-row arrays may be row-strided: a *_stride field gives the distance in
+ * elements between consecutive rows (0 = contiguous: 1 for density, sh_data_dim
+ * for SH).  The Python package stores each voxel as one 128-byte row
+ * [SH(27) | density | pad] (stride 32) so a corner costs one aligned line, and
+ * lays its float32 gradients out the same way.
  */
 #ifndef SVOX2_AMD_H_
 #define SVOX2_AMD_H_
@@ -60,6 +65,8 @@ typedef struct svox2_grid {
     int32_t storage;     /* SVOX2_STORAGE_* for density_data and sh_data */
     float offset[3];
     float scaling[3];
+    int64_t density_stride; /* elements between density rows (0 = 1) */
+    int64_t sh_stride;      /* elements between SH rows (0 = sh_data_dim) */
 } svox2_grid_t;
 
 /* RenderOptions (svox2/csrc/include/data_spec.hpp:115-135). */
@@ -87,6 +94,8 @@ typedef struct svox2_grads {
     float* grad_density_out; /* rows x 1  */
     float* grad_sh_out;      /* rows x sh_data_dim */
     uint8_t* mask_out;       /* rows, set to 1 for every row a sample touched */
+    int64_t grad_density_stride; /* elements between rows (0 = 1) */
+    int64_t grad_sh_stride;      /* elements between rows (0 = sh_data_dim) */
 } svox2_grads_t;
 
 /* Forward render of n_rays rays; rgb_out [n][3].  ray_clip, if not NULL, is
@@ -151,23 +160,29 @@ int svox2_sample_grid(const svox2_grid_t* grid, const float* points,
                       int64_t n_points, int32_t want_colors,
                       float* density_out, float* sh_out, void* stream);
 
-/* Adjoint of svox2_sample_grid; accumulates into grad_density_out / grad_sh_out.
+/* Adjoint of svox2_sample_grid; accumulates into grad_density_out / grad_sh_out
+ * (row strides in elements, 0 = contiguous).
  * Replaces sample_grid_backward (svox2.cpp:59; svox2_kernel.cu:219). */
 int svox2_sample_grid_backward(const svox2_grid_t* grid, const float* points,
                                int64_t n_points,
                                const float* grad_out_density,
                                const float* grad_out_sh,
                                float* grad_density_out, float* grad_sh_out,
+                               int64_t grad_density_stride,
+                               int64_t grad_sh_stride,
                                int32_t want_colors, void* stream);
 
 /* Sparse total-variation gradient over n_cells flat cell indices (rand_cells),
- * channels [start_dim, end_dim) of data (rows x data_dim), scaled by
+ * channels [start_dim, end_dim) of data (rows x data_dim, consecutive rows
+ * data_stride elements apart; grad_data rows grad_stride apart; 0 =
+ * contiguous), scaled by
  * scale / n_cells and accumulated into grad_data; mask_out (NULL allowed) is
  * set for every row that receives a gradient.
  * Replaces tv_grad_sparse (svox2.cpp:45; loss_kernel.cu:558). */
 int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
                          const void* data, int32_t storage, int64_t rows,
-                         int32_t data_dim, const int32_t* rand_cells,
+                         int32_t data_dim, int64_t data_stride,
+                         int64_t grad_stride, const int32_t* rand_cells,
                          int64_t n_cells, uint8_t* mask_out, int32_t start_dim,
                          int32_t end_dim, float scale, int32_t use_logalpha,
                          float logalpha_delta, int32_t ignore_edge,
@@ -175,11 +190,14 @@ int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
                          float ndc_coeffy, float* grad_data, void* stream);
 
 /* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
- * rms and grad; grad is zeroed where updated.  indexer_kind selects every row,
+ * rms and grad; grad is zeroed where updated.  Row strides (elements, 0 =
+ * dim) apply to data, rms and grad separately.  indexer_kind selects every row,
  * a byte mask (n_index = rows) or int64 row indices (n_index entries).
  * Replaces rmsprop_step (svox2.cpp:66; optim_kernel.cu:154). */
 int svox2_rmsprop_step(void* data, int32_t storage, float* rms, float* grad,
-                       int64_t rows, int32_t dim, int32_t indexer_kind,
+                       int64_t rows, int32_t dim, int64_t data_stride,
+                       int64_t rms_stride, int64_t grad_stride,
+                       int32_t indexer_kind,
                        const void* indexer, int64_t n_index, float beta,
                        float lr, float epsilon, float minval, float lr_last,
                        void* stream);

@@ -63,6 +63,8 @@ class _Grid(ctypes.Structure):
         ("storage", ctypes.c_int32),
         ("offset", ctypes.c_float * 3),
         ("scaling", ctypes.c_float * 3),
+        ("density_stride", ctypes.c_int64),
+        ("sh_stride", ctypes.c_int64),
     ]
 
 
@@ -93,6 +95,8 @@ class _Grads(ctypes.Structure):
         ("grad_density_out", ctypes.c_void_p),
         ("grad_sh_out", ctypes.c_void_p),
         ("mask_out", ctypes.c_void_p),
+        ("grad_density_stride", ctypes.c_int64),
+        ("grad_sh_stride", ctypes.c_int64),
     ]
 
 
@@ -122,11 +126,11 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, Gr, vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
-        "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i32, vp],
-        "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, vp, i64, vp, i32,
-                                 i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
-        "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i32, vp, i64, f32, f32, f32, f32,
-                               f32, vp],
+        "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
+        "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, vp, i64,
+                                 vp, i32, i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
+        "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
+                               f32, f32, f32, vp],
         "svox2_accel_dist_prop": [vp, ctypes.POINTER(i32), vp, ctypes.c_size_t, vp],
     }
     for name, argtypes in sig.items():
@@ -233,6 +237,30 @@ def _check(t: Optional[torch.Tensor], name: str, dtypes=None, ndim: Optional[int
     return t
 
 
+def _check_rows(t: Optional[torch.Tensor], name: str, dtypes=None, cols: Optional[int] = None):
+    """A 2-d device tensor whose rows are contiguous but may be strided apart
+    (the interleaved row layout): stride(1) == 1 and stride(0) >= size(1)."""
+    if t is None:
+        raise RuntimeError(f"{name} must be defined")
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} must be a tensor")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if t.dim() != 2:
+        raise RuntimeError(f"{name} must have 2 dims, got {t.dim()}")
+    if dtypes is not None and t.dtype not in dtypes:
+        raise RuntimeError(f"{name} must have dtype in {dtypes}, got {t.dtype}")
+    if cols is not None and t.size(1) != cols:
+        raise RuntimeError(f"{name} must have {cols} columns, got {t.size(1)}")
+    if t.size(0) > 1 and not ((t.size(1) == 1 or t.stride(1) == 1) and t.stride(0) >= t.size(1)):
+        raise RuntimeError(f"{name} must be contiguous or row-strided (stride(1) == 1)")
+    return t
+
+
+def _row_stride(t: torch.Tensor) -> int:
+    return int(t.stride(0)) if t.size(0) > 1 else int(t.size(1))
+
+
 def _ptr(t: Optional[torch.Tensor]):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
@@ -280,10 +308,10 @@ def _vec3(v) -> Tuple[float, float, float]:
 
 def _grid_struct(g: SparseGridSpec, need_sh: bool = True) -> _Grid:
     links = _check(g.links, "links", (torch.int32,), 3)
-    dens = _check(g.density_data, "density_data", _DATA_DTYPES, 2)
+    dens = _check_rows(g.density_data, "density_data", _DATA_DTYPES)
     sh = g.sh_data
     if need_sh or sh is not None:
-        sh = _check(sh, "sh_data", _DATA_DTYPES, 2)
+        sh = _check_rows(sh, "sh_data", _DATA_DTYPES)
         if sh.dtype != dens.dtype:
             raise RuntimeError("density_data and sh_data must share a storage dtype")
         if sh.size(0) != dens.size(0):
@@ -309,6 +337,8 @@ def _grid_struct(g: SparseGridSpec, need_sh: bool = True) -> _Grid:
     for i in range(3):
         s.offset[i] = off[i]
         s.scaling[i] = sc[i]
+    s.density_stride = _row_stride(dens)
+    s.sh_stride = _row_stride(sh) if sh is not None else 0
     return s
 
 
@@ -340,15 +370,17 @@ def _rays(r: RaysSpec):
 def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
     s = _Grads()
     if gr.grad_density_out is not None:
-        t = _check(gr.grad_density_out, "grad_density_out", (torch.float32,))
-        if t.numel() != rows:
+        t = _check_rows(gr.grad_density_out, "grad_density_out", (torch.float32,), 1)
+        if t.size(0) != rows:
             raise RuntimeError("grad_density_out must have one value per row")
         s.grad_density_out = t.data_ptr()
+        s.grad_density_stride = _row_stride(t)
     if gr.grad_sh_out is not None:
-        t = _check(gr.grad_sh_out, "grad_sh_out", (torch.float32,))
-        if t.numel() != rows * sh_dim:
+        t = _check_rows(gr.grad_sh_out, "grad_sh_out", (torch.float32,), sh_dim)
+        if t.size(0) != rows:
             raise RuntimeError("grad_sh_out must be (rows, sh_data_dim)")
         s.grad_sh_out = t.data_ptr()
+        s.grad_sh_stride = _row_stride(t)
     if gr.mask_out is not None and gr.mask_out.numel() > 0:
         t = _check(gr.mask_out, "mask_out", (torch.bool, torch.uint8))
         if t.numel() != rows:
@@ -483,20 +515,21 @@ def sample_grid_backward(grid: SparseGridSpec, points: torch.Tensor,
     _check(points, "points", (torch.float32,), 2)
     n = points.size(0)
     _check(grad_out_density, "grad_out_density", (torch.float32,), 2)
-    _check(grad_density_out, "grad_density_out", (torch.float32,))
-    if grad_out_density.shape != (n, 1) or grad_density_out.numel() != g.rows:
+    _check_rows(grad_density_out, "grad_density_out", (torch.float32,), 1)
+    if grad_out_density.shape != (n, 1) or grad_density_out.size(0) != g.rows:
         raise RuntimeError("density gradient shapes do not match")
+    gss = 0
     if want_colors:
         _check(grad_out_sh, "grad_out_sh", (torch.float32,), 2)
-        _check(grad_sh_out, "grad_sh_out", (torch.float32,))
-        if grad_out_sh.shape != (n, g.sh_data_dim) or \
-                grad_sh_out.numel() != g.rows * g.sh_data_dim:
+        _check_rows(grad_sh_out, "grad_sh_out", (torch.float32,), g.sh_data_dim)
+        if grad_out_sh.shape != (n, g.sh_data_dim) or grad_sh_out.size(0) != g.rows:
             raise RuntimeError("sh gradient shapes do not match")
+        gss = _row_stride(grad_sh_out)
     with _on_device(points):
         _call("svox2_sample_grid_backward", ctypes.byref(g), _ptr(points), n,
               _ptr(grad_out_density), _ptr(grad_out_sh) if want_colors else None,
               _ptr(grad_density_out), _ptr(grad_sh_out) if want_colors else None,
-              int(bool(want_colors)), _stream(points))
+              _row_stride(grad_density_out), gss, int(bool(want_colors)), _stream(points))
 
 
 def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Tensor,
@@ -506,8 +539,8 @@ def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Te
                    grad_data: torch.Tensor) -> None:
     """tv_grad_sparse (loss_kernel.cu:558-604)."""
     _check(links, "links", (torch.int32,), 3)
-    _check(data, "data", _DATA_DTYPES, 2)
-    _check(grad_data, "grad_data", (torch.float32,), 2)
+    _check_rows(data, "data", _DATA_DTYPES)
+    _check_rows(grad_data, "grad_data", (torch.float32,))
     _check(rand_cells, "rand_cells", (torch.int32,))
     if grad_data.shape != data.shape:
         raise RuntimeError("grad_data must match data's shape")
@@ -520,7 +553,8 @@ def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Te
     size = (ctypes.c_int32 * 3)(*links.shape)
     with _on_device(data):
         _call("svox2_tv_grad_sparse", _ptr(links), size, _ptr(data), _storage_of(data),
-              data.size(0), data.size(1), _ptr(rand_cells), rand_cells.numel(), mask_ptr,
+              data.size(0), data.size(1), _row_stride(data), _row_stride(grad_data),
+              _ptr(rand_cells), rand_cells.numel(), mask_ptr,
               int(start_dim), int(end_dim), float(scale), int(bool(use_logalpha)),
               float(logalpha_delta), int(bool(ignore_edge)), int(bool(ignore_last_z)),
               float(ndc_coeffx), float(ndc_coeffy), _ptr(grad_data), _stream(data))
@@ -532,9 +566,9 @@ def rmsprop_step(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor,
     """rmsprop_step (optim_kernel.cu:154-218).  ``indexer``: 0-dim tensor = all
     rows, bool/uint8 (rows,) mask, or int64 row indices; an empty 1-d indexer
     skips the update."""
-    _check(data, "data", _DATA_DTYPES, 2)
-    _check(rms, "rms", (torch.float32,))
-    _check(grad, "grad", (torch.float32,))
+    _check_rows(data, "data", _DATA_DTYPES)
+    _check_rows(rms, "rms", (torch.float32,))
+    _check_rows(grad, "grad", (torch.float32,))
     if rms.shape != data.shape or grad.shape != data.shape:
         raise RuntimeError("rms / grad must match data's shape")
     rows, dim = data.shape
@@ -550,8 +584,9 @@ def rmsprop_step(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor,
         kind, iptr, n = INDEXER_INDEX, _ptr(indexer), indexer.numel()
     with _on_device(data):
         _call("svox2_rmsprop_step", _ptr(data), _storage_of(data), _ptr(rms), _ptr(grad), rows,
-              dim, kind, iptr, n, float(beta), float(lr), float(epsilon), float(minval),
-              float(lr_last), _stream(data))
+              dim, _row_stride(data), _row_stride(rms), _row_stride(grad), kind, iptr, n,
+              float(beta), float(lr), float(epsilon), float(minval), float(lr_last),
+              _stream(data))
 
 
 def accel_dist_prop(links: torch.Tensor) -> None:

@@ -31,6 +31,15 @@ int check_grid(const svox2_grid_t* g, bool need_sh) {
                   "basis_dim %d not supported (1, 4 or 9)", g->basis_dim);
     SVOX2_REQUIRE(g->sh_data_dim == 3 * g->basis_dim,
                   "sh_data_dim %d != 3 * basis_dim %d", g->sh_data_dim, g->basis_dim);
+    SVOX2_REQUIRE(g->density_stride >= 0 && g->density_stride < INT_MAX,
+                  "density_stride %lld out of range", (long long)g->density_stride);
+    SVOX2_REQUIRE(g->sh_stride == 0 || (g->sh_stride >= g->sh_data_dim && g->sh_stride < INT_MAX),
+                  "sh_stride %lld must be >= sh_data_dim %d", (long long)g->sh_stride,
+                  g->sh_data_dim);
+    const int64_t ss = g->sh_stride > 0 ? g->sh_stride : g->sh_data_dim;
+    const int64_t ds = g->density_stride > 0 ? g->density_stride : 1;
+    SVOX2_REQUIRE(g->rows * ss < (INT64_C(1) << 40) && g->rows * ds < (INT64_C(1) << 40),
+                  "grid data too large");
     SVOX2_REQUIRE(g->rows == 0 || g->density_data != nullptr, "grid.density_data is NULL");
     if (need_sh) SVOX2_REQUIRE(g->rows == 0 || g->sh_data != nullptr, "grid.sh_data is NULL");
     return ok();

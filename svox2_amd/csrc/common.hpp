@@ -98,6 +98,9 @@ struct GridView {
     int rows;
     int basis_dim;
     int sh_dim;
+    int dstride;   // elements between density rows
+    int sstride;   // elements between SH rows
+    bool vec_rows; // SH rows are 16-byte aligned with room for whole 16-byte loads
     float offset[3];
     float scaling[3];
     // A link addresses a data row iff 0 <= link < rows; negative links are
@@ -130,11 +133,25 @@ inline GridView<DT> make_view(const svox2_grid_t& g) {
     v.rows = static_cast<int>(g.rows);
     v.basis_dim = g.basis_dim;
     v.sh_dim = g.sh_data_dim;
+    v.dstride = g.density_stride > 0 ? static_cast<int>(g.density_stride) : 1;
+    v.sstride = g.sh_stride > 0 ? static_cast<int>(g.sh_stride) : g.sh_data_dim;
+    // Whole 16-byte loads of a row stay inside it when the row stride is a
+    // multiple of 16 bytes (it is then >= the channel count rounded up to 16 bytes).
+    const int per16 = 16 / static_cast<int>(sizeof(DT));
+    v.vec_rows = (reinterpret_cast<uintptr_t>(g.sh_data) % 16 == 0) && (v.sstride % per16 == 0);
     return v;
 }
 
 // Validates a grid descriptor; `need_sh` requires sh_data to be present.
 int check_grid(const svox2_grid_t* g, bool need_sh);
+
+// Resolved row strides of a gradient descriptor.
+inline int64_t grad_dstride(const svox2_grads_t& g) {
+    return g.grad_density_stride > 0 ? g.grad_density_stride : 1;
+}
+inline int64_t grad_sstride(const svox2_grads_t& g, int C) {
+    return g.grad_sh_stride > 0 ? g.grad_sh_stride : C;
+}
 
 inline unsigned blocks_for(int64_t n, int threads) {
     return static_cast<unsigned>((n + threads - 1) / threads);

@@ -19,7 +19,8 @@ constexpr int kBlock = 256;
 template <typename DT>
 __global__ __launch_bounds__(kBlock) void k_tv_grad_sparse(
         const int32_t* __restrict__ links, int s0, int s1, int s2,
-        const DT* __restrict__ data, int rows, int D, const int32_t* __restrict__ cells,
+        const DT* __restrict__ data, int rows, int64_t dst, int64_t gst,
+        const int32_t* __restrict__ cells,
         int64_t total, int start_dim, int nch, float scale, int ignore_edge,
         int ignore_last_z, uint8_t* __restrict__ mask, float* __restrict__ grad) {
     const int64_t tid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -46,11 +47,11 @@ __global__ __launch_bounds__(kBlock) void k_tv_grad_sparse(
     if (ignore_last_z && z == s2 - 2) return;
 
     auto ok = [&](int l) { return static_cast<uint32_t>(l) < static_cast<uint32_t>(rows); };
-    const float v000 = ok(l000) ? ld(data, static_cast<int64_t>(l000) * D + idx) : 0.f;
+    const float v000 = ok(l000) ? ld(data, static_cast<int64_t>(l000) * dst + idx) : 0.f;
     const float nul = ignore_edge ? v000 : 0.f;
-    const float v001 = ok(l001) ? ld(data, static_cast<int64_t>(l001) * D + idx) : nul;
-    const float v010 = ok(l010) ? ld(data, static_cast<int64_t>(l010) * D + idx) : nul;
-    const float v100 = ok(l100) ? ld(data, static_cast<int64_t>(l100) * D + idx) : nul;
+    const float v001 = ok(l001) ? ld(data, static_cast<int64_t>(l001) * dst + idx) : nul;
+    const float v010 = ok(l010) ? ld(data, static_cast<int64_t>(l010) * dst + idx) : nul;
+    const float v100 = ok(l100) ? ld(data, static_cast<int64_t>(l100) * dst + idx) : nul;
 
     float dx = v100 - v000;
     float dy = v010 - v000;
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_tv_grad_sparse(
     const float sm = -(dx + dy + dz);
     auto add = [&](int l, float val) {
         if (ok(l) && val != 0.f) {
-            atomic_add_f32(grad + static_cast<int64_t>(l) * D + idx, val * idelta);
+            atomic_add_f32(grad + static_cast<int64_t>(l) * gst + idx, val * idelta);
             if (mask != nullptr) mask[l] = 1;
         }
     };
@@ -79,6 +80,7 @@ using namespace svox2amd;
 
 extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3], const void* data,
                                     int32_t storage, int64_t rows, int32_t data_dim,
+                                    int64_t data_stride, int64_t grad_stride,
                                     const int32_t* rand_cells, int64_t n_cells,
                                     uint8_t* mask_out, int32_t start_dim, int32_t end_dim,
                                     float scale, int32_t use_logalpha, float logalpha_delta,
@@ -103,6 +105,10 @@ extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
     if (n_cells == 0 || rows == 0) return ok();
     SVOX2_REQUIRE(data != nullptr && rand_cells != nullptr && grad_data != nullptr,
                   "data / rand_cells / grad_data are NULL");
+    SVOX2_REQUIRE(data_stride >= 0 && grad_stride >= 0, "negative row stride");
+    const int64_t dst = data_stride > 0 ? data_stride : data_dim;
+    const int64_t gst = grad_stride > 0 ? grad_stride : data_dim;
+    SVOX2_REQUIRE(dst >= data_dim && gst >= data_dim, "row stride below data_dim");
     const int nch = end_dim - start_dim;
     const int64_t total = n_cells * nch;
     // The reference divides the scale by the number of cells (loss_kernel.cu:597).
@@ -112,12 +118,12 @@ extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
     if (storage == SVOX2_STORAGE_BF16) {
         k_tv_grad_sparse<uint16_t><<<blocks, kBlock, 0, s>>>(
             links, size[0], size[1], size[2], static_cast<const uint16_t*>(data),
-            static_cast<int>(rows), data_dim, rand_cells, total, start_dim, nch, sc, ignore_edge,
+            static_cast<int>(rows), dst, gst, rand_cells, total, start_dim, nch, sc, ignore_edge,
             ignore_last_z, mask_out, grad_data);
     } else {
         k_tv_grad_sparse<float><<<blocks, kBlock, 0, s>>>(
             links, size[0], size[1], size[2], static_cast<const float*>(data),
-            static_cast<int>(rows), data_dim, rand_cells, total, start_dim, nch, sc, ignore_edge,
+            static_cast<int>(rows), dst, gst, rand_cells, total, start_dim, nch, sc, ignore_edge,
             ignore_last_z, mask_out, grad_data);
     }
     return check_launch("tv_grad_sparse");

@@ -308,7 +308,9 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
 
     float dn[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dn[k] = (eval && g.valid(c.lk[k])) ? ld(g.density, c.lk[k]) : 0.f;
+    for (int k = 0; k < 8; ++k)
+        dn[k] = (eval && g.valid(c.lk[k])) ? ld(g.density, static_cast<int64_t>(c.lk[k]) * g.dstride)
+                                           : 0.f;
     c.sigma = trilerp8(dn, c.p);
     c.ws = (o.last_opaque && ti + o.step > r.tmax) ? 1e9f : r.ws;
     c.contrib = eval && c.sigma > o.sigma_thresh;
@@ -353,6 +355,49 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
     }
 }
 
+// Loads the C SH channels of row lk.  Rows that are 16-byte aligned with a
+// stride of whole 16-byte units are read with 16-byte loads (one aligned
+// 128-byte line for a float32 SH-2 row of stride 32, 64 bytes in bf16).
+template <int C>
+__device__ __forceinline__ void load_row(const GridView<float>& g, int lk, float* x) {
+    const float* row = g.sh + static_cast<int64_t>(lk) * g.sstride;
+    if (g.vec_rows) {
+        const float4* r4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(row, 16));
+#pragma unroll
+        for (int i = 0; i < (C + 3) / 4; ++i) {
+            const float4 v = r4[i];
+            if (4 * i + 0 < C) x[4 * i + 0] = v.x;
+            if (4 * i + 1 < C) x[4 * i + 1] = v.y;
+            if (4 * i + 2 < C) x[4 * i + 2] = v.z;
+            if (4 * i + 3 < C) x[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) x[ch] = row[ch];
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void load_row(const GridView<uint16_t>& g, int lk, float* x) {
+    const uint16_t* row = g.sh + static_cast<int64_t>(lk) * g.sstride;
+    if (g.vec_rows) {
+        const uint4* r4 = reinterpret_cast<const uint4*>(__builtin_assume_aligned(row, 16));
+#pragma unroll
+        for (int i = 0; i < (C + 7) / 8; ++i) {
+            const uint4 v = r4[i];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                if (8 * i + 2 * h < C) x[8 * i + 2 * h] = __uint_as_float(w[h] << 16);
+                if (8 * i + 2 * h + 1 < C) x[8 * i + 2 * h + 1] = __uint_as_float(w[h] & 0xffff0000u);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) x[ch] = ld(row, ch);
+    }
+}
+
 // Colours of up to four live samples of a chunk at once: lane = (sample s, corner
 // k) with s = lane / 8.  Each lane reads its corner's SH row, projects it on
 // the SH basis and weights it trilinearly; the eight corners are summed within
@@ -379,10 +424,8 @@ __device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx
     float v0 = 0.f, v1 = 0.f, v2 = 0.f;
     if (has && g.valid(lk)) {
         constexpr int C = 3 * B;
-        const DT* row = g.sh + static_cast<int64_t>(lk) * C;
         float x[C];
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) x[ch] = ld(row, ch);
+        load_row<C>(g, lk, x);
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             v0 = fmaf(rc.basis[j], x[j], v0);
@@ -468,9 +511,123 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
 }
 
 struct GradOut {
-    float* __restrict__ gd;   // grad density (rows)
-    float* __restrict__ gs;   // grad sh (rows x C)
+    float* __restrict__ gd;   // grad density, rows gds elements apart
+    float* __restrict__ gs;   // grad sh, rows gss elements apart
     uint8_t* __restrict__ mask;
+    int64_t gds;
+    int64_t gss;
+};
+
+// Channel-parallel gradient accumulators for the 8 corners of the cell the
+// ray is in.  Moving to an adjacent cell keeps the 4 (or 2, or 1) corners the
+// two cells share and flushes only the corners left behind, so each grid point
+// is flushed once per visit.  A flush is one no-return atomic instruction per
+// corner: lanes 0..C-1 add the SH channels and lane C the density (the same
+// 128-byte row in the interleaved layout); lane C+1 marks the row.
+template <int C>
+struct GradWindow {
+    int c0 = INT_MIN, c1 = 0, c2 = 0;
+    int wlk[8];
+    float acc_c[8];
+    float acc_d[8];
+    uint32_t used = 0;
+
+    __device__ __forceinline__ GradWindow() {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            wlk[k] = -1;
+            acc_c[k] = 0.f;
+            acc_d[k] = 0.f;
+        }
+    }
+
+    template <typename DT>
+    __device__ __forceinline__ void flush_slot(const GridView<DT>& g, const GradOut& go, int lane,
+                                               float basis_j, int k) {
+        const int lk = wlk[k];
+        if (g.valid(lk)) {
+            float* ptr = nullptr;
+            float val = 0.f;
+            if (lane < C) {
+                if (go.gs != nullptr) {
+                    ptr = go.gs + static_cast<int64_t>(lk) * go.gss + lane;
+                    val = basis_j * acc_c[k];
+                }
+            } else if (lane == C) {
+                if (go.gd != nullptr) {
+                    ptr = go.gd + static_cast<int64_t>(lk) * go.gds;
+                    val = acc_d[k];
+                }
+            }
+            if (ptr != nullptr) atomic_add_f32(ptr, val);
+            if (lane == C + 1 && go.mask != nullptr) go.mask[lk] = 1;
+        }
+        acc_c[k] = 0.f;
+        acc_d[k] = 0.f;
+    }
+
+    template <typename DT>
+    __device__ __forceinline__ void flush_all(const GridView<DT>& g, const GradOut& go, int lane,
+                                              float basis_j) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (used & (1u << k)) flush_slot(g, go, lane, basis_j, k);
+        used = 0;
+    }
+
+    // Shift along one axis (corner bit `bit`) by d in {-1, 0, 1}.
+    template <int bit, typename DT>
+    __device__ __forceinline__ void shift(int d, const GridView<DT>& g, const GradOut& go,
+                                          int lane, float basis_j) {
+        constexpr uint32_t lo_mask = bit == 4 ? 0x0Fu : (bit == 2 ? 0x33u : 0x55u);
+        if (d > 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k & bit) continue;
+                if (used & (1u << k)) flush_slot(g, go, lane, basis_j, k);
+                acc_c[k] = acc_c[k | bit];
+                acc_d[k] = acc_d[k | bit];
+                wlk[k] = wlk[k | bit];
+                acc_c[k | bit] = 0.f;
+                acc_d[k | bit] = 0.f;
+            }
+            used = (used >> bit) & lo_mask;
+        } else if (d < 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k & bit) continue;
+                if (used & (1u << (k | bit))) flush_slot(g, go, lane, basis_j, k | bit);
+                acc_c[k | bit] = acc_c[k];
+                acc_d[k | bit] = acc_d[k];
+                wlk[k | bit] = wlk[k];
+                acc_c[k] = 0.f;
+                acc_d[k] = 0.f;
+            }
+            used = (used & lo_mask) << bit;
+        }
+    }
+
+    // Makes cell l with corner links nlk current.
+    template <typename DT>
+    __device__ __forceinline__ void move(const int* l, const int* nlk, const GridView<DT>& g,
+                                         const GradOut& go, int lane, float basis_j) {
+        if (l[0] == c0 && l[1] == c1 && l[2] == c2) return;
+        const int d0 = l[0] - c0, d1 = l[1] - c1, d2 = l[2] - c2;
+        const bool adjacent = c0 != INT_MIN && d0 >= -1 && d0 <= 1 && d1 >= -1 && d1 <= 1 &&
+                              d2 >= -1 && d2 <= 1;
+        if (adjacent) {
+            shift<4>(d0, g, go, lane, basis_j);
+            shift<2>(d1, g, go, lane, basis_j);
+            shift<1>(d2, g, go, lane, basis_j);
+        } else {
+            flush_all(g, go, lane, basis_j);
+        }
+        c0 = l[0];
+        c1 = l[1];
+        c2 = l[2];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) wlk[k] = nlk[k];
+    }
 };
 
 // Backward march of one ray given dL/drgb (gq) and accum = rgb . gq (+ beta
@@ -491,34 +648,7 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     const int color = act ? lane / B : 2;
     const float basis_j = act ? rc.basis[lane % B] : 0.f;
     const float gout = pick3(gq, color);
-    int wl0 = INT_MIN, wl1 = 0, wl2 = 0;
-    int wlk[8];
-    float acc_c[8], acc_d[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        wlk[k] = -1;
-        acc_c[k] = 0.f;
-        acc_d[k] = 0.f;
-    }
-    bool touched = false;
-
-    auto flush = [&]() {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int lk = wlk[k];
-            if (g.valid(lk)) {
-                if (go.gs != nullptr && act)
-                    atomic_add_f32(go.gs + static_cast<int64_t>(lk) * C + lane,
-                                   basis_j * acc_c[k]);
-                if (lane == 0) {
-                    if (go.gd != nullptr) atomic_add_f32(go.gd + lk, acc_d[k]);
-                    if (go.mask != nullptr) go.mask[lk] = 1;
-                }
-            }
-            acc_c[k] = 0.f;
-            acc_d[k] = 0.f;
-        }
-    };
+    GradWindow<C> win;
 
     float t = r.tmin;
     float log_t = 0.f;
@@ -561,29 +691,26 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
                 float grad_sigma = ws * (total * __expf(lt_after) - accum);
                 if (sparsity > 0.f)
                     grad_sigma += sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
-                if (!(l[0] == wl0 && l[1] == wl1 && l[2] == wl2)) {
-                    if (touched) flush();
-                    touched = false;
-                    wl0 = l[0];
-                    wl1 = l[1];
-                    wl2 = l[2];
+                if (!(l[0] == win.c0 && l[1] == win.c1 && l[2] == win.c2)) {
+                    int nlk[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) wlk[k] = __shfl(c.lk[k], s, kGroup);
+                    for (int k = 0; k < 8; ++k) nlk[k] = __shfl(c.lk[k], s, kGroup);
+                    win.move(l, nlk, g, go, lane, basis_j);
                 }
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const float wk = corner_weight(k, p);
-                    acc_c[k] = fmaf(wk, grad_common, acc_c[k]);
-                    acc_d[k] = fmaf(wk, grad_sigma, acc_d[k]);
+                    win.acc_c[k] = fmaf(wk, grad_common, win.acc_c[k]);
+                    win.acc_d[k] = fmaf(wk, grad_sigma, win.acc_d[k]);
                 }
-                touched = true;
+                win.used = 0xFFu;
             }
         }
         log_t = c.log_t_end;
         t = c.next_t;
         more = c.more;
     }
-    if (touched) flush();
+    win.flush_all(g, go, lane, basis_j);
 }
 
 // ---------------------------------------------------------------------------
@@ -869,7 +996,8 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(grad_out != nullptr && color_cache != nullptr,
                   "grad_out / color_cache are NULL");
-    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out};
+    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
+                     grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
@@ -897,7 +1025,8 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     SVOX2_REQUIRE(grads != nullptr, "grads is NULL");
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
-    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out};
+    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
+                     grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const float norm = 2.f / static_cast<float>(3 * n_rays);
     const float beta = beta_loss / static_cast<float>(n_rays);
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);

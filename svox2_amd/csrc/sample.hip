@@ -95,8 +95,8 @@ __global__ __launch_bounds__(kBlock) void k_sample_all(GridView<DT> g,
     for (int k = 0; k < 8; ++k) {
         const bool ok = g.valid(lk[k]);
         v[k] = !ok ? 0.f
-                   : (lane < C ? ld(g.sh, static_cast<int64_t>(lk[k]) * C + lane)
-                               : ld(g.density, lk[k]));
+                   : (lane < C ? ld(g.sh, static_cast<int64_t>(lk[k]) * g.sstride + lane)
+                               : ld(g.density, static_cast<int64_t>(lk[k]) * g.dstride));
     }
     const float r = trilerp8(v, p);
     if (lane < C) out_sh[i * C + lane] = r;
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_sample_density(GridView<DT> g,
     corner_links(g, base, lk);
     float v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = g.valid(lk[k]) ? ld(g.density, lk[k]) : 0.f;
+    for (int k = 0; k < 8; ++k)
+        v[k] = g.valid(lk[k]) ? ld(g.density, static_cast<int64_t>(lk[k]) * g.dstride) : 0.f;
     out_d[i] = trilerp8(v, p);
 }
 
@@ -123,7 +124,7 @@ template <typename DT>
 __global__ __launch_bounds__(kBlock) void k_sample_all_backward(
         GridView<DT> g, const float* __restrict__ pts, int64_t n,
         const float* __restrict__ gout_d, const float* __restrict__ gout_sh,
-        float* __restrict__ grad_d, float* __restrict__ grad_sh) {
+        float* __restrict__ grad_d, float* __restrict__ grad_sh, int64_t gds, int64_t gss) {
     const int64_t i = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     const int C = g.sh_dim;
@@ -138,15 +139,15 @@ __global__ __launch_bounds__(kBlock) void k_sample_all_backward(
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (!g.valid(lk[k])) continue;
-        if (lane < C) atomic_add_f32(grad_sh + static_cast<int64_t>(lk[k]) * C + lane, w[k]);
-        else atomic_add_f32(grad_d + lk[k], w[k]);
+        if (lane < C) atomic_add_f32(grad_sh + static_cast<int64_t>(lk[k]) * gss + lane, w[k]);
+        else atomic_add_f32(grad_d + static_cast<int64_t>(lk[k]) * gds, w[k]);
     }
 }
 
 template <typename DT>
 __global__ __launch_bounds__(kBlock) void k_sample_density_backward(
         GridView<DT> g, const float* __restrict__ pts, int64_t n,
-        const float* __restrict__ gout_d, float* __restrict__ grad_d) {
+        const float* __restrict__ gout_d, float* __restrict__ grad_d, int64_t gds) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     float p[3];
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_density_backward(
     corner_weights(p, gout_d[i], w);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        if (g.valid(lk[k])) atomic_add_f32(grad_d + lk[k], w[k]);
+        if (g.valid(lk[k])) atomic_add_f32(grad_d + static_cast<int64_t>(lk[k]) * gds, w[k]);
 }
 
 template <typename Fn>
@@ -197,7 +198,8 @@ extern "C" int svox2_sample_grid(const svox2_grid_t* grid, const float* points,
 extern "C" int svox2_sample_grid_backward(const svox2_grid_t* grid, const float* points,
                                           int64_t n_points, const float* grad_out_density,
                                           const float* grad_out_sh, float* grad_density_out,
-                                          float* grad_sh_out, int32_t want_colors,
+                                          float* grad_sh_out, int64_t grad_density_stride,
+                                          int64_t grad_sh_stride, int32_t want_colors,
                                           void* stream) {
     int st = check_grid(grid, false);
     if (st) return st;
@@ -207,16 +209,20 @@ extern "C" int svox2_sample_grid_backward(const svox2_grid_t* grid, const float*
                   "points / grad_out_density / grad_density_out are NULL");
     SVOX2_REQUIRE(!want_colors || (grad_out_sh != nullptr && grad_sh_out != nullptr),
                   "grad_out_sh / grad_sh_out are NULL");
+    SVOX2_REQUIRE(grad_density_stride >= 0 && grad_sh_stride >= 0, "negative gradient stride");
+    const int64_t gds = grad_density_stride > 0 ? grad_density_stride : 1;
+    const int64_t gss = grad_sh_stride > 0 ? grad_sh_stride : grid->sh_data_dim;
+    SVOX2_REQUIRE(gss >= grid->sh_data_dim, "grad_sh_stride below sh_data_dim");
     hipStream_t s = static_cast<hipStream_t>(stream);
     with_view(grid, [&](auto g) {
         using DT = typename ViewDT<decltype(g)>::type;
         if (want_colors) {
             k_sample_all_backward<DT><<<blocks_for(n_points * kGroup, kBlock), kBlock, 0, s>>>(
                 g, points, n_points, grad_out_density, grad_out_sh, grad_density_out,
-                grad_sh_out);
+                grad_sh_out, gds, gss);
         } else {
             k_sample_density_backward<DT><<<blocks_for(n_points, kBlock), kBlock, 0, s>>>(
-                g, points, n_points, grad_out_density, grad_density_out);
+                g, points, n_points, grad_out_density, grad_density_out, gds);
         }
     });
     return check_launch("sample_grid_backward");

@@ -137,9 +137,8 @@ def to_sparse_grid(scene: SceneData, device="cuda", dtype=torch.float32, acceler
     N = scene.links.shape
     grid = SparseGrid(reso=list(N), basis_dim=scene.basis_dim, device="cpu", dtype=dtype)
     grid.links = scene.links.to(device)
-    grid.density_data = torch.nn.Parameter(scene.density.to(device=device, dtype=dtype))
-    grid.sh_data = torch.nn.Parameter(scene.sh.to(device=device, dtype=dtype))
-    grid._set_grad_dtypes()
+    grid._set_data(scene.density.to(device=device, dtype=torch.float64),
+                   scene.sh.to(device=device, dtype=torch.float64), dtype, device)
     grid.capacity = grid.sh_data.size(0)
     grid.basis_data = torch.nn.Parameter(grid.basis_data.data.to(device), requires_grad=False)
     grid.background_data = torch.nn.Parameter(grid.background_data.data.to(device),

@@ -202,3 +202,37 @@ def test_image_kernel_matches_ray_kernel(gpu, dense64, march):
     rgb64 = O.render_fp64(g, ro, rd, OPT, torch.from_numpy(g.density),
                           torch.from_numpy(g.sh)).numpy()
     assert np.abs(rgb - rgb64).max() < 2e-5
+
+
+def test_contiguous_and_interleaved_layouts_agree(gpu, dense64):
+    """Separate contiguous density (R,1) / sh (R,27) tensors, as an external
+    caller passes them, render and differentiate exactly as the package's
+    interleaved row layout."""
+    _, o, d, gt, sc = dense64
+    grid = _grid(sc, gpu)
+    assert grid.sh_data.stride(0) == 32 and grid.density_data.stride(0) == 32
+    spec = grid._to_cpp()
+    spec.density_data = grid.density_data.data.contiguous()
+    spec.sh_data = grid.sh_data.data.contiguous()
+    rays = svox2_amd.Rays(torch.from_numpy(o[:1024]).to(gpu),
+                          torch.from_numpy(d[:1024]).to(gpu))._to_cpp()
+    opt = grid.opt._to_cpp()
+    a = _C.volume_render_cuvol(grid._to_cpp(), rays, opt)
+    b = _C.volume_render_cuvol(spec, rays, opt)
+    assert torch.equal(a, b)
+    gt_t = torch.from_numpy(gt[:1024]).to(gpu)
+    outs = []
+    for sp, interleaved in ((grid._to_cpp(), True), (spec, False)):
+        h = _C.GridOutputGrads()
+        if interleaved:
+            gdv, gsv = svox2_amd.svox2.interleaved_rows(sp.sh_data.size(0), 27, torch.float32, gpu)
+        else:
+            gdv = torch.zeros(sp.sh_data.size(0), 1, device=gpu)
+            gsv = torch.zeros(sp.sh_data.size(0), 27, device=gpu)
+        h.grad_density_out, h.grad_sh_out = gdv, gsv
+        out = torch.empty_like(a)
+        _C.volume_render_cuvol_fused(sp, rays, opt, gt_t, 0.0, 0.0, out, h)
+        outs.append((out, gdv.contiguous(), gsv.contiguous()))
+    assert torch.equal(outs[0][0], outs[1][0])
+    for x, y in zip(outs[0][1:], outs[1][1:]):
+        assert (x - y).abs().max() <= 1e-6 * y.abs().max()
