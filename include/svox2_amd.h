@@ -130,8 +130,11 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                        void* stream);
 
 /* Forward render plus the gradient of mean((rgb - rgb_gt)^2) (over 3n terms),
- * with optional beta and Cauchy sparsity terms, in one launch; writes rgb_out
- * and accumulates into grads.
+ * with optional beta and Cauchy sparsity terms: a forward launch writing
+ * rgb_out, then a backward launch that forms the MSE gradient from it and
+ * accumulates into grads.  log_transmit_out (n floats, may be NULL unless
+ * beta_loss > 0) receives each ray's final log transmittance, which the beta
+ * term needs.
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
@@ -139,6 +142,7 @@ int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const svox2_render_options_t* opt,
                                     const float* rgb_gt, float beta_loss,
                                     float sparsity_loss, float* rgb_out,
+                                    float* log_transmit_out,
                                     const svox2_grads_t* grads, void* stream);
 
 /* Per-ray march counters [n][4] (int32): loop iterations, iterations not

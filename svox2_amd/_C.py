@@ -123,7 +123,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_volume_render_cuvol": [G, vp, vp, vp, i64, O, vp, vp, vp],
         "svox2_volume_render_cuvol_image": [G, Cm, O, vp, vp, vp],
         "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
-        "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, Gr, vp],
+        "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp, Gr,
+                                            vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
         "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
@@ -483,10 +484,12 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     if rgb_gt.shape != o.shape or rgb_out.shape != o.shape:
         raise RuntimeError("rgb_gt / rgb_out must be (n, 3)")
     gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
+             if beta_loss > 0 else None)
     with _on_device(o):
         _call("svox2_volume_render_cuvol_fused", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
               o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(rgb_gt), float(beta_loss),
-              float(sparsity_loss), _ptr(rgb_out), ctypes.byref(gs), _stream(o))
+              float(sparsity_loss), _ptr(rgb_out), _ptr(log_t), ctypes.byref(gs), _stream(o))
 
 
 def sample_grid(grid: SparseGridSpec, points: torch.Tensor,

@@ -518,125 +518,83 @@ struct GradOut {
     int64_t gss;
 };
 
-// Channel-parallel gradient accumulators for the 8 corners of the cell the
-// ray is in.  Moving to an adjacent cell keeps the 4 (or 2, or 1) corners the
-// two cells share and flushes only the corners left behind, so each grid point
-// is flushed once per visit.  A flush is one no-return atomic instruction per
-// corner: lanes 0..C-1 add the SH channels and lane C the density (the same
-// 128-byte row in the interleaved layout); lane C+1 marks the row.
-template <int C>
-struct GradWindow {
-    int c0 = INT_MIN, c1 = 0, c2 = 0;
-    int wlk[8];
-    float acc_c[8];
-    float acc_d[8];
-    uint32_t used = 0;
-
-    __device__ __forceinline__ GradWindow() {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            wlk[k] = -1;
-            acc_c[k] = 0.f;
-            acc_d[k] = 0.f;
-        }
-    }
+// Gradient scatter state of one ray.  Lane L of the 32-lane group holds the
+// accumulator of (corner k = L & 7, quantity q = L >> 3) of the cell the ray is
+// in: q = 0, 1, 2 the colour weight factor of corner k for R, G, B (the SH
+// gradient of channel (c, j) is that factor times basis_j) and q = 3 the
+// density gradient.  Consecutive samples in the same cell add into the same
+// accumulators; moving to an adjacent cell keeps the corners the two cells
+// share (a lane permutation) and flushes only the corners left behind, so a
+// grid point is flushed once per visit.  A flush of corner k is one no-return
+// atomic instruction with lane = SH channel (lanes 0..C-1) plus the density
+// (lane C): the same 128-byte row in the interleaved layout.
+template <int B>
+struct Scatter {
+    static constexpr int C = 3 * B;
+    int cur = -1;        // flat index of the current cell (-1: none)
+    int c0 = 0, c1 = 0, c2 = 0;
+    float acc = 0.f;     // this lane's (k, q) accumulator
+    int link = -1;       // link of corner k (identical for the 4 q lanes)
+    uint32_t used = 0;   // corners holding gradient
 
     template <typename DT>
-    __device__ __forceinline__ void flush_slot(const GridView<DT>& g, const GradOut& go, int lane,
-                                               float basis_j, int k) {
-        const int lk = wlk[k];
+    __device__ __forceinline__ void flush_corner(const GridView<DT>& g, const GradOut& go,
+                                                 int lane, float basis_j, int k) const {
+        const int lk = __shfl(link, k, kGroup);
+        const int src = lane < C ? k + 8 * (lane / B) : k + 24;
+        float v = __shfl(acc, src, kGroup);
         if (g.valid(lk)) {
-            float* ptr = nullptr;
-            float val = 0.f;
             if (lane < C) {
-                if (go.gs != nullptr) {
-                    ptr = go.gs + static_cast<int64_t>(lk) * go.gss + lane;
-                    val = basis_j * acc_c[k];
-                }
+                if (go.gs != nullptr)
+                    atomic_add_f32(go.gs + static_cast<int64_t>(lk) * go.gss + lane, basis_j * v);
             } else if (lane == C) {
-                if (go.gd != nullptr) {
-                    ptr = go.gd + static_cast<int64_t>(lk) * go.gds;
-                    val = acc_d[k];
-                }
+                if (go.gd != nullptr) atomic_add_f32(go.gd + static_cast<int64_t>(lk) * go.gds, v);
             }
-            if (ptr != nullptr) atomic_add_f32(ptr, val);
-            if (lane == C + 1 && go.mask != nullptr) go.mask[lk] = 1;
         }
-        acc_c[k] = 0.f;
-        acc_d[k] = 0.f;
     }
 
+    // Flushes the corners in `which` (bit k = corner k) and marks their rows.
     template <typename DT>
-    __device__ __forceinline__ void flush_all(const GridView<DT>& g, const GradOut& go, int lane,
-                                              float basis_j) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (used & (1u << k)) flush_slot(g, go, lane, basis_j, k);
-        used = 0;
+    __device__ __forceinline__ void flush(const GridView<DT>& g, const GradOut& go, int lane,
+                                          float basis_j, uint32_t which) {
+        which &= used;
+        if (which == 0) return;
+        for (uint32_t m = which; m; m &= m - 1) flush_corner(g, go, lane, basis_j, __ffs(m) - 1);
+        if (go.mask != nullptr && lane < 8 && ((which >> lane) & 1u) && g.valid(link))
+            go.mask[link] = 1;
+        used &= ~which;
     }
 
-    // Shift along one axis (corner bit `bit`) by d in {-1, 0, 1}.
+    // Shift by d in {-1, 0, 1} along the axis of corner bit `bit`.
     template <int bit, typename DT>
     __device__ __forceinline__ void shift(int d, const GridView<DT>& g, const GradOut& go,
                                           int lane, float basis_j) {
-        constexpr uint32_t lo_mask = bit == 4 ? 0x0Fu : (bit == 2 ? 0x33u : 0x55u);
+        if (d == 0) return;
+        constexpr uint32_t lo = bit == 4 ? 0x0Fu : (bit == 2 ? 0x33u : 0x55u);
+        const bool hi_lane = (lane & bit) != 0;
+        flush(g, go, lane, basis_j, d > 0 ? lo : (~lo & 0xFFu));
+        const float a = __shfl_xor(acc, bit, kGroup);
+        const int lk = __shfl_xor(link, bit, kGroup);
         if (d > 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k & bit) continue;
-                if (used & (1u << k)) flush_slot(g, go, lane, basis_j, k);
-                acc_c[k] = acc_c[k | bit];
-                acc_d[k] = acc_d[k | bit];
-                wlk[k] = wlk[k | bit];
-                acc_c[k | bit] = 0.f;
-                acc_d[k | bit] = 0.f;
-            }
-            used = (used >> bit) & lo_mask;
-        } else if (d < 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k & bit) continue;
-                if (used & (1u << (k | bit))) flush_slot(g, go, lane, basis_j, k | bit);
-                acc_c[k | bit] = acc_c[k];
-                acc_d[k | bit] = acc_d[k];
-                wlk[k | bit] = wlk[k];
-                acc_c[k] = 0.f;
-                acc_d[k] = 0.f;
-            }
-            used = (used & lo_mask) << bit;
-        }
-    }
-
-    // Makes cell l with corner links nlk current.
-    template <typename DT>
-    __device__ __forceinline__ void move(const int* l, const int* nlk, const GridView<DT>& g,
-                                         const GradOut& go, int lane, float basis_j) {
-        if (l[0] == c0 && l[1] == c1 && l[2] == c2) return;
-        const int d0 = l[0] - c0, d1 = l[1] - c1, d2 = l[2] - c2;
-        const bool adjacent = c0 != INT_MIN && d0 >= -1 && d0 <= 1 && d1 >= -1 && d1 <= 1 &&
-                              d2 >= -1 && d2 <= 1;
-        if (adjacent) {
-            shift<4>(d0, g, go, lane, basis_j);
-            shift<2>(d1, g, go, lane, basis_j);
-            shift<1>(d2, g, go, lane, basis_j);
+            acc = hi_lane ? 0.f : a;
+            link = hi_lane ? -1 : lk;
+            used = (used >> bit) & lo;
         } else {
-            flush_all(g, go, lane, basis_j);
+            acc = hi_lane ? a : 0.f;
+            link = hi_lane ? lk : -1;
+            used = (used & lo) << bit;
         }
-        c0 = l[0];
-        c1 = l[1];
-        c2 = l[2];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) wlk[k] = nlk[k];
     }
 };
 
 // Backward march of one ray given dL/drgb (gq) and accum = rgb . gq (+ beta
 // term); accumulates into the gradient buffers (trace_ray_cuvol_backward,
 // render_lerp_kernel_cuvol.cu:227-383).  Samples are rematerialised chunk by
-// chunk exactly as in the forward; each live sample's gradient is then added,
-// in sample order and with lane = SH channel, into per-corner register
-// accumulators of the current cell, which are flushed with one coalesced
-// no-return atomic per (corner, channel) when the ray leaves the cell.
+// chunk exactly as in the forward.  Each live sample's gradient scalars (the
+// colour factors weight * [colour >= 0] * dL/dcolour and the density gradient
+// ws * (total * T_after - accum)) are formed sample-parallel, with the running
+// `accum` of the reference as a prefix sum over the chunk's samples, then
+// added in sample order into the (corner, quantity) accumulators.
 template <typename DT, int B>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
@@ -644,11 +602,10 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
                                                const GradOut& go) {
     if (r.tmin > r.tmax) return;
     constexpr int C = 3 * B;
-    const bool act = lane < C;
-    const int color = act ? lane / B : 2;
-    const float basis_j = act ? rc.basis[lane % B] : 0.f;
-    const float gout = pick3(gq, color);
-    GradWindow<C> win;
+    const float basis_j = lane < C ? rc.basis[lane % B] : 0.f;
+    const int my_k = lane & 7;
+    const int my_q = lane >> 3;
+    Scatter<B> sc;
 
     float t = r.tmin;
     float log_t = 0.f;
@@ -658,59 +615,89 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
         Chunk c;
         march_chunk(g, o, r, lane, t, log_t, c);
         const int n_live = __popc(c.live_mask);
-        for (int pass = 0; pass * 4 < n_live; ++pass) {
-            float col[3];
-            int src;
-            bool has;
-            chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
-            // Apply the pass's samples in sample order, lane = channel.
-            const int n_here = min(4, n_live - pass * 4);
-            for (int qq = 0; qq < n_here; ++qq) {
-                const int s = nth_set_bit(c.live_mask, pass * 4 + qq);
-                float cs[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) cs[a] = __shfl(col[a], qq * 8, kGroup) + 0.5f;
-                const float weight = __shfl(c.weight, s, kGroup);
-                const float lt_after = __shfl(c.lt_after, s, kGroup);
-                const float ws = __shfl(c.ws, s, kGroup);
-                const float sigma = __shfl(c.sigma, s, kGroup);
-                float p[3];
-                int l[3];
+        if (n_live > 0) {
+            // Colours of the live samples, moved to each sample's own lane.
+            const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
+            float cs[3] = {0.f, 0.f, 0.f};
+            for (int pass = 0; pass * 4 < n_live; ++pass) {
+                float col[3];
+                int src;
+                bool has;
+                chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
+                const int from = ((rank - pass * 4) & 3) * 8;
+                const bool mine = c.live && rank >= pass * 4 && rank < pass * 4 + 4;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    p[a] = __shfl(c.p[a], s, kGroup);
-                    l[a] = __shfl(c.l[a], s, kGroup);
+                    const float v = __shfl(col[a], from, kGroup);
+                    cs[a] = mine ? v + 0.5f : cs[a];
                 }
-                float total = 0.f;
+            }
+            // Per-sample gradient scalars.
+            float total = 0.f;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * gq[a];
-                const float mine = pick3(cs, color);
-                const float in01 = (fmaxf(mine, 0.f) == mine) ? 1.f : 0.f;
-                const float grad_common = weight * in01 * gout;
-                accum -= weight * total;
-                float grad_sigma = ws * (total * __expf(lt_after) - accum);
-                if (sparsity > 0.f)
-                    grad_sigma += sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
-                if (!(l[0] == win.c0 && l[1] == win.c1 && l[2] == win.c2)) {
-                    int nlk[8];
+            for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * gq[a];
+            float run = c.live ? c.weight * total : 0.f;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) nlk[k] = __shfl(c.lk[k], s, kGroup);
-                    win.move(l, nlk, g, go, lane, basis_j);
+            for (int off = 1; off < 32; off <<= 1) {
+                const float y = __shfl_up(run, off, kGroup);
+                if (lane >= off) run += y;
+            }
+            const float acc_s = accum - run;
+            accum -= __shfl(run, 31, kGroup);
+            float gsig = c.ws * (total * __expf(c.lt_after) - acc_s);
+            if (sparsity > 0.f) gsig += sparsity * (4.f * c.sigma / (1.f + 2.f * (c.sigma * c.sigma)));
+            float gc[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                gc[a] = c.weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * gq[a];
+            // Accumulate in sample order.
+            for (uint32_t m = c.live_mask; m; m &= m - 1) {
+                const int s = __ffs(m) - 1;
+                const int cell = __shfl(c.base, s, kGroup);
+                if (cell != sc.cur) {
+                    int l[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) l[a] = __shfl(c.l[a], s, kGroup);
+                    const int d0 = l[0] - sc.c0, d1 = l[1] - sc.c1, d2 = l[2] - sc.c2;
+                    const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 &&
+                                          d1 <= 1 && d2 >= -1 && d2 <= 1;
+                    if (adjacent) {
+                        sc.template shift<4>(d0, g, go, lane, basis_j);
+                        sc.template shift<2>(d1, g, go, lane, basis_j);
+                        sc.template shift<1>(d2, g, go, lane, basis_j);
+                    } else {
+                        sc.flush(g, go, lane, basis_j, 0xFFu);
+                        sc.acc = 0.f;
+                    }
+                    int nl = __shfl(c.lk[0], s, kGroup);
+#pragma unroll
+                    for (int kk = 1; kk < 8; ++kk) {
+                        const int v = __shfl(c.lk[kk], s, kGroup);
+                        nl = my_k == kk ? v : nl;
+                    }
+                    sc.link = nl;
+                    sc.cur = cell;
+                    sc.c0 = l[0];
+                    sc.c1 = l[1];
+                    sc.c2 = l[2];
                 }
+                float p[3];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const float wk = corner_weight(k, p);
-                    win.acc_c[k] = fmaf(wk, grad_common, win.acc_c[k]);
-                    win.acc_d[k] = fmaf(wk, grad_sigma, win.acc_d[k]);
-                }
-                win.used = 0xFFu;
+                for (int a = 0; a < 3; ++a) p[a] = __shfl(c.p[a], s, kGroup);
+                const float v0 = __shfl(gc[0], s, kGroup);
+                const float v1 = __shfl(gc[1], s, kGroup);
+                const float v2 = __shfl(gc[2], s, kGroup);
+                const float v3 = __shfl(gsig, s, kGroup);
+                const float val = my_q == 0 ? v0 : (my_q == 1 ? v1 : (my_q == 2 ? v2 : v3));
+                sc.acc = fmaf(corner_weight(my_k, p), val, sc.acc);
+                sc.used = 0xFFu;
             }
         }
         log_t = c.log_t_end;
         t = c.next_t;
         more = c.more;
     }
-    win.flush_all(g, go, lane, basis_j);
+    sc.flush(g, go, lane, basis_j, 0xFFu);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,15 +803,17 @@ __global__ __launch_bounds__(kBlock) void k_render_backward(
     march_backward<DT, B>(g, o, r, rc, lane, gq, accum, 0.f, go);
 }
 
-// Forward then backward of the same ray in one pass of the workgroup: the
-// forward colour stays in registers and feeds the MSE gradient
-// g = (rgb - gt) * 2 / (3n) (render_ray_backward_kernel, :735-741).
+// Backward of the fused MSE step: dL/drgb = (rgb - gt) * 2 / (3n) from the
+// forward colours (render_ray_backward_kernel with grad_out_is_rgb,
+// render_lerp_kernel_cuvol.cu:735-741), plus the beta term from the forward's
+// final log transmittance (:257-262) and the Cauchy sparsity term (:345-348).
 template <typename DT, int B>
-__global__ __launch_bounds__(kBlock) void k_render_fused(
+__global__ __launch_bounds__(kBlock) void k_render_backward_mse(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
-        const float* __restrict__ rgb_gt, float beta_per_ray, float sparsity,
-        float norm, float* __restrict__ rgb_out, GradOut go) {
+        const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
+        const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
+        GradOut go) {
     const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (ray >= n) return;
@@ -833,15 +822,15 @@ __global__ __launch_bounds__(kBlock) void k_render_fused(
     const RayCtx<B> rc(wd);
     Ray r;
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
-    float rgb[3];
-    const float log_t = march_forward<DT, B, false>(g, o, r, rc, lane, rgb, nullptr);
-    if (lane < 3) rgb_out[3 * ray + lane] = pick3(rgb, lane);
-    float gq[3];
+    float c3[3], gq[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) gq[c] = (rgb[c] - rgb_gt[3 * ray + c]) * norm;
-    float accum = fmaf(rgb[0], gq[0], fmaf(rgb[1], gq[1], rgb[2] * gq[2]));
+    for (int c = 0; c < 3; ++c) {
+        c3[c] = rgb[3 * ray + c];
+        gq[c] = (c3[c] - rgb_gt[3 * ray + c]) * norm;
+    }
+    float accum = fmaf(c3[0], gq[0], fmaf(c3[1], gq[1], c3[2] * gq[2]));
     if (beta_per_ray > 0.f) {
-        const float tin = __expf(log_t);
+        const float tin = __expf(log_t_fwd[ray]);
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
     march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go);
@@ -1016,6 +1005,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                                const svox2_render_options_t* opt,
                                                const float* rgb_gt, float beta_loss,
                                                float sparsity_loss, float* rgb_out,
+                                               float* log_transmit_out,
                                                const svox2_grads_t* grads, void* stream) {
     int st = check_grid(grid, true);
     if (st) return st;
@@ -1025,17 +1015,23 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     SVOX2_REQUIRE(grads != nullptr, "grads is NULL");
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
+    SVOX2_REQUIRE(!(beta_loss > 0.f) || log_transmit_out != nullptr,
+                  "beta_loss > 0 needs log_transmit_out (n floats)");
     const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
                      grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const float norm = 2.f / static_cast<float>(3 * n_rays);
     const float beta = beta_loss / static_cast<float>(n_rays);
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    float* lt = beta_loss > 0.f ? log_transmit_out : nullptr;
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
-        hipLaunchKernelGGL((k_render_fused<DT, B>), dim3(blocks), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
+                           as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
+                           (int32_t*)nullptr);
+        hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_gt,
-                           beta, sparsity_loss, norm, rgb_out, go);
+                           rgb_out, lt, beta, sparsity_loss, norm, go);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");
