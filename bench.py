@@ -178,7 +178,8 @@ def cpu_baseline(wl: Workload, budget_s: float = 12.0):
     """The scalar C port of the reference's fused forward + MSE backward
     (oracle/march_ref.c: trace_ray_cuvol and trace_ray_cuvol_backward, one
     thread, every contributing sample scattered into its 8 corner rows) on
-    this host, over the first rays of a benchmark batch of the same grid."""
+    this host, over whole 65536-ray batches of the same grid until about
+    `budget_s` seconds of CPU time are spent."""
     import oracle.oracle as O
     O.build(force=False)
     grid = wl.grid(torch.float32)
@@ -187,24 +188,39 @@ def cpu_baseline(wl: Workload, budget_s: float = 12.0):
                 grid.sh_data.data.cpu().double().numpy(), 9, offset=0.5 * N - 0.5,
                 scaling=0.5 * N)
     opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
-    o, d = wl.cams.random_batch(BATCH, torch.Generator().manual_seed(123))
-    gt = torch.rand(BATCH, 3, generator=torch.Generator().manual_seed(124))
     march = O.march()
-
-    def run(n):
+    gen = torch.Generator().manual_seed(123)
+    rays, spent, n_batches = 0, 0.0, 0
+    while spent < budget_s and n_batches < 64:
+        o, d = wl.cams.random_batch(BATCH, gen)
+        gt = torch.rand(BATCH, 3, generator=gen)
         t = time.perf_counter()
-        march.fused(hg, o[:n].numpy(), d[:n].numpy(), opt, gt[:n].numpy())
-        return time.perf_counter() - t
+        march.fused(hg, o.numpy(), d.numpy(), opt, gt.numpy())
+        spent += time.perf_counter() - t
+        rays += BATCH
+        n_batches += 1
+        log(f"CPU baseline: batch {n_batches}, {rays} rays in {spent:.2f}s")
+    return dict(value=rays / spent, unit="rays/s", cores=1, kind="port",
+                sample=f"{n_batches} random 65536-ray batches on the same 512^3 grid "
+                       f"({rays} rays, {spent:.1f}s): scalar C port of the reference fused "
+                       f"forward+backward (one thread, float32)")
 
-    n = 256
-    dt = run(n)
-    n = int(min(BATCH, max(256, n * budget_s / max(dt, 1e-3))))
-    dt = run(n)
-    log(f"CPU baseline: {n} rays in {dt:.2f}s")
-    return dict(value=n / dt, unit="rays/s", cores=1, kind="port",
-                sample=f"first {n} rays of a 65536-ray batch on the same 512^3 grid: scalar C "
-                       f"port of the reference fused forward+backward (one thread, float32), "
-                       f"{dt:.2f}s")
+
+def pmc_run(reps: int = 3):
+    """A short run for hardware-counter passes: `reps` fused calls on one
+    65536-ray batch of the 512^3 grid, then a 1 GiB device copy whose
+    FETCH_SIZE / WRITE_SIZE calibrate the counters (2^30 bytes each way)."""
+    dev = torch.device("cuda:0")
+    wl = Workload(dev)
+    batches = wl.batches(BATCH, 1)
+    g32 = wl.grid(torch.float32)
+    kt = kernel_times(g32, batches, reps)
+    x = torch.empty(2 ** 28, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_run": {"fused_calls": reps, "fused_ms": [t for t, _ in kt],
+                                  "copy_bytes": 2 ** 30}}), flush=True)
 
 
 def main():
@@ -214,6 +230,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline measurement only")
+    ap.add_argument("--pmc", action="store_true",
+                    help="short run for rocprofv3 counter passes (3 fused calls + 1 GiB copy)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the renderer runs on a single device "
@@ -225,6 +243,9 @@ def main():
         sys.exit(2)
     from svox2_amd import _C
     _C.load_library()
+    if args.pmc:
+        pmc_run()
+        return
     dev = torch.device("cuda:0")
 
     wl = Workload(dev)
@@ -287,7 +308,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_render_fused (float32 storage, SH-2)",
+            "kernel": "fused call (float32 storage, SH-2): k_render_rays, ray-order sort "
+                      "(k_zero_words, k_cost_hist, k_cost_scan, k_cost_scatter), "
+                      "k_render_backward_mse",
             "kernel_ms": head["kernel_ms"],
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
             "counters_per_launch": head["counters"],

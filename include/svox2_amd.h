@@ -134,7 +134,13 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  * rgb_out, then a backward launch that forms the MSE gradient from it and
  * accumulates into grads.  log_transmit_out (n floats, may be NULL unless
  * beta_loss > 0) receives each ray's final log transmittance, which the beta
- * term needs.
+ * term needs.  With a device workspace of at least
+ * svox2_volume_render_cuvol_fused_workspace_bytes(n_rays) bytes and a batch
+ * of at least four times the rays the device holds resident (40960 on a
+ * 256-CU MI355X), the forward records each ray's chunk count and the backward
+ * processes the rays in descending order of it (same results; the rays of a
+ * wave are then of similar length and the longest start first).  Otherwise,
+ * or with workspace NULL, the backward takes the rays in input order.
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
@@ -143,7 +149,14 @@ int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* rgb_gt, float beta_loss,
                                     float sparsity_loss, float* rgb_out,
                                     float* log_transmit_out,
-                                    const svox2_grads_t* grads, void* stream);
+                                    const svox2_grads_t* grads,
+                                    void* workspace, size_t workspace_bytes,
+                                    void* stream);
+
+/* Device workspace bytes svox2_volume_render_cuvol_fused uses to order the
+ * rays of an n_rays batch (0 for an empty batch).  No counterpart in the
+ * reference. */
+size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
 /* Per-ray march counters [n][4] (int32): loop iterations, iterations not
  * skipped by the empty-space skip, samples with sigma > sigma_thresh, and a

@@ -41,6 +41,7 @@ INDEXER_ALL, INDEXER_MASK, INDEXER_INDEX = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_image",
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
+    "svox2_volume_render_cuvol_fused_workspace_bytes",
     "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
@@ -124,7 +125,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_volume_render_cuvol_image": [G, Cm, O, vp, vp, vp],
         "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
         "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp, Gr,
-                                            vp],
+                                            vp, ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
         "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
@@ -138,6 +139,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = ctypes.c_int
+    lib.svox2_volume_render_cuvol_fused_workspace_bytes.argtypes = [i64]
+    lib.svox2_volume_render_cuvol_fused_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
     lib.svox2_accel_dist_prop_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_last_error.restype = ctypes.c_char_p
@@ -475,8 +478,13 @@ def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: Rend
 
 def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
                               rgb_gt: torch.Tensor, beta_loss: float, sparsity_loss: float,
-                              rgb_out: torch.Tensor, grads: GridOutputGrads) -> None:
-    """volume_render_cuvol_fused (render_lerp_kernel_cuvol.cu:1070-1152)."""
+                              rgb_out: torch.Tensor, grads: GridOutputGrads,
+                              ordered: bool = True) -> None:
+    """volume_render_cuvol_fused (render_lerp_kernel_cuvol.cu:1070-1152).
+
+    ordered: run the backward over the rays in descending order of the work
+    the forward measured (same results, shorter tail); False keeps input order.
+    """
     g = _grid_struct(grid)
     o, d, clip = _rays(rays)
     _check(rgb_gt, "rgb_gt", (torch.float32,), 2)
@@ -486,10 +494,15 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     gs = _grads_struct(grads, g.rows, g.sh_data_dim)
     log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
              if beta_loss > 0 else None)
+    # Workspace for the backward's ray ordering (from torch's caching allocator).
+    nbytes = (load_library().svox2_volume_render_cuvol_fused_workspace_bytes(o.size(0))
+              if ordered else 0)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
         _call("svox2_volume_render_cuvol_fused", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
               o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(rgb_gt), float(beta_loss),
-              float(sparsity_loss), _ptr(rgb_out), _ptr(log_t), ctypes.byref(gs), _stream(o))
+              float(sparsity_loss), _ptr(rgb_out), _ptr(log_t), ctypes.byref(gs), _ptr(ws),
+              nbytes, _stream(o))
 
 
 def sample_grid(grid: SparseGridSpec, points: torch.Tensor,

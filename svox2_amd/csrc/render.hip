@@ -10,10 +10,19 @@
 // time with lane = (sample, corner): each lane reads one contiguous SH row and
 // projects it on the SH basis; the 8 corners are summed with lane shuffles.
 //
-// The backward pass rematerialises the same chunks and applies each live
-// sample in order with lane = SH channel, accumulating every corner's
-// gradient in registers while the ray stays in a cell and issuing one
-// coalesced no-return atomic per (corner, channel) when it leaves.
+// The backward pass rematerialises the same chunks, forms every live sample's
+// gradient scalars sample-parallel (the reference's running `accum` becomes a
+// prefix sum over the chunk), and adds them in sample order into registers
+// with lane = (corner, R/G/B/density): a ray that stays in a cell, or moves to
+// a neighbour (the shared corners are kept, a lane permutation), accumulates
+// without memory traffic; a corner it leaves is flushed with one no-return
+// atomic instruction covering the corner's 27 SH channels and its density
+// (one 128-byte row in the interleaved layout).
+//
+// Launches use 64-thread workgroups (one wavefront), so the hardware refills a
+// CU as soon as any wavefront's two rays end.  The fused call records each
+// ray's chunk count in the forward and, for large batches, runs the backward
+// over the rays in descending order of it.
 //
 // Semantics follow the reference kernels: trace_ray_cuvol
 // (svox2/csrc/render_lerp_kernel_cuvol.cu:30-120), trace_ray_cuvol_backward
@@ -32,7 +41,10 @@ namespace svox2amd {
 namespace {
 
 constexpr int kGroup = 32;    // lanes per ray
-constexpr int kBlock = 256;   // threads per workgroup
+#ifndef SVOX2_MARCH_BLOCK
+#define SVOX2_MARCH_BLOCK 64
+#endif
+constexpr int kBlock = SVOX2_MARCH_BLOCK;  // threads per workgroup
 constexpr int kMaxIters = 1 << 22;  // hard per-ray iteration cap (termination guard)
 
 struct OptView {
@@ -448,14 +460,32 @@ __device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx
     col[2] = v2;
 }
 
+// Ray ordering for the backward launch.  A ray's backward costs close to a
+// fixed amount per chunk (the dependent link, density and row round trips),
+// so the forward records its chunk count and, for batches several times
+// larger than the rays the device holds at once, the backward starts the rays
+// with the most chunks first: the rays of a wave are then of similar length
+// (no half-wave idles behind a long ray) and the longest rays no longer finish
+// last on an otherwise idle device.  Smaller batches run in one round, where
+// pairing the longest rays in the same waves only slows them, and keep the
+// input order.
+constexpr int kCostBins = 1024;
+
+__device__ __forceinline__ uint32_t backward_cost(uint32_t n_chunks) {
+    return n_chunks < static_cast<uint32_t>(kCostBins) ? n_chunks
+                                                       : static_cast<uint32_t>(kCostBins - 1);
+}
+
 // Forward march of one ray.  out[3] = composited colour (all lanes),
 // returns the final log transmittance.
 template <typename DT, int B, bool STATS>
 __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
-                                               float* out, int32_t* stats) {
+                                               float* out, int32_t* stats,
+                                               uint32_t* cost = nullptr) {
     float acc[3] = {0.f, 0.f, 0.f};  // this lane's share of the composited colour
     uint32_t n_iter = 0, n_eval = 0, n_hit = 0;
+    uint32_t n_chunks = 0;
     Fnv fnv;
     float log_t = 0.f;
     if (r.tmin <= r.tmax) {
@@ -466,6 +496,7 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
             Chunk c;
             march_chunk(g, o, r, lane, t, log_t, c);
             const int n_live = __popc(c.live_mask);
+            ++n_chunks;
             for (int pass = 0; pass * 4 < n_live; ++pass) {
                 float col[3];
                 int src;
@@ -500,6 +531,7 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     const float tb = __expf(log_t) * o.bg;
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = __shfl(acc[a], 0, kGroup) + tb;
+    if (cost != nullptr) *cost = backward_cost(n_chunks);
     if (STATS && lane == 0) {
         fnv.mix(__float_as_uint(log_t));
         stats[0] = static_cast<int32_t>(n_iter);
@@ -704,12 +736,26 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
 // Kernels
 // ---------------------------------------------------------------------------
 
+// Minimum waves per SIMD the march kernels are compiled for (the register
+// budget is 512 / waves VGPRs).  The marches are latency bound: occupancy
+// hides the dependent link -> row loads better than unrolled passes do.
+#ifndef SVOX2_FWD_WAVES
+#define SVOX2_FWD_WAVES 5
+#endif
+#ifndef SVOX2_BWD_WAVES
+#define SVOX2_BWD_WAVES 4
+#endif
+#define SVOX2_FWD_KERNEL __global__ __launch_bounds__(kBlock) \
+    __attribute__((amdgpu_waves_per_eu(SVOX2_FWD_WAVES)))
+#define SVOX2_BWD_KERNEL __global__ __launch_bounds__(kBlock) \
+    __attribute__((amdgpu_waves_per_eu(SVOX2_BWD_WAVES)))
+
 template <typename DT, int B, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_render_rays(
+SVOX2_FWD_KERNEL void k_render_rays(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         float* __restrict__ rgb_out, float* __restrict__ log_t_out,
-        int32_t* __restrict__ stats) {
+        int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out) {
     const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (ray >= n) return;
@@ -719,11 +765,14 @@ __global__ __launch_bounds__(kBlock) void k_render_rays(
     Ray r;
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float out[3];
+    uint32_t cost = 0;
     const float log_t = march_forward<DT, B, STATS>(g, o, r, rc, lane, out,
-                                                    STATS ? stats + 4 * ray : nullptr);
+                                                    STATS ? stats + 4 * ray : nullptr,
+                                                    cost_out != nullptr ? &cost : nullptr);
     if (!STATS) {
         if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
         if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
+        if (cost_out != nullptr && lane == 0) cost_out[ray] = cost;
     }
 }
 
@@ -763,7 +812,7 @@ __device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, in
 }
 
 template <typename DT, int B>
-__global__ __launch_bounds__(kBlock) void k_render_image(
+SVOX2_FWD_KERNEL void k_render_image(
         GridView<DT> g, OptView o, svox2_camera_t cam, float* __restrict__ rgb_out,
         float* __restrict__ log_t_out) {
     const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
@@ -784,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_render_image(
 }
 
 template <typename DT, int B>
-__global__ __launch_bounds__(kBlock) void k_render_backward(
+SVOX2_BWD_KERNEL void k_render_backward(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ grad_out, const float* __restrict__ color_cache,
@@ -808,15 +857,16 @@ __global__ __launch_bounds__(kBlock) void k_render_backward(
 // render_lerp_kernel_cuvol.cu:735-741), plus the beta term from the forward's
 // final log transmittance (:257-262) and the Cauchy sparsity term (:345-348).
 template <typename DT, int B>
-__global__ __launch_bounds__(kBlock) void k_render_backward_mse(
+SVOX2_BWD_KERNEL void k_render_backward_mse(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
         const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
-        GradOut go) {
-    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+        const uint32_t* __restrict__ order, GradOut go) {
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
-    if (ray >= n) return;
+    if (slot >= n) return;
+    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
@@ -834,6 +884,113 @@ __global__ __launch_bounds__(kBlock) void k_render_backward_mse(
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
     march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go);
+}
+
+// Counting sort of the rays by descending cost: histogram, scan, scatter.
+// The workspace holds [hist | offsets] (kCostBins words each), the per-ray
+// costs and the resulting order (n words each).
+constexpr int kSortBlock = 256;
+
+__global__ __launch_bounds__(kSortBlock) void k_cost_hist(const uint32_t* __restrict__ cost,
+                                                          int64_t n, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kCostBins];
+    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) h[b] = 0u;
+    __syncthreads();
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kSortBlock)
+        atomicAdd(&h[cost[i]], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock)
+        if (h[b] != 0u) atomicAdd(&hist[b], h[b]);
+}
+
+// offsets[b] = number of rays with a cost above b (one block of kCostBins).
+__global__ __launch_bounds__(kCostBins) void k_cost_scan(const uint32_t* __restrict__ hist,
+                                                         uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t a[kCostBins];
+    const int t = threadIdx.x;
+    const int b = kCostBins - 1 - t;  // position t holds bin b: descending bins
+    a[t] = hist[b];
+    __syncthreads();
+    for (int off = 1; off < kCostBins; off <<= 1) {
+        const uint32_t v = t >= off ? a[t - off] : 0u;
+        __syncthreads();
+        a[t] += v;
+        __syncthreads();
+    }
+    offsets[b] = a[t] - hist[b];  // exclusive
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_cost_scatter(const uint32_t* __restrict__ cost,
+                                                             int64_t n,
+                                                             uint32_t* __restrict__ offsets,
+                                                             uint32_t* __restrict__ order) {
+    __shared__ uint32_t h[kCostBins];
+    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) h[b] = 0u;
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
+    uint32_t bin = 0, rank = 0;
+    if (i < n) {
+        bin = cost[i];
+        rank = atomicAdd(&h[bin], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock)
+        if (h[b] != 0u) h[b] = atomicAdd(&offsets[b], h[b]);
+    __syncthreads();
+    if (i < n) order[h[bin] + rank] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_zero_words(uint32_t* __restrict__ p, int n) {
+    const int i = blockIdx.x * kSortBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+struct OrderWorkspace {
+    uint32_t* hist = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* cost = nullptr;
+    uint32_t* order = nullptr;
+};
+
+size_t order_workspace_bytes(int64_t n) {
+    const size_t words = 2 * static_cast<size_t>(kCostBins) + 2 * static_cast<size_t>(n);
+    return (words * 4 + 255) / 256 * 256;
+}
+
+OrderWorkspace order_workspace(void* ws, int64_t n) {
+    OrderWorkspace w;
+    uint32_t* p = static_cast<uint32_t*>(ws);
+    w.hist = p;
+    w.offsets = p + kCostBins;
+    w.cost = p + 2 * kCostBins;
+    w.order = w.cost + n;
+    return w;
+}
+
+// Batches at least this large are ordered: four times the rays the current
+// device holds resident (two per wave, at the forward's occupancy).
+int64_t order_min_rays() {
+    static int64_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return INT64_MAX;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            return INT64_MAX;
+        cached[dev] = 4 * static_cast<int64_t>(cus) * 4 * SVOX2_FWD_WAVES * (64 / kGroup);
+    }
+    return cached[dev];
+}
+
+// Sorts the rays by the costs the forward wrote (w.cost) into w.order.
+void launch_cost_order(const OrderWorkspace& w, int64_t n, hipStream_t s) {
+    const unsigned hb = static_cast<unsigned>(std::min<int64_t>(blocks_for(n, kSortBlock), 1024));
+    hipLaunchKernelGGL(k_cost_hist, dim3(hb), dim3(kSortBlock), 0, s, w.cost, n, w.hist);
+    hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(kCostBins), 0, s, w.hist, w.offsets);
+    hipLaunchKernelGGL(k_cost_scatter, dim3(blocks_for(n, kSortBlock)), dim3(kSortBlock), 0, s,
+                       w.cost, n, w.offsets, w.order);
 }
 
 // ---------------------------------------------------------------------------
@@ -913,7 +1070,7 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
-                           log_transmit_out, (int32_t*)nullptr);
+                           log_transmit_out, (int32_t*)nullptr, (uint32_t*)nullptr);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol");
@@ -936,7 +1093,7 @@ extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const f
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, true>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays,
-                           (float*)nullptr, (float*)nullptr, stats_out);
+                           (float*)nullptr, (float*)nullptr, stats_out, (uint32_t*)nullptr);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_stats");
@@ -999,6 +1156,10 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     return check_launch("volume_render_cuvol_backward");
 }
 
+extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
+    return n_rays > 0 ? order_workspace_bytes(n_rays) : 0;
+}
+
 extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                                const float* origins, const float* dirs,
                                                const float* ray_clip, int64_t n_rays,
@@ -1006,7 +1167,8 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                                const float* rgb_gt, float beta_loss,
                                                float sparsity_loss, float* rgb_out,
                                                float* log_transmit_out,
-                                               const svox2_grads_t* grads, void* stream) {
+                                               const svox2_grads_t* grads, void* workspace,
+                                               size_t workspace_bytes, void* stream) {
     int st = check_grid(grid, true);
     if (st) return st;
     OptView o;
@@ -1017,21 +1179,36 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
     SVOX2_REQUIRE(!(beta_loss > 0.f) || log_transmit_out != nullptr,
                   "beta_loss > 0 needs log_transmit_out (n floats)");
+    if (workspace != nullptr) {
+        SVOX2_REQUIRE(workspace_bytes >= order_workspace_bytes(n_rays),
+                      "workspace of %zu bytes is below the %zu bytes %lld rays need",
+                      workspace_bytes, order_workspace_bytes(n_rays), (long long)n_rays);
+        SVOX2_REQUIRE(n_rays <= static_cast<int64_t>(UINT32_MAX),
+                      "ray ordering supports at most 2^32 - 1 rays per call");
+    }
     const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
                      grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const float norm = 2.f / static_cast<float>(3 * n_rays);
     const float beta = beta_loss / static_cast<float>(n_rays);
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     float* lt = beta_loss > 0.f ? log_transmit_out : nullptr;
+    const hipStream_t s = as_stream(stream);
+    OrderWorkspace w;
+    if (workspace != nullptr && n_rays >= order_min_rays()) {
+        w = order_workspace(workspace, n_rays);
+        hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)),
+                           dim3(kSortBlock), 0, s, w.hist, kCostBins);
+    }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
-        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
-                           as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
-                           (int32_t*)nullptr);
-        hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0,
-                           as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_gt,
-                           rgb_out, lt, beta, sparsity_loss, norm, go);
+        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s, g, o,
+                           origins, dirs, ray_clip, n_rays, rgb_out, lt, (int32_t*)nullptr,
+                           w.cost);
+        if (w.order != nullptr) launch_cost_order(w, n_rays, s);
+        hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
+                           origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, lt, beta,
+                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");

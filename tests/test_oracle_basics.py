@@ -170,3 +170,95 @@ def test_rmsprop_closed_form():
     # clamp to minval
     assert d[1, 1] == pytest.approx(max(-4 - 0.2 * 0.1 / (0.1 + 1e-8), -3.5))
     assert (g[:2] == 0).all() and (g[2] == grad[2]).all() and (d[2] == data[2]).all()
+
+
+def _tv_energy(links, data, cells, lo, hi, scale, ignore_edge, ignore_last_z):
+    """Sparse TV energy whose data gradient the reference kernel evaluates on a
+    256-cube (loss_kernel.cu:178-246, scale / n_cells from :597): per selected
+    cell and channel, (scale / n) * sqrt(1e-9 + dx^2 + dy^2 + dz^2) of the
+    forward differences, with the kernel's neighbour and edge rules (a missing
+    +z/+y/+x neighbour past the grid reads row 0; an empty link reads 0, or
+    v000 when ignore_edge)."""
+    s0, s1, s2 = links.shape
+    n = len(cells)
+    e = data.sum() * 0.0
+    for xyz in cells:
+        z, y, x = xyz % s2, (xyz // s2) % s1, xyz // (s1 * s2)
+        l000 = int(links[x, y, z])
+        if ignore_edge and l000 == 0:
+            continue
+        if ignore_last_z and z == s2 - 2:
+            continue
+        l001 = int(links[x, y, z + 1]) if (z + 1 < s2 and (not ignore_last_z or z != s2 - 2)) else 0
+        l010 = int(links[x, y + 1, z]) if y + 1 < s1 else 0
+        l100 = int(links[x + 1, y, z]) if x + 1 < s0 else 0
+        v000 = data[l000, lo:hi] if l000 >= 0 else torch.zeros(hi - lo, dtype=data.dtype)
+        nul = v000 if ignore_edge else torch.zeros(hi - lo, dtype=data.dtype)
+        nb = [data[lk, lo:hi] if lk >= 0 else nul for lk in (l100, l010, l001)]
+        d2 = sum((v - v000) ** 2 for v in nb)
+        e = e + (scale / n) * torch.sqrt(1e-9 + d2).sum()
+    return e
+
+
+def _tv_case(size, seed, block_at):
+    """Links of a `size` cube: -1 except random occupancy (70%) in a 14^3 block
+    at `block_at` and in the 6^3 block at the far corner; random rows."""
+    g = np.random.default_rng(seed)
+    links = np.full((size,) * 3, -1, np.int32)
+    a = block_at
+    occ = g.random((14, 14, 14)) < 0.7
+    links[a:a + 14, a:a + 14, a:a + 14][occ] = 1  # marked, numbered below
+    occ2 = g.random((6, 6, 6)) < 0.7
+    links[size - 6:, size - 6:, size - 6:][occ2] = 1
+    idx = np.flatnonzero(links.reshape(-1) >= 0)
+    g.shuffle(idx)
+    flat = links.reshape(-1)
+    flat[idx] = np.arange(len(idx), dtype=np.int32)
+    cells = []
+    for x in range(a, a + 12):
+        for y in range(a, a + 12):
+            for z in range(a, a + 12):
+                cells.append((x * size + y) * size + z)
+    for x in range(size - 5, size):
+        for y in range(size - 5, size):
+            for z in range(size - 5, size):
+                cells.append((x * size + y) * size + z)
+    return links, len(idx), np.array(cells, np.int64)
+
+
+@pytest.mark.parametrize("ignore_edge,ignore_last_z", [(False, False), (True, False),
+                                                       (False, True)])
+def test_tv_grad_is_energy_gradient_at_256(ignore_edge, ignore_last_z):
+    """On a 256-cube the per-axis factor N/256 is 1 and the TV gradient the
+    oracle forms is exactly the autograd gradient of the TV energy; its mask is
+    the set of rows whose gradient is non-zero."""
+    links, R, cells = _tv_case(256, 5, 100)
+    g = np.random.default_rng(6)
+    data = g.normal(size=(R, 5))
+    lo, hi, scale = 1, 4, 0.37
+    grad, mask = O.tv_grad_sparse(links, data, cells, lo, hi, scale, ignore_edge, ignore_last_z)
+    t = torch.tensor(data, requires_grad=True)
+    _tv_energy(links, t, cells, lo, hi, scale, ignore_edge, ignore_last_z).backward()
+    ref = t.grad.numpy()
+    assert np.abs(grad - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(mask, np.abs(grad).sum(1) > 0)
+    assert mask.sum() > 100
+
+
+def test_tv_grad_axis_factor():
+    """The factor N/256 scales the differences after the normaliser is formed
+    (loss_kernel.cu:220-225): the same neighbourhoods in a 128-cube give
+    exactly half the 256-cube gradient (it would be equal if the normaliser
+    used the scaled differences)."""
+    l256, R, _ = _tv_case(256, 8, 60)
+    l128 = np.full((128,) * 3, -1, np.int32)
+    l128[60:74, 60:74, 60:74] = l256[60:74, 60:74, 60:74]
+    cells = np.array([(x * 256 + y) * 256 + z for x in range(60, 72) for y in range(60, 72)
+                      for z in range(60, 72)], np.int64)
+    cells128 = np.array([(x * 128 + y) * 128 + z for x in range(60, 72) for y in range(60, 72)
+                         for z in range(60, 72)], np.int64)
+    data = np.random.default_rng(9).normal(size=(R, 2))
+    g256, m256 = O.tv_grad_sparse(l256, data, cells, 0, 2, 1.0, False, False)
+    g128, m128 = O.tv_grad_sparse(l128, data, cells128, 0, 2, 1.0, False, False)
+    assert np.array_equal(m256, m128) and m256.sum() > 100
+    assert np.array_equal(g128, 0.5 * g256)
