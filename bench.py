@@ -174,6 +174,99 @@ def image_forward(grid, cams, reps=5):
     return dict(ms=ms, rays_per_s=cam.width * cam.height / (ms * 1e-3))
 
 
+def stress_1m(wl: Workload, grid, steps: int = 3):
+    """Million-ray differentiation step on the 512^3 grid: 1,048,576 random
+    camera rays with per-ray (t_near ~ U[0, 0.5], t_far ~ U[2, 4]) clips, the
+    autograd path (SparseGrid.volume_render, MSE, backward re-marching every
+    ray), timed with events; peak HBM is the allocator's high-water mark
+    during the steps above what was allocated before them."""
+    import svox2_amd
+    from svox2_amd import scenes
+    n = 1 << 20
+    gen = torch.Generator().manual_seed(77)
+    o, d = wl.cams.random_batch(n, gen)
+    clip = scenes.config_clip(n, gen)
+    gt = torch.rand(n, 3, generator=gen)
+    rays = svox2_amd.Rays(o.to(wl.dev), d.to(wl.dev), clip.to(wl.dev))
+    gt = gt.to(wl.dev)
+
+    def step():
+        grid.density_data.grad = None
+        grid.sh_data.grad = None
+        rgb = grid.volume_render(rays)
+        loss = torch.nn.functional.mse_loss(rgb, gt)
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    grid.density_data.grad = None
+    grid.sh_data.grad = None
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        step()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    peak = torch.cuda.max_memory_allocated() - base
+    grid.density_data.grad = None
+    grid.sh_data.grad = None
+    return dict(rays=n, ms_per_step=ms, rays_per_s=n / (ms * 1e-3), peak_hbm_bytes=int(peak),
+                peak_hbm_gb=peak / 1e9,
+                note="autograd volume_render + MSE backward, per-ray clips; peak above the "
+                     "resident grid")
+
+
+SERVE_SIDES = (128, 256, 512, 800, 1024)
+
+
+def serve_requests(cam_sets, n: int, qps: float, seed: int):
+    """Poisson arrivals at `qps`; each request a uniform random grid of four,
+    a uniform random camera of that grid's hemisphere set and a uniform side
+    from SERVE_SIDES (square images, the 800-pixel focal length scaled)."""
+    from svox2_amd.serve import RenderRequest
+    g = np.random.default_rng(seed)
+    t = np.cumsum(g.exponential(1.0 / qps, size=n))
+    reqs = []
+    for i in range(n):
+        grid = int(g.integers(4))
+        cams = cam_sets[grid]
+        side = int(SERVE_SIDES[g.integers(len(SERVE_SIDES))])
+        cam = cams.camera(int(g.integers(cams.c2w.shape[0])), width=side, height=side)
+        reqs.append(RenderRequest(grid=grid, camera=cam, arrival=float(t[i])))
+    return reqs
+
+
+def serve_bench(wl: Workload, qps: float = 200.0, n: int = 600):
+    """The serving loop on four resident grids (256^3 ~8% and 512^3 ~3%
+    occupied, one of each in bf16): the same Poisson request stream served by
+    the continuous-batching server and by one volume_render_image launch per
+    request."""
+    from svox2_amd import scenes
+    from svox2_amd.serve import RenderServer, serve_one_per_launch
+    sc256, cams256, _ = scenes.workload_blobs(256, scenes.SPARSE256, seed=1, device=wl.dev)
+    g256 = {}
+    for dt in (torch.float32, torch.bfloat16):
+        g = scenes.to_sparse_grid(sc256, device=wl.dev, dtype=dt)
+        g.opt.sigma_thresh, g.opt.stop_thresh = 1e-8, 1e-7
+        g256[dt] = g
+    grids = [g256[torch.float32], g256[torch.bfloat16], wl.grid(torch.float32),
+             wl.grid(torch.bfloat16)]
+    cam_sets = [cams256, cams256, wl.cams, wl.cams]
+    server = RenderServer(grids, hbm_budget=1 << 30, tile_pixels=8192, max_tiles=64,
+                          max_active=64)
+    server.run(serve_requests(cam_sets, 16, qps, seed=100), realtime=False)  # warm-up
+    out = {"qps": qps, "requests": n, "sides": list(SERVE_SIDES),
+           "grids": "256^3 f32, 256^3 bf16, 512^3 f32, 512^3 bf16"}
+    out["batched"] = server.run(serve_requests(cam_sets, n, qps, seed=7))
+    out["batched"]["steps"] = server.steps
+    serve_one_per_launch(grids, serve_requests(cam_sets, 8, qps, seed=101), realtime=False)
+    out["one_per_launch"] = serve_one_per_launch(grids, serve_requests(cam_sets, n, qps, seed=7))
+    return out
+
+
 def cpu_baseline(wl: Workload, budget_s: float = 12.0):
     """The scalar C port of the reference's fused forward + MSE backward
     (oracle/march_ref.c: trace_ray_cuvol and trace_ray_cuvol_backward, one
@@ -269,6 +362,16 @@ def main():
         extra["bf16_5000"] = measure(gb, small, args.steps, args.warmup, 2, 2)
         log(f"5000-ray batches: fp32 {extra['fp32_5000']['rays_per_s']:.4g}, bf16 "
             f"{extra['bf16_5000']['rays_per_s']:.4g} rays/s")
+        log("1M-ray autograd steps with per-ray clips")
+        extra["stress_1m_fp32"] = stress_1m(wl, g32)
+        log(f"1M rays: {extra['stress_1m_fp32']['ms_per_step']:.2f} ms/step, peak HBM "
+            f"{extra['stress_1m_fp32']['peak_hbm_gb']:.2f} GB")
+        log("serving loop, 200 QPS")
+        extra["serve"] = serve_bench(wl)
+        sv = extra["serve"]
+        log(f"serve: batched p50 {sv['batched']['p50_ms']:.2f} ms p99 {sv['batched']['p99_ms']:.2f} "
+            f"ms {sv['batched']['rays_per_s']:.3g} rays/s; one-per-launch p50 "
+            f"{sv['one_per_launch']['p50_ms']:.2f} ms p99 {sv['one_per_launch']['p99_ms']:.2f} ms")
         log("800x800 image forward")
         extra["image_800_fp32"] = image_forward(g32, wl.cams)
         extra["image_800_bf16"] = image_forward(gb, wl.cams)
@@ -315,6 +418,13 @@ def main():
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
             "counters_per_launch": head["counters"],
         },
+        "serve": ({"qps": extra["serve"]["qps"],
+                   "p50_ms": extra["serve"]["batched"]["p50_ms"],
+                   "p99_ms": extra["serve"]["batched"]["p99_ms"],
+                   "rays_per_s": extra["serve"]["batched"]["rays_per_s"],
+                   "one_per_launch_p50_ms": extra["serve"]["one_per_launch"]["p50_ms"],
+                   "one_per_launch_p99_ms": extra["serve"]["one_per_launch"]["p99_ms"]}
+                  if "serve" in extra else None),
         "cpu_baseline": cpu,
         "detail": {"fp32_65536": head, **extra},
     }

@@ -67,6 +67,11 @@ typedef struct svox2_grid {
     float scaling[3];
     int64_t density_stride; /* elements between density rows (0 = 1) */
     int64_t sh_stride;      /* elements between SH rows (0 = sh_data_dim) */
+    /* Optional (NULL = none) cell map of these links and rows, as written by
+     * svox2_build_cell_map; it must be rebuilt whenever links or rows change.
+     * The march then decides skips and empty cells from it and reads the
+     * links only for cells with an occupied corner.  Results are the same. */
+    const uint16_t* cell_map;
 } svox2_grid_t;
 
 /* RenderOptions (svox2/csrc/include/data_spec.hpp:115-135). */
@@ -101,13 +106,15 @@ typedef struct svox2_grads {
 /* Forward render of n_rays rays; rgb_out [n][3].  ray_clip, if not NULL, is
  * [n][2] (t_near, t_far) in world distance along the ray direction vector and
  * narrows each ray's [tmin, tmax].  log_transmit_out, if not NULL, receives the
- * final log transmittance per ray.
+ * final log transmittance per ray.  ray_cost_out, if not NULL, receives each
+ * ray's march chunk count, which svox2_volume_render_cuvol_backward can use
+ * to order its rays.
  * Replaces volume_render_cuvol (svox2.cpp:28; render_lerp_kernel_cuvol.cu:918). */
 int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
                               const float* dirs, const float* ray_clip,
                               int64_t n_rays, const svox2_render_options_t* opt,
                               float* rgb_out, float* log_transmit_out,
-                              void* stream);
+                              uint32_t* ray_cost_out, void* stream);
 
 /* Forward render of a pinhole image; rgb_out [height][width][3].
  * Replaces volume_render_cuvol_image (svox2.cpp:31; render_lerp_kernel_cuvol.cu:960). */
@@ -118,7 +125,11 @@ int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
                                     void* stream);
 
 /* Backward of svox2_volume_render_cuvol given dL/drgb (grad_out [n][3]) and
- * the forward colours (color_cache [n][3]); accumulates into grads.
+ * the forward colours (color_cache [n][3]); accumulates into grads.  With the
+ * forward's ray_cost_out (ray_cost) and a device workspace of
+ * svox2_ray_order_workspace_bytes(n_rays) bytes, batches of at least four
+ * times the rays the device holds resident are processed in descending order
+ * of cost (same results); otherwise, or with NULLs, in input order.
  * Replaces volume_render_cuvol_backward (svox2.cpp:33; render_lerp_kernel_cuvol.cu:1010). */
 int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                        const float* origins, const float* dirs,
@@ -127,6 +138,8 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                        const float* grad_out,
                                        const float* color_cache,
                                        const svox2_grads_t* grads,
+                                       const uint32_t* ray_cost,
+                                       void* workspace, size_t workspace_bytes,
                                        void* stream);
 
 /* Forward render plus the gradient of mean((rgb - rgb_gt)^2) (over 3n terms),
@@ -135,7 +148,7 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  * accumulates into grads.  log_transmit_out (n floats, may be NULL unless
  * beta_loss > 0) receives each ray's final log transmittance, which the beta
  * term needs.  With a device workspace of at least
- * svox2_volume_render_cuvol_fused_workspace_bytes(n_rays) bytes and a batch
+ * svox2_ray_order_workspace_bytes(n_rays) bytes and a batch
  * of at least four times the rays the device holds resident (40960 on a
  * 256-CU MI355X), the forward records each ray's chunk count and the backward
  * processes the rays in descending order of it (same results; the rays of a
@@ -153,10 +166,37 @@ int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     void* workspace, size_t workspace_bytes,
                                     void* stream);
 
-/* Device workspace bytes svox2_volume_render_cuvol_fused uses to order the
- * rays of an n_rays batch (0 for an empty batch).  No counterpart in the
- * reference. */
-size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
+/* Device workspace bytes svox2_volume_render_cuvol_fused and
+ * svox2_volume_render_cuvol_backward use to order the rays of an n_rays batch
+ * (0 for an empty batch).  No counterpart in the reference. */
+size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
+
+/* One tile of a multi-request render launch: n_pixels consecutive pixels
+ * (row-major, starting at pixel0) of the image that camera `camera` sees of
+ * grid `grid`, written to rgb_out[pixel][3] (the request's whole image). */
+typedef struct svox2_tile {
+    int32_t grid;     /* index into the grid array */
+    int32_t camera;   /* index into the camera array */
+    int32_t pixel0;
+    int32_t n_pixels; /* <= tile_pixels */
+    float* rgb_out;
+} svox2_tile_t;
+
+/* Renders the tiles of many image requests in one launch: grids
+ * [n_grids] (device memory, validated descriptors of one storage kind and
+ * basis_dim, with their cell maps built), cameras [n_cameras] and tiles
+ * [max_tiles] in device memory, of which the first *n_tiles (a device int,
+ * read by the kernel) are rendered.  Every pixel is computed exactly as
+ * svox2_volume_render_cuvol_image computes it.  Entries naming a grid or
+ * camera out of range are skipped.  The launch shape depends only on
+ * max_tiles and tile_pixels, so a captured graph can be replayed with new
+ * table contents.  No counterpart in the reference: the serving path. */
+int svox2_volume_render_cuvol_tiles(const svox2_grid_t* grids, int32_t n_grids,
+                                    int32_t storage, int32_t basis_dim,
+                                    const svox2_camera_t* cameras, int32_t n_cameras,
+                                    const svox2_render_options_t* opt,
+                                    const svox2_tile_t* tiles, const int32_t* n_tiles,
+                                    int32_t max_tiles, int32_t tile_pixels, void* stream);
 
 /* Per-ray march counters [n][4] (int32): loop iterations, iterations not
  * skipped by the empty-space skip, samples with sigma > sigma_thresh, and a
@@ -228,6 +268,22 @@ size_t svox2_accel_dist_prop_workspace_bytes(const int32_t size[3]);
 int svox2_accel_dist_prop(int32_t* links, const int32_t size[3],
                           void* workspace, size_t workspace_bytes,
                           void* stream);
+
+/* Bytes of the cell map of a links table of this size: one 16-bit entry per
+ * 2x2x2 block of cells, ((size[i] - 2) / 2 + 1) blocks per axis.  No
+ * counterpart in the reference. */
+size_t svox2_cell_map_bytes(const int32_t size[3]);
+
+/* Writes the cell map of links (rows data rows): per block of cells with
+ * lower corners (2bx..2bx+1, 2by..2by+1, 2bz..2bz+1), bits 0-7 hold the
+ * empty-space skip level k shared by the block (every lower-corner link is
+ * -1 - k, k >= 1), 0 if no lower-corner link is below -1, or 255 if the
+ * block mixes them; bit 8 + 4*(x&1) + 2*(y&1) + (z&1) is set iff the cell
+ * with lower corner (x, y, z) has a corner link in [0, rows).  No
+ * counterpart in the reference: an acceleration structure for
+ * svox2_volume_render_cuvol and the calls built on it. */
+int svox2_build_cell_map(const int32_t* links, const int32_t size[3], int64_t rows,
+                         uint16_t* map_out, size_t map_bytes, void* stream);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* svox2_last_error(void);

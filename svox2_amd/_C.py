@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+from collections import OrderedDict
 from typing import Optional, Tuple
 
 import torch
@@ -41,7 +42,8 @@ INDEXER_ALL, INDEXER_MASK, INDEXER_INDEX = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_image",
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
-    "svox2_volume_render_cuvol_fused_workspace_bytes",
+    "svox2_ray_order_workspace_bytes", "svox2_cell_map_bytes",
+    "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
@@ -66,6 +68,7 @@ class _Grid(ctypes.Structure):
         ("scaling", ctypes.c_float * 3),
         ("density_stride", ctypes.c_int64),
         ("sh_stride", ctypes.c_int64),
+        ("cell_map", ctypes.c_void_p),
     ]
 
 
@@ -88,6 +91,14 @@ class _Cam(ctypes.Structure):
         ("cx", ctypes.c_float), ("cy", ctypes.c_float),
         ("width", ctypes.c_int32), ("height", ctypes.c_int32),
         ("ndc_coeffx", ctypes.c_float), ("ndc_coeffy", ctypes.c_float),
+    ]
+
+
+class _Tile(ctypes.Structure):
+    _fields_ = [
+        ("grid", ctypes.c_int32), ("camera", ctypes.c_int32),
+        ("pixel0", ctypes.c_int32), ("n_pixels", ctypes.c_int32),
+        ("rgb_out", ctypes.c_void_p),
     ]
 
 
@@ -121,9 +132,10 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     G, O, Cm, Gr = (ctypes.POINTER(_Grid), ctypes.POINTER(_Opt), ctypes.POINTER(_Cam),
                     ctypes.POINTER(_Grads))
     sig = {
-        "svox2_volume_render_cuvol": [G, vp, vp, vp, i64, O, vp, vp, vp],
+        "svox2_volume_render_cuvol": [G, vp, vp, vp, i64, O, vp, vp, vp, vp],
         "svox2_volume_render_cuvol_image": [G, Cm, O, vp, vp, vp],
-        "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
+        "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp, vp,
+                                               ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp, Gr,
                                             vp, ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
@@ -134,13 +146,18 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
                                f32, f32, f32, vp],
         "svox2_accel_dist_prop": [vp, ctypes.POINTER(i32), vp, ctypes.c_size_t, vp],
+        "svox2_build_cell_map": [vp, ctypes.POINTER(i32), i64, vp, ctypes.c_size_t, vp],
+        "svox2_volume_render_cuvol_tiles": [vp, i32, i32, i32, vp, i32, O, vp, vp, i32, i32,
+                                            vp],
     }
     for name, argtypes in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
         fn.restype = ctypes.c_int
-    lib.svox2_volume_render_cuvol_fused_workspace_bytes.argtypes = [i64]
-    lib.svox2_volume_render_cuvol_fused_workspace_bytes.restype = ctypes.c_size_t
+    lib.svox2_cell_map_bytes.argtypes = [ctypes.POINTER(i32)]
+    lib.svox2_cell_map_bytes.restype = ctypes.c_size_t
+    lib.svox2_ray_order_workspace_bytes.argtypes = [i64]
+    lib.svox2_ray_order_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
     lib.svox2_accel_dist_prop_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_last_error.restype = ctypes.c_char_p
@@ -175,6 +192,10 @@ class SparseGridSpec:
         self.basis_dim: int = 9
         self.basis_type: int = BASIS_TYPE_SH
         self.basis_data: Optional[torch.Tensor] = None
+        # Cell map of the links (svox2_build_cell_map).  None: built on first
+        # use and cached per links tensor (rebuilt when the links change);
+        # False: the march reads links only.
+        self.cell_map = None
 
 
 class RaysSpec:
@@ -310,7 +331,7 @@ def _vec3(v) -> Tuple[float, float, float]:
     return v[0], v[1], v[2]
 
 
-def _grid_struct(g: SparseGridSpec, need_sh: bool = True) -> _Grid:
+def _grid_struct(g: SparseGridSpec, need_sh: bool = True, need_map: bool = True) -> _Grid:
     links = _check(g.links, "links", (torch.int32,), 3)
     dens = _check_rows(g.density_data, "density_data", _DATA_DTYPES)
     sh = g.sh_data
@@ -343,7 +364,55 @@ def _grid_struct(g: SparseGridSpec, need_sh: bool = True) -> _Grid:
         s.scaling[i] = sc[i]
     s.density_stride = _row_stride(dens)
     s.sh_stride = _row_stride(sh) if sh is not None else 0
+    cmap = g.cell_map if need_map else False
+    if cmap is None:
+        cmap = _cached_cell_map(links, dens.size(0))
+    if cmap is not False:
+        need = load_library().svox2_cell_map_bytes(s.size)
+        _check(cmap, "cell_map", (torch.int16,), 1)
+        if cmap.numel() * 2 < need or cmap.device != links.device:
+            raise RuntimeError(f"cell_map must hold {need} bytes on the links' device")
+        s.cell_map = cmap.data_ptr()
     return s
+
+
+# Cell maps by links tensor: (device, address, shape, version, rows, accel
+# epoch) -> map.  The accel epoch counts in-place accel_dist_prop calls, which
+# write through a raw pointer and so do not bump the tensor version.
+_CELL_MAPS: "OrderedDict[tuple, torch.Tensor]" = OrderedDict()
+_CELL_MAP_CACHE = 4
+_ACCEL_EPOCH: dict = {}
+
+
+def _cached_cell_map(links: torch.Tensor, rows: int) -> torch.Tensor:
+    key = (links.device.index, links.data_ptr(), tuple(links.shape), links._version, int(rows),
+           _ACCEL_EPOCH.get((links.device.index, links.data_ptr()), 0))
+    cmap = _CELL_MAPS.get(key)
+    if cmap is None:
+        cmap = build_cell_map(links, rows)
+        # Drop stale maps of the same links tensor, then the oldest ones.
+        for k in [k for k in _CELL_MAPS if k[:3] == key[:3]]:
+            del _CELL_MAPS[k]
+        _CELL_MAPS[key] = cmap
+        while len(_CELL_MAPS) > _CELL_MAP_CACHE:
+            _CELL_MAPS.popitem(last=False)
+    else:
+        _CELL_MAPS.move_to_end(key)
+    return cmap
+
+
+def build_cell_map(links: torch.Tensor, rows: int) -> torch.Tensor:
+    """svox2_build_cell_map: the cell map (int16 words) of ``links`` for a grid
+    of ``rows`` data rows, on the links' device and current stream."""
+    _check(links, "links", (torch.int32,), 3)
+    lib = load_library()
+    size = (ctypes.c_int32 * 3)(*links.shape)
+    nbytes = int(lib.svox2_cell_map_bytes(size))
+    cmap = torch.empty(max(nbytes // 2, 1), dtype=torch.int16, device=links.device)
+    with _on_device(links):
+        _call("svox2_build_cell_map", _ptr(links), size, int(rows), _ptr(cmap),
+              ctypes.c_size_t(nbytes), _stream(links))
+    return cmap
 
 
 def _opt_struct(o: RenderOptions) -> _Opt:
@@ -399,19 +468,25 @@ def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
 # Entry points
 # ---------------------------------------------------------------------------
 def volume_render_cuvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
-                        log_transmit_out: Optional[torch.Tensor] = None) -> torch.Tensor:
-    """volume_render_cuvol (render_lerp_kernel_cuvol.cu:918-958): (n, 3) colours."""
+                        log_transmit_out: Optional[torch.Tensor] = None,
+                        ray_cost_out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """volume_render_cuvol (render_lerp_kernel_cuvol.cu:918-958): (n, 3) colours.
+
+    ray_cost_out: optional (n,) int32 tensor receiving each ray's march chunk
+    count, for ``volume_render_cuvol_backward(..., ray_cost=...)``.
+    """
     g = _grid_struct(grid)
     o, d, clip = _rays(rays)
     out = torch.empty_like(o)
-    if log_transmit_out is not None:
-        _check(log_transmit_out, "log_transmit_out", (torch.float32,))
-        if log_transmit_out.numel() != o.size(0):
-            raise RuntimeError("log_transmit_out must have one value per ray")
+    for t, name in ((log_transmit_out, "log_transmit_out"), (ray_cost_out, "ray_cost_out")):
+        if t is not None:
+            _check(t, name, (torch.float32,) if t is log_transmit_out else (torch.int32,))
+            if t.numel() != o.size(0):
+                raise RuntimeError(f"{name} must have one value per ray")
     with _on_device(o):
         _call("svox2_volume_render_cuvol", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
               o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _ptr(log_transmit_out),
-              _stream(o))
+              _ptr(ray_cost_out), _stream(o))
     return out
 
 
@@ -461,8 +536,13 @@ def volume_render_cuvol_image(grid: SparseGridSpec, cam: CameraSpec, opt: Render
 
 def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
                                  grad_out: torch.Tensor, color_cache: torch.Tensor,
-                                 grads: GridOutputGrads) -> None:
-    """volume_render_cuvol_backward (render_lerp_kernel_cuvol.cu:1010-1068)."""
+                                 grads: GridOutputGrads,
+                                 ray_cost: Optional[torch.Tensor] = None) -> None:
+    """volume_render_cuvol_backward (render_lerp_kernel_cuvol.cu:1010-1068).
+
+    ray_cost: the forward's ``ray_cost_out``; with it, large batches run in
+    descending order of cost (same results).
+    """
     g = _grid_struct(grid)
     o, d, clip = _rays(rays)
     _check(grad_out, "grad_out", (torch.float32,), 2)
@@ -470,10 +550,18 @@ def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: Rend
     if grad_out.shape != o.shape or color_cache.shape != o.shape:
         raise RuntimeError("grad_out / color_cache must be (n, 3)")
     gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    ws, nbytes = None, 0
+    if ray_cost is not None:
+        _check(ray_cost, "ray_cost", (torch.int32,))
+        if ray_cost.numel() != o.size(0):
+            raise RuntimeError("ray_cost must have one value per ray")
+        nbytes = int(load_library().svox2_ray_order_workspace_bytes(o.size(0)))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
         _call("svox2_volume_render_cuvol_backward", ctypes.byref(g), _ptr(o), _ptr(d),
               _ptr(clip), o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(grad_out),
-              _ptr(color_cache), ctypes.byref(gs), _stream(o))
+              _ptr(color_cache), ctypes.byref(gs), _ptr(ray_cost), _ptr(ws), nbytes,
+              _stream(o))
 
 
 def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
@@ -495,7 +583,7 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
              if beta_loss > 0 else None)
     # Workspace for the backward's ray ordering (from torch's caching allocator).
-    nbytes = (load_library().svox2_volume_render_cuvol_fused_workspace_bytes(o.size(0))
+    nbytes = (load_library().svox2_ray_order_workspace_bytes(o.size(0))
               if ordered else 0)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
@@ -508,7 +596,7 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
 def sample_grid(grid: SparseGridSpec, points: torch.Tensor,
                 want_colors: bool) -> Tuple[torch.Tensor, torch.Tensor]:
     """sample_grid (svox2_kernel.cu:181-217): (density (n,1), sh (n,C) or (0,C))."""
-    g = _grid_struct(grid, need_sh=bool(want_colors))
+    g = _grid_struct(grid, need_sh=bool(want_colors), need_map=False)
     _check(points, "points", (torch.float32,), 2)
     if points.size(1) != 3:
         raise RuntimeError("points must be (n, 3)")
@@ -527,7 +615,7 @@ def sample_grid_backward(grid: SparseGridSpec, points: torch.Tensor,
                          grad_density_out: torch.Tensor, grad_sh_out: torch.Tensor,
                          want_colors: bool) -> None:
     """sample_grid_backward (svox2_kernel.cu:219-267)."""
-    g = _grid_struct(grid, need_sh=False)
+    g = _grid_struct(grid, need_sh=False, need_map=False)
     _check(points, "points", (torch.float32,), 2)
     n = points.size(0)
     _check(grad_out_density, "grad_out_density", (torch.float32,), 2)
@@ -615,3 +703,53 @@ def accel_dist_prop(links: torch.Tensor) -> None:
     with _on_device(links):
         _call("svox2_accel_dist_prop", _ptr(links), size, _ptr(ws), ctypes.c_size_t(nbytes),
               _stream(links))
+    key = (links.device.index, links.data_ptr())
+    _ACCEL_EPOCH[key] = _ACCEL_EPOCH.get(key, 0) + 1
+
+
+# ---------------------------------------------------------------------------
+# Multi-request tile rendering (the serving path; no reference counterpart)
+# ---------------------------------------------------------------------------
+def pack_structs(items) -> torch.Tensor:
+    """Bytes of a list of ctypes structures as a CPU uint8 tensor."""
+    if not items:
+        return torch.empty(0, dtype=torch.uint8)
+    blob = b"".join(bytes(memoryview(x)) for x in items)
+    return torch.frombuffer(bytearray(blob), dtype=torch.uint8)
+
+
+def grid_struct(grid: SparseGridSpec) -> _Grid:
+    """Validated C descriptor of a grid (its cell map built and cached)."""
+    return _grid_struct(grid)
+
+
+def camera_struct(cam: CameraSpec) -> _Cam:
+    return _cam_struct(cam)
+
+
+def tile_struct(grid: int, camera: int, pixel0: int, n_pixels: int,
+                rgb_out: torch.Tensor) -> _Tile:
+    t = _Tile()
+    t.grid, t.camera, t.pixel0, t.n_pixels = int(grid), int(camera), int(pixel0), int(n_pixels)
+    t.rgb_out = rgb_out.data_ptr()
+    return t
+
+
+def volume_render_cuvol_tiles(grids: torch.Tensor, n_grids: int, storage: int, basis_dim: int,
+                              cameras: torch.Tensor, n_cameras: int, opt: RenderOptions,
+                              tiles: torch.Tensor, n_tiles: torch.Tensor, max_tiles: int,
+                              tile_pixels: int) -> None:
+    """svox2_volume_render_cuvol_tiles: grids / cameras / tiles are device
+    uint8 tensors holding packed svox2_grid_t / svox2_camera_t / svox2_tile_t
+    arrays, n_tiles a device int32 tensor of one element."""
+    for t, name in ((grids, "grids"), (cameras, "cameras"), (tiles, "tiles")):
+        _check(t, name, (torch.uint8,), 1)
+    _check(n_tiles, "n_tiles", (torch.int32,))
+    if grids.numel() < n_grids * ctypes.sizeof(_Grid) or \
+            cameras.numel() < n_cameras * ctypes.sizeof(_Cam) or \
+            tiles.numel() < max_tiles * ctypes.sizeof(_Tile):
+        raise RuntimeError("grid / camera / tile tables are smaller than their counts")
+    with _on_device(tiles):
+        _call("svox2_volume_render_cuvol_tiles", _ptr(grids), int(n_grids), int(storage),
+              int(basis_dim), _ptr(cameras), int(n_cameras), ctypes.byref(_opt_struct(opt)),
+              _ptr(tiles), _ptr(n_tiles), int(max_tiles), int(tile_pixels), _stream(tiles))

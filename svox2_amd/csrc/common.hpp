@@ -87,6 +87,9 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
 // ---------------------------------------------------------------------------
 // Grid view passed by value to kernels.
 // ---------------------------------------------------------------------------
+// Cell-map blocks along an axis of n grid points (cells 0..n-2, two per block).
+__host__ __device__ inline int cell_map_blocks(int n) { return (n - 2) / 2 + 1; }
+
 template <typename DT>
 struct GridView {
     const int32_t* __restrict__ links;
@@ -103,6 +106,9 @@ struct GridView {
     bool vec_rows; // SH rows are 16-byte aligned with room for whole 16-byte loads
     float offset[3];
     float scaling[3];
+    const uint16_t* __restrict__ cmap;  // cell map (may be null)
+    int mstride_x;  // map blocks per x step: mb[1] * mb[2]
+    int mstride_y;  // mb[2]
     // A link addresses a data row iff 0 <= link < rows; negative links are
     // empty cells and out-of-range links are treated as empty.
     __device__ __forceinline__ bool valid(int link) const {
@@ -118,7 +124,7 @@ struct ViewDT<GridView<DT>> {
 };
 
 template <typename DT>
-inline GridView<DT> make_view(const svox2_grid_t& g) {
+__host__ __device__ inline GridView<DT> make_view(const svox2_grid_t& g) {
     GridView<DT> v;
     v.links = g.links;
     v.density = static_cast<const DT*>(g.density_data);
@@ -139,6 +145,9 @@ inline GridView<DT> make_view(const svox2_grid_t& g) {
     // multiple of 16 bytes (it is then >= the channel count rounded up to 16 bytes).
     const int per16 = 16 / static_cast<int>(sizeof(DT));
     v.vec_rows = (reinterpret_cast<uintptr_t>(g.sh_data) % 16 == 0) && (v.sstride % per16 == 0);
+    v.cmap = g.cell_map;
+    v.mstride_y = cell_map_blocks(g.size[2]);
+    v.mstride_x = cell_map_blocks(g.size[1]) * v.mstride_y;
     return v;
 }
 

@@ -119,3 +119,89 @@ extern "C" int svox2_accel_dist_prop(int32_t* links, const int32_t size[3], void
     k_accel_prop<<<blocks, kBlock, 0, s>>>(links, size[0], size[1], size[2], tmp);
     return check_launch("accel_dist_prop");
 }
+
+// ---------------------------------------------------------------------------
+// Cell map: per 2x2x2 block of cells, the shared skip level of its lower
+// corners and one occupancy bit per cell (any corner link addresses a row).
+// One thread per block reads the up to 3x3x3 links the block's cells touch.
+// ---------------------------------------------------------------------------
+namespace svox2amd {
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_build_cell_map(const int32_t* __restrict__ links,
+                                                           int s0, int s1, int s2, int rows,
+                                                           int m0, int m1, int m2,
+                                                           uint16_t* __restrict__ map) {
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (tid >= static_cast<int64_t>(m0) * m1 * m2) return;
+    const int bz = static_cast<int>(tid % m2);
+    const int by = static_cast<int>((tid / m2) % m1);
+    const int bx = static_cast<int>(tid / (static_cast<int64_t>(m1) * m2));
+    // v[i][j][k] = link at (2bx+i, 2by+j, 2bz+k), or -1 past the grid.
+    int v[3][3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int x = 2 * bx + i, y = 2 * by + j, z = 2 * bz + k;
+                v[i][j][k] = (x < s0 && y < s1 && z < s2)
+                                 ? links[(static_cast<int64_t>(x) * s1 + y) * s2 + z]
+                                 : -1;
+            }
+    int level = -1;  // -1: none seen yet
+    bool mixed = false;
+    uint32_t occ = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                // Cell with lower corner (2bx+i, 2by+j, 2bz+k) exists iff that
+                // corner is at most size-2 on every axis.
+                if (2 * bx + i > s0 - 2 || 2 * by + j > s1 - 2 || 2 * bz + k > s2 - 2) continue;
+                const int l0 = v[i][j][k];
+                const int lv = l0 < -1 ? -l0 - 1 : 0;
+                if (level < 0) level = lv;
+                else if (level != lv) mixed = true;
+                bool any = false;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int lk = v[i + (c >> 2)][j + ((c >> 1) & 1)][k + (c & 1)];
+                    any |= static_cast<uint32_t>(lk) < static_cast<uint32_t>(rows);
+                }
+                if (any) occ |= 1u << (4 * i + 2 * j + k);
+            }
+    if (level < 0) level = 0;
+    const uint32_t lvl = (mixed || level > 254) ? 255u : static_cast<uint32_t>(level);
+    map[tid] = static_cast<uint16_t>(lvl | (occ << 8));
+}
+
+}  // namespace
+}  // namespace svox2amd
+
+extern "C" size_t svox2_cell_map_bytes(const int32_t size[3]) {
+    if (size == nullptr || size[0] < 2 || size[1] < 2 || size[2] < 2) return 0;
+    return static_cast<size_t>(cell_map_blocks(size[0])) * cell_map_blocks(size[1]) *
+           cell_map_blocks(size[2]) * sizeof(uint16_t);
+}
+
+extern "C" int svox2_build_cell_map(const int32_t* links, const int32_t size[3], int64_t rows,
+                                    uint16_t* map_out, size_t map_bytes, void* stream) {
+    SVOX2_REQUIRE(links != nullptr && size != nullptr && map_out != nullptr,
+                  "links / size / map_out are NULL");
+    for (int i = 0; i < 3; ++i) SVOX2_REQUIRE(size[i] >= 2, "grid size[%d] = %d", i, size[i]);
+    const int64_t cells = static_cast<int64_t>(size[0]) * size[1] * size[2];
+    SVOX2_REQUIRE(cells < INT32_MAX, "too many cells");
+    SVOX2_REQUIRE(rows >= 0 && rows < INT32_MAX, "rows %lld out of range", (long long)rows);
+    const size_t need = svox2_cell_map_bytes(size);
+    SVOX2_REQUIRE(map_bytes >= need, "map has %zu bytes, %zu needed", map_bytes, need);
+    const int m0 = cell_map_blocks(size[0]), m1 = cell_map_blocks(size[1]),
+              m2 = cell_map_blocks(size[2]);
+    const int64_t blocks = static_cast<int64_t>(m0) * m1 * m2;
+    k_build_cell_map<<<blocks_for(blocks, kBlock), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+        links, size[0], size[1], size[2], static_cast<int>(rows), m0, m1, m2, map_out);
+    return check_launch("build_cell_map");
+}

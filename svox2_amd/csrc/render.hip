@@ -293,30 +293,66 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
     const bool in_range = ti <= r.tmax;
     c.base = sample_cell(g, r, ti, c.p, c.l);
     const int sx = g.stride_x, sy = g.stride_y;
-    if (in_range) {
-        c.lk[0] = g.links[c.base];
-        c.lk[1] = g.links[c.base + 1];
-        c.lk[2] = g.links[c.base + sy];
-        c.lk[3] = g.links[c.base + sy + 1];
-        c.lk[4] = g.links[c.base + sx];
-        c.lk[5] = g.links[c.base + sx + 1];
-        c.lk[6] = g.links[c.base + sx + sy];
-        c.lk[7] = g.links[c.base + sx + sy + 1];
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c.lk[k] = -1;
-    }
     float skip = 0.f;
     bool is_skip = false;
-    if (in_range && c.lk[0] < -1) {
-        skip = skip_dist(c.lk[0], c.l, c.p, r);
-        is_skip = skip >= o.step;
+    bool occupied = true;  // the cell may have a corner link that addresses a row
+    if (g.cmap != nullptr) {
+        // Skip level and occupancy from the cell map: a cache-resident read
+        // in place of the 8 link reads for cells whose corners are all empty.
+        uint32_t info = 0;
+        if (in_range)
+            info = g.cmap[(c.l[0] >> 1) * g.mstride_x + (c.l[1] >> 1) * g.mstride_y +
+                          (c.l[2] >> 1)];
+        int level = static_cast<int>(info & 0xFFu);
+        if (level == 255) {  // block mixes levels: read this cell's own link
+            const int l0 = g.links[c.base];
+            level = l0 < -1 ? -l0 - 1 : 0;
+        }
+        occupied = ((info >> (8 + ((c.l[0] & 1) << 2) + ((c.l[1] & 1) << 1) + (c.l[2] & 1))) &
+                    1u) != 0;
+        if (in_range && level > 0) {
+            skip = skip_dist(-1 - level, c.l, c.p, r);
+            is_skip = skip >= o.step;
+        }
+    } else {
+        if (in_range) {
+            c.lk[0] = g.links[c.base];
+            c.lk[1] = g.links[c.base + 1];
+            c.lk[2] = g.links[c.base + sy];
+            c.lk[3] = g.links[c.base + sy + 1];
+            c.lk[4] = g.links[c.base + sx];
+            c.lk[5] = g.links[c.base + sx + 1];
+            c.lk[6] = g.links[c.base + sx + sy];
+            c.lk[7] = g.links[c.base + sx + sy + 1];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c.lk[k] = -1;
+        }
+        if (in_range && c.lk[0] < -1) {
+            skip = skip_dist(c.lk[0], c.l, c.p, r);
+            is_skip = skip >= o.step;
+        }
     }
     const uint32_t skip_mask = group_ballot(is_skip);
     const int n_range = __popc(group_ballot(in_range));
     const int first_skip = skip_mask ? (__ffs(skip_mask) - 1) : 32;
     const int n_eval = min(first_skip, n_range);
     const bool eval = lane < n_eval;
+    if (g.cmap != nullptr) {
+        if (eval && occupied) {
+            c.lk[0] = g.links[c.base];
+            c.lk[1] = g.links[c.base + 1];
+            c.lk[2] = g.links[c.base + sy];
+            c.lk[3] = g.links[c.base + sy + 1];
+            c.lk[4] = g.links[c.base + sx];
+            c.lk[5] = g.links[c.base + sx + 1];
+            c.lk[6] = g.links[c.base + sx + sy];
+            c.lk[7] = g.links[c.base + sx + sy + 1];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c.lk[k] = -1;
+        }
+    }
 
     float dn[8];
 #pragma unroll
@@ -329,9 +365,14 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
     const float pcnt = c.contrib ? c.ws * c.sigma : 0.f;
 
     // Log transmittance before this lane's sample, subtracting the earlier
-    // samples' optical depths one by one in sample order.
+    // samples' optical depths one by one in sample order.  Only lanes up to
+    // the last contributing one subtract anything (x - 0 == x), so the replay
+    // stops there, and chunks without a contribution (empty-space crawls)
+    // skip it.
     float lt = log_t;
-    for (int j = 0; j < 31; ++j) {
+    const uint32_t contrib_mask = group_ballot(c.contrib);
+    const int n_rep = contrib_mask ? 32 - __clz(contrib_mask) : 0;
+    for (int j = 0; j < n_rep; ++j) {
         const float pj = __shfl(pcnt, j, kGroup);
         if (j < lane) lt -= pj;
     }
@@ -832,15 +873,46 @@ SVOX2_FWD_KERNEL void k_render_image(
     if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
 }
 
+// Pixels of many image requests per launch.  blockIdx.y selects the tile;
+// tiles past *n_tiles, and pixels past a tile's n_pixels, exit at once.
+template <typename DT, int B>
+SVOX2_FWD_KERNEL void k_render_tiles(const svox2_grid_t* __restrict__ grids, int n_grids,
+                                     const svox2_camera_t* __restrict__ cams, int n_cams,
+                                     OptView o, const svox2_tile_t* __restrict__ tiles,
+                                     const int32_t* __restrict__ n_tiles) {
+    const int t = blockIdx.y;
+    if (t >= *n_tiles) return;
+    const svox2_tile_t tile = tiles[t];
+    const int64_t local = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (local >= tile.n_pixels) return;
+    if (tile.grid < 0 || tile.grid >= n_grids || tile.camera < 0 || tile.camera >= n_cams) return;
+    const svox2_camera_t cam = cams[tile.camera];
+    const int64_t pixel = tile.pixel0 + local;
+    if (cam.width <= 0 || pixel >= static_cast<int64_t>(cam.width) * cam.height) return;
+    const GridView<DT> g = make_view<DT>(grids[tile.grid]);
+    const int ix = static_cast<int>(pixel % cam.width);
+    const int iy = static_cast<int>(pixel / cam.width);
+    float wd[3], wo[3];
+    camera_ray(cam, ix, iy, wd, wo);
+    const RayCtx<B> rc(wd);
+    Ray r;
+    setup_ray(g, o, wo, wd, nullptr, r);
+    float out[3];
+    march_forward<DT, B, false>(g, o, r, rc, lane, out, nullptr);
+    if (lane < 3) tile.rgb_out[3 * pixel + lane] = pick3(out, lane);
+}
+
 template <typename DT, int B>
 SVOX2_BWD_KERNEL void k_render_backward(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ grad_out, const float* __restrict__ color_cache,
-        GradOut go) {
-    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+        const uint32_t* __restrict__ order, GradOut go) {
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
-    if (ray >= n) return;
+    if (slot >= n) return;
+    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
@@ -968,6 +1040,16 @@ OrderWorkspace order_workspace(void* ws, int64_t n) {
     return w;
 }
 
+int check_order_workspace(const void* workspace, size_t bytes, int64_t n) {
+    if (workspace == nullptr) return ok();
+    SVOX2_REQUIRE(bytes >= order_workspace_bytes(n),
+                  "workspace of %zu bytes is below the %zu bytes %lld rays need", bytes,
+                  order_workspace_bytes(n), (long long)n);
+    SVOX2_REQUIRE(n <= static_cast<int64_t>(UINT32_MAX),
+                  "ray ordering supports at most 2^32 - 1 rays per call");
+    return ok();
+}
+
 // Batches at least this large are ordered: four times the rays the current
 // device holds resident (two per wave, at the forward's occupancy).
 int64_t order_min_rays() {
@@ -985,12 +1067,15 @@ int64_t order_min_rays() {
 }
 
 // Sorts the rays by the costs the forward wrote (w.cost) into w.order.
-void launch_cost_order(const OrderWorkspace& w, int64_t n, hipStream_t s) {
+void launch_cost_order(const uint32_t* cost, const OrderWorkspace& w, int64_t n,
+                       hipStream_t s) {
     const unsigned hb = static_cast<unsigned>(std::min<int64_t>(blocks_for(n, kSortBlock), 1024));
-    hipLaunchKernelGGL(k_cost_hist, dim3(hb), dim3(kSortBlock), 0, s, w.cost, n, w.hist);
+    hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)), dim3(kSortBlock), 0,
+                       s, w.hist, kCostBins);
+    hipLaunchKernelGGL(k_cost_hist, dim3(hb), dim3(kSortBlock), 0, s, cost, n, w.hist);
     hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(kCostBins), 0, s, w.hist, w.offsets);
     hipLaunchKernelGGL(k_cost_scatter, dim3(blocks_for(n, kSortBlock)), dim3(kSortBlock), 0, s,
-                       w.cost, n, w.offsets, w.order);
+                       cost, n, w.offsets, w.order);
 }
 
 // ---------------------------------------------------------------------------
@@ -1056,7 +1141,7 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
                                          const float* dirs, const float* ray_clip,
                                          int64_t n_rays, const svox2_render_options_t* opt,
                                          float* rgb_out, float* log_transmit_out,
-                                         void* stream) {
+                                         uint32_t* ray_cost_out, void* stream) {
     int st = check_grid(grid, true);
     if (st) return st;
     OptView o;
@@ -1070,7 +1155,7 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
-                           log_transmit_out, (int32_t*)nullptr, (uint32_t*)nullptr);
+                           log_transmit_out, (int32_t*)nullptr, ray_cost_out);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol");
@@ -1125,6 +1210,47 @@ extern "C" int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
     return check_launch("volume_render_cuvol_image");
 }
 
+extern "C" int svox2_volume_render_cuvol_tiles(const svox2_grid_t* grids, int32_t n_grids,
+                                               int32_t storage, int32_t basis_dim,
+                                               const svox2_camera_t* cameras, int32_t n_cameras,
+                                               const svox2_render_options_t* opt,
+                                               const svox2_tile_t* tiles, const int32_t* n_tiles,
+                                               int32_t max_tiles, int32_t tile_pixels,
+                                               void* stream) {
+    OptView o;
+    int st = make_opt(opt, o);
+    if (st) return st;
+    SVOX2_REQUIRE(grids != nullptr && cameras != nullptr && tiles != nullptr && n_tiles != nullptr,
+                  "grids / cameras / tiles / n_tiles are NULL");
+    SVOX2_REQUIRE(n_grids >= 1 && n_cameras >= 1, "no grids or cameras");
+    SVOX2_REQUIRE(max_tiles >= 0 && max_tiles <= 65535, "max_tiles %d out of range", max_tiles);
+    SVOX2_REQUIRE(tile_pixels >= 1 && tile_pixels <= (1 << 20), "tile_pixels %d out of range",
+                  tile_pixels);
+    SVOX2_REQUIRE(storage == SVOX2_STORAGE_F32 || storage == SVOX2_STORAGE_BF16,
+                  "unknown storage kind %d", storage);
+    if (max_tiles == 0) return ok();
+    const dim3 grid_dim(blocks_for(static_cast<int64_t>(tile_pixels) * kGroup, kBlock),
+                        static_cast<unsigned>(max_tiles));
+    const hipStream_t s = as_stream(stream);
+    auto launch = [&](auto dtp, auto b) {
+        constexpr int B = decltype(b)::value;
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        hipLaunchKernelGGL((k_render_tiles<DT, B>), grid_dim, dim3(kBlock), 0, s, grids, n_grids,
+                           cameras, n_cameras, o, tiles, n_tiles);
+    };
+#define SVOX2_TILE_CASE(B)                                                         \
+    if (basis_dim == B) {                                                          \
+        if (storage == SVOX2_STORAGE_BF16) launch((uint16_t*)nullptr, std::integral_constant<int, B>()); \
+        else launch((float*)nullptr, std::integral_constant<int, B>());           \
+        return check_launch("volume_render_cuvol_tiles");                        \
+    }
+    SVOX2_TILE_CASE(1)
+    SVOX2_TILE_CASE(4)
+    SVOX2_TILE_CASE(9)
+#undef SVOX2_TILE_CASE
+    return fail(SVOX2_ERR_UNSUPPORTED, "basis_dim %d not supported (1, 4 or 9)", basis_dim);
+}
+
 extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                                   const float* origins, const float* dirs,
                                                   const float* ray_clip, int64_t n_rays,
@@ -1132,7 +1258,8 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                                   const float* grad_out,
                                                   const float* color_cache,
                                                   const svox2_grads_t* grads,
-                                                  void* stream) {
+                                                  const uint32_t* ray_cost, void* workspace,
+                                                  size_t workspace_bytes, void* stream) {
     int st = check_grid(grid, true);
     if (st) return st;
     OptView o;
@@ -1142,21 +1269,28 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(grad_out != nullptr && color_cache != nullptr,
                   "grad_out / color_cache are NULL");
+    if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
     const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
                      grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    const hipStream_t s = as_stream(stream);
+    OrderWorkspace w;
+    if (ray_cost != nullptr && workspace != nullptr && n_rays >= order_min_rays()) {
+        w = order_workspace(workspace, n_rays);
+        launch_cost_order(ray_cost, w, n_rays, s);
+    }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
-        hipLaunchKernelGGL((k_render_backward<DT, B>), dim3(blocks), dim3(kBlock), 0,
-                           as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, grad_out,
-                           color_cache, go);
+        hipLaunchKernelGGL((k_render_backward<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
+                           origins, dirs, ray_clip, n_rays, grad_out, color_cache,
+                           static_cast<const uint32_t*>(w.order), go);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_backward");
 }
 
-extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
+extern "C" size_t svox2_ray_order_workspace_bytes(int64_t n_rays) {
     return n_rays > 0 ? order_workspace_bytes(n_rays) : 0;
 }
 
@@ -1179,13 +1313,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
     SVOX2_REQUIRE(!(beta_loss > 0.f) || log_transmit_out != nullptr,
                   "beta_loss > 0 needs log_transmit_out (n floats)");
-    if (workspace != nullptr) {
-        SVOX2_REQUIRE(workspace_bytes >= order_workspace_bytes(n_rays),
-                      "workspace of %zu bytes is below the %zu bytes %lld rays need",
-                      workspace_bytes, order_workspace_bytes(n_rays), (long long)n_rays);
-        SVOX2_REQUIRE(n_rays <= static_cast<int64_t>(UINT32_MAX),
-                      "ray ordering supports at most 2^32 - 1 rays per call");
-    }
+    if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
     const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
                      grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const float norm = 2.f / static_cast<float>(3 * n_rays);
@@ -1194,18 +1322,14 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     float* lt = beta_loss > 0.f ? log_transmit_out : nullptr;
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
-    if (workspace != nullptr && n_rays >= order_min_rays()) {
-        w = order_workspace(workspace, n_rays);
-        hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)),
-                           dim3(kSortBlock), 0, s, w.hist, kCostBins);
-    }
+    if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s, g, o,
                            origins, dirs, ray_clip, n_rays, rgb_out, lt, (int32_t*)nullptr,
                            w.cost);
-        if (w.order != nullptr) launch_cost_order(w, n_rays, s);
+        if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s);
         hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
                            origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, lt, beta,
                            sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go);

@@ -39,10 +39,10 @@ def test_struct_layouts_match_c(tmp_path):
 #include <stddef.h>
 #include "svox2_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(svox2_grid_t), offsetof(svox2_grid_t, rows),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(svox2_grid_t), offsetof(svox2_grid_t, rows),
          offsetof(svox2_grid_t, size), offsetof(svox2_grid_t, storage),
          offsetof(svox2_grid_t, offset), offsetof(svox2_grid_t, scaling),
-         offsetof(svox2_grid_t, sh_stride));
+         offsetof(svox2_grid_t, sh_stride), offsetof(svox2_grid_t, cell_map));
   printf("%zu %zu\n", sizeof(svox2_render_options_t), offsetof(svox2_render_options_t, last_sample_opaque));
   printf("%zu %zu %zu\n", sizeof(svox2_camera_t), offsetof(svox2_camera_t, width), offsetof(svox2_camera_t, ndc_coeffy));
   printf("%zu %zu\n", sizeof(svox2_grads_t), offsetof(svox2_grads_t, grad_sh_stride));
@@ -56,7 +56,8 @@ int main(void) {
     c = [int(x) for x in out]
     G, O, Cm, Gr = _C._Grid, _C._Opt, _C._Cam, _C._Grads
     py = [ctypes.sizeof(G), G.rows.offset, G.size.offset, G.storage.offset, G.offset.offset,
-          G.scaling.offset, G.sh_stride.offset, ctypes.sizeof(O), O.last_sample_opaque.offset, ctypes.sizeof(Cm),
+          G.scaling.offset, G.sh_stride.offset, G.cell_map.offset, ctypes.sizeof(O),
+          O.last_sample_opaque.offset, ctypes.sizeof(Cm),
           Cm.width.offset, Cm.ndc_coeffy.offset, ctypes.sizeof(Gr), Gr.grad_sh_stride.offset]
     assert c == py
 
@@ -66,7 +67,7 @@ def test_invalid_arguments_return_status_and_message():
     opt = _C._Opt()
     opt.step_size = 0.5
     st = lib.svox2_volume_render_cuvol(None, None, None, None, 0, ctypes.byref(opt), None, None,
-                                       None)
+                                       None, None)
     assert st == 1
     assert b"grid is NULL" in lib.svox2_last_error()
     g = _C._Grid()
@@ -79,11 +80,11 @@ def test_invalid_arguments_return_status_and_message():
     bad = _C._Opt()
     bad.step_size = 0.0  # would never terminate: rejected
     st = lib.svox2_volume_render_cuvol(ctypes.byref(g), None, None, None, 0, ctypes.byref(bad),
-                                       None, None, None)
+                                       None, None, None, None)
     assert st == 1 and b"step_size" in lib.svox2_last_error()
     g.basis_dim, g.sh_data_dim = 16, 48
     st = lib.svox2_volume_render_cuvol(ctypes.byref(g), None, None, None, 0, ctypes.byref(opt),
-                                       None, None, None)
+                                       None, None, None, None)
     assert st == 1 and b"basis_dim" in lib.svox2_last_error()
     g.basis_dim, g.sh_data_dim = 9, 27
     st = lib.svox2_rmsprop_step(None, 0, None, None, 10, 1, 0, 0, 0, 7, None, 0, 0.9, 0.1,

@@ -174,7 +174,7 @@ def test_fused_ray_order(gpu, dense64):
     n = rays.origins.size(0)
     gt_t = torch.rand(n, 3, generator=gen).to(gpu)
     lib = _C.load_library()
-    nbytes = lib.svox2_volume_render_cuvol_fused_workspace_bytes(n)
+    nbytes = lib.svox2_ray_order_workspace_bytes(n)
     assert nbytes >= 4 * (2 * 1024 + 2 * n)
     ws = torch.zeros(nbytes // 4, dtype=torch.int32, device=gpu)
 

@@ -236,3 +236,66 @@ def test_contiguous_and_interleaved_layouts_agree(gpu, dense64):
     assert torch.equal(outs[0][0], outs[1][0])
     for x, y in zip(outs[0][1:], outs[1][1:]):
         assert (x - y).abs().max() <= 1e-6 * y.abs().max()
+
+
+def _cell_map_np(links: np.ndarray, rows: int) -> np.ndarray:
+    """The cell map as the header defines it, by plain loops over blocks."""
+    s = links.shape
+    m = [(n - 2) // 2 + 1 for n in s]
+    out = np.zeros(m, np.uint16)
+    pad = np.full((s[0] + 2, s[1] + 2, s[2] + 2), -1, np.int64)
+    pad[:s[0], :s[1], :s[2]] = links
+    valid = (pad >= 0) & (pad < rows)
+    for bx in range(m[0]):
+        for by in range(m[1]):
+            for bz in range(m[2]):
+                levels, occ = set(), 0
+                for i in range(2):
+                    for j in range(2):
+                        for k in range(2):
+                            x, y, z = 2 * bx + i, 2 * by + j, 2 * bz + k
+                            if x > s[0] - 2 or y > s[1] - 2 or z > s[2] - 2:
+                                continue
+                            l0 = pad[x, y, z]
+                            levels.add(int(-l0 - 1) if l0 < -1 else 0)
+                            if valid[x:x + 2, y:y + 2, z:z + 2].any():
+                                occ |= 1 << (4 * i + 2 * j + k)
+                lv = levels.pop() if len(levels) == 1 else 255
+                out[bx, by, bz] = min(lv, 255) | (occ << 8)
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (17, 12, 9)])
+def test_cell_map_definition(gpu, shape):
+    """svox2_build_cell_map against the header's definition on random links
+    mixing rows, -1, skip levels and out-of-range values (odd sizes too)."""
+    g = np.random.default_rng(3)
+    links = g.integers(-4, 40, size=shape).astype(np.int32)
+    links[links > 30] = -1
+    cmap = _C.build_cell_map(torch.from_numpy(links).to(gpu), 25)
+    torch.cuda.synchronize()
+    got = cmap.cpu().numpy().view(np.uint16)
+    assert np.array_equal(got, _cell_map_np(links, 25))
+
+
+def test_cell_map_march_bit_exact(gpu, sparse256, march):
+    """The march with the cell map equals the oracle and the march without
+    it, bit for bit, on accelerated links and on links whose empty entries are
+    arbitrary negative values (blocks of mixed levels take the per-cell path)."""
+    sc, grid, links_acc, o, d = sparse256
+    g = np.random.default_rng(11)
+    mixed = links_acc.copy()
+    empty = mixed < 0
+    mixed[empty] = -g.integers(1, 6, size=int(empty.sum())).astype(np.int32)
+    for lk in (links_acc, mixed):
+        grid.links = torch.from_numpy(lk).to(gpu)
+        st = {}
+        for use_map in (None, False):
+            spec = grid._to_cpp()
+            spec.cell_map = use_map
+            rays = svox2_amd.Rays(o.to(gpu), d.to(gpu))
+            st[use_map] = _C.volume_render_cuvol_stats(spec, rays._to_cpp(),
+                                                       grid.opt._to_cpp()).cpu().numpy()
+        _, _, ref = march.render(host_grid(sc, links=lk), o.numpy(), d.numpy(), OPT)
+        assert np.array_equal(st[None], ref) and np.array_equal(st[False], ref)
+    grid.links = torch.from_numpy(links_acc).to(gpu)
