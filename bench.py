@@ -35,6 +35,23 @@ BATCH = 65536
 POOL = 8
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "fused_512_f32_65536.md")
+
+
+def measured_traffic():
+    """HBM bytes per fused call of the headline workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE passes of
+    `bench.py --pmc`, each scaled by what the counter reads for a 1 GiB copy),
+    or None if the summary is absent."""
+    try:
+        text = open(PROFILE_SUMMARY).read()
+    except OSError:
+        return None
+    import re
+    m = re.search(r"^traffic_bytes_per_call: (\d+)$", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -410,7 +427,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": "profiles/fused_512_f32_65536.md (rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE of `bench.py --pmc`, calibrated on a 1 GiB copy)",
             "kernel": "fused call (float32 storage, SH-2): k_render_rays, ray-order sort "
                       "(k_zero_words, k_cost_hist, k_cost_scan, k_cost_scatter), "
                       "k_render_backward_mse",

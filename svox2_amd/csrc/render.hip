@@ -195,6 +195,32 @@ __device__ __forceinline__ uint32_t group_ballot(bool pred) {
     return static_cast<uint32_t>(m >> (threadIdx.x & 32u));
 }
 
+// Sum over each aligned group of 8 lanes, in every lane of the group, by
+// data-parallel-primitive moves (VALU, no LDS round trip): lane ^ 1 and
+// lane ^ 2 by quad permutes, then the other quad by the half-row mirror
+// (lane 7 - i), which after the first two steps holds the other quad's sum.
+// Same additions in the same order as three xor-shuffle steps 1, 2, 4.
+// Every lane of the group must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float sum8(float v) {
+    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+    v += dpp<0x141>(v);  // row_half_mirror: lane 7 - i within 8
+    return v;
+}
+
+// Value of lane ^ BIT (BIT = 1, 2 or 4): quad permutes for 1 and 2, a
+// bit-mode swizzle (xor mask 4 within 32 lanes) for 4.
+template <int BIT>
+__device__ __forceinline__ float xor_lanes(float v) {
+    if constexpr (BIT == 1) return dpp<0xB1>(v);
+    else if constexpr (BIT == 2) return dpp<0x4E>(v);
+    else return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+}
+
 // Position (0-based) of the n-th set bit of m; m must have more than n bits set.
 __device__ __forceinline__ int nth_set_bit(uint32_t m, int n) {
     int pos = 0;
@@ -278,16 +304,42 @@ struct Chunk {
     float log_t_end;  // log transmittance after the chunk
 };
 
+// True when t0, t0 + step, ..., t0 + 31 step are all exactly representable in
+// the binade of t0: then the running sum t += step never rounds.  t0 must be
+// a positive normal float; the binade test uses the rounded end point, which
+// reaches the next power of two whenever the exact one does (rounding is
+// monotone and powers of two are representable), so it is conservative.
+__device__ __forceinline__ bool sum_chain_exact(float t0, float step) {
+    const uint32_t e = (__float_as_uint(t0) >> 23) & 0xFFu;
+    if (!(t0 > 0.f) || e == 0u || e >= 254u || !(step > 0.f)) return false;
+    const float t_end = fmaf(31.f, step, t0);
+    if (((__float_as_uint(t_end) >> 23) & 0xFFu) != e) return false;
+    // step / ulp(t0), ulp(t0) = 2^(e - 150): an exact power-of-two scaling.
+    const int sh = 150 - static_cast<int>(e);
+    if (sh > 100 || sh < -100) return false;
+    const float q = ldexpf(step, sh);
+    return isfinite(q) && q == floorf(q);
+}
+
 template <typename DT>
 __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView& o,
                                             const Ray& r, int lane, float t0, float log_t,
                                             Chunk& c) {
-    // t of the lane-th step, by sequential float additions.
-    float ti = t0;
+    // t of the lane-th step, as the sequential float additions t += step
+    // give it.  When every t of the chunk lies in the binade of t0 (t0 > 0
+    // normal) and step is a whole multiple of that binade's ulp, each of those
+    // additions is exact, so t0 + lane * step (one fma, exact) is the same
+    // float; otherwise the chain is replayed.
+    float ti;
+    if (sum_chain_exact(t0, o.step)) {
+        ti = fmaf(static_cast<float>(lane), o.step, t0);
+    } else {
+        ti = t0;
 #pragma unroll
-    for (int k = 1; k < 32; ++k) {
-        const float nx = ti + o.step;
-        ti = (k <= lane) ? nx : ti;
+        for (int k = 1; k < 32; ++k) {
+            const float nx = ti + o.step;
+            ti = (k <= lane) ? nx : ti;
+        }
     }
     c.t = ti;
     const bool in_range = ti <= r.tmax;
@@ -490,15 +542,9 @@ __device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx
         v1 *= w;
         v2 *= w;
     }
-#pragma unroll
-    for (int m = 1; m < 8; m <<= 1) {
-        v0 += __shfl_xor(v0, m, kGroup);
-        v1 += __shfl_xor(v1, m, kGroup);
-        v2 += __shfl_xor(v2, m, kGroup);
-    }
-    col[0] = v0;
-    col[1] = v1;
-    col[2] = v2;
+    col[0] = sum8(v0);
+    col[1] = sum8(v1);
+    col[2] = sum8(v2);
 }
 
 // Ray ordering for the backward launch.  A ray's backward costs close to a
@@ -516,6 +562,78 @@ __device__ __forceinline__ uint32_t backward_cost(uint32_t n_chunks) {
     return n_chunks < static_cast<uint32_t>(kCostBins) ? n_chunks
                                                        : static_cast<uint32_t>(kCostBins - 1);
 }
+
+// Colours of a chunk's live samples, grouped by cell.  Consecutive live
+// samples in the same cell form a run; they share the 8 corner rows, so
+// each row is read and projected on the SH basis once per run.  Up to four
+// runs per pass, lane = (run, corner): the lane loads its corner's row of
+// the run's cell and keeps the three projections, then for each sample of
+// the run in turn weights them trilinearly at the sample's position and sums
+// the 8 corners.  emit(pass, i, act, s, col) is called by every lane for the
+// i-th sample of each pass's runs: act says whether the lane's group has a
+// sample s at step i, col (identical in the group's 8 lanes) is its raw
+// colour.  The per-sample colour is the same float computation as a
+// per-sample gather.
+template <typename DT, int B, typename Emit>
+__device__ __forceinline__ void chunk_run_colors(const GridView<DT>& g, const RayCtx<B>& rc,
+                                                 const Chunk& c, int lane, uint32_t head_mask,
+                                                 Emit&& emit) {
+    const int n_heads = __popc(head_mask);
+    const int k = lane & 7;
+    for (int pass = 0; pass * 4 < n_heads; ++pass) {
+        const int h = pass * 4 + (lane >> 3);
+        const bool has = h < n_heads;
+        const int src = has ? nth_set_bit(head_mask, h) : 0;
+        const uint32_t later = src < 31 ? (head_mask >> (src + 1)) << (src + 1) : 0u;
+        const uint32_t below_end = later ? ((1u << (__ffs(later) - 1)) - 1u) : 0xFFFFFFFFu;
+        uint32_t run = has ? (c.live_mask & below_end & (0xFFFFFFFFu << src)) : 0u;
+        int lk = __shfl(c.lk[0], src, kGroup);
+#pragma unroll
+        for (int kk = 1; kk < 8; ++kk) {
+            const int v = __shfl(c.lk[kk], src, kGroup);
+            lk = (k == kk) ? v : lk;
+        }
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        if (has && g.valid(lk)) {
+            constexpr int C = 3 * B;
+            float x[C];
+            load_row<C>(g, lk, x);
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                c0 = fmaf(rc.basis[j], x[j], c0);
+                c1 = fmaf(rc.basis[j], x[B + j], c1);
+                c2 = fmaf(rc.basis[j], x[2 * B + j], c2);
+            }
+        }
+        for (int i = 0;; ++i) {
+            const bool act = run != 0u;
+            if (!__any(act)) break;
+            const int s = act ? __ffs(run) - 1 : 0;
+            run &= run - 1u;
+            float p[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] = __shfl(c.p[a], s, kGroup);
+            const float w = corner_weight(k, p);
+            float col[3];
+            col[0] = sum8(c0 * w);
+            col[1] = sum8(c1 * w);
+            col[2] = sum8(c2 * w);
+            emit(pass, i, act, s, col);
+        }
+    }
+}
+
+// Run heads of a chunk: live samples whose previous lane is not a live
+// sample of the same cell.
+__device__ __forceinline__ uint32_t run_heads(const Chunk& c, int lane) {
+    const int prev_cell = __shfl_up(c.base, 1, kGroup);
+    const bool prev_live = lane > 0 && ((c.live_mask >> (lane - 1)) & 1u);
+    return group_ballot(c.live && !(prev_live && prev_cell == c.base));
+}
+
+#ifndef SVOX2_COLOUR_RUNS
+#define SVOX2_COLOUR_RUNS 1
+#endif
 
 // Forward march of one ray.  out[3] = composited colour (all lanes),
 // returns the final log transmittance.
@@ -538,15 +656,29 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
             march_chunk(g, o, r, lane, t, log_t, c);
             const int n_live = __popc(c.live_mask);
             ++n_chunks;
-            for (int pass = 0; pass * 4 < n_live; ++pass) {
-                float col[3];
-                int src;
-                bool has;
-                chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
-                const float w = __shfl(c.weight, src, kGroup);
-                if (has && (lane & 7) == 0) {
+            if (SVOX2_COLOUR_RUNS) {
+                if (n_live > 0) {
+                    chunk_run_colors<DT, B>(
+                        g, rc, c, lane, run_heads(c, lane),
+                        [&](int, int, bool act, int s, const float* col) {
+                            const float w = __shfl(c.weight, s, kGroup);
+                            if (act && (lane & 7) == 0) {
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
+                                for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
+                            }
+                        });
+                }
+            } else {
+                for (int pass = 0; pass * 4 < n_live; ++pass) {
+                    float col[3];
+                    int src;
+                    bool has;
+                    chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
+                    const float w = __shfl(c.weight, src, kGroup);
+                    if (has && (lane & 7) == 0) {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
+                    }
                 }
             }
             if (STATS) {
@@ -646,8 +778,8 @@ struct Scatter {
         constexpr uint32_t lo = bit == 4 ? 0x0Fu : (bit == 2 ? 0x33u : 0x55u);
         const bool hi_lane = (lane & bit) != 0;
         flush(g, go, lane, basis_j, d > 0 ? lo : (~lo & 0xFFu));
-        const float a = __shfl_xor(acc, bit, kGroup);
-        const int lk = __shfl_xor(link, bit, kGroup);
+        const float a = xor_lanes<bit>(acc);
+        const int lk = __float_as_int(xor_lanes<bit>(__int_as_float(link)));
         if (d > 0) {
             acc = hi_lane ? 0.f : a;
             link = hi_lane ? -1 : lk;
@@ -690,19 +822,41 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
         const int n_live = __popc(c.live_mask);
         if (n_live > 0) {
             // Colours of the live samples, moved to each sample's own lane.
-            const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
             float cs[3] = {0.f, 0.f, 0.f};
-            for (int pass = 0; pass * 4 < n_live; ++pass) {
-                float col[3];
-                int src;
-                bool has;
-                chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
-                const int from = ((rank - pass * 4) & 3) * 8;
-                const bool mine = c.live && rank >= pass * 4 && rank < pass * 4 + 4;
+            if (SVOX2_COLOUR_RUNS) {
+                const uint32_t heads = run_heads(c, lane);
+                const uint32_t upto = (lane == 31) ? 0xFFFFFFFFu : ((2u << lane) - 1u);
+                const uint32_t my_heads = heads & upto;
+                // This lane's run (head index) and its place in the run.
+                const int my_run = __popc(my_heads) - 1;
+                const int head_pos = my_heads ? 31 - __clz(my_heads) : 0;
+                const int my_idx = __popc(c.live_mask & ((1u << lane) - 1u) &
+                                          (0xFFFFFFFFu << head_pos));
+                chunk_run_colors<DT, B>(
+                    g, rc, c, lane, heads, [&](int pass, int i, bool, int, const float* col) {
+                        const int grp = my_run - 4 * pass;
+                        const bool mine = c.live && grp >= 0 && grp < 4 && my_idx == i;
+                        const int from = mine ? 8 * grp : 0;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float v = __shfl(col[a], from, kGroup);
-                    cs[a] = mine ? v + 0.5f : cs[a];
+                        for (int a = 0; a < 3; ++a) {
+                            const float v = __shfl(col[a], from, kGroup);
+                            cs[a] = mine ? v + 0.5f : cs[a];
+                        }
+                    });
+            } else {
+                const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
+                for (int pass = 0; pass * 4 < n_live; ++pass) {
+                    float col[3];
+                    int src;
+                    bool has;
+                    chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
+                    const int from = ((rank - pass * 4) & 3) * 8;
+                    const bool mine = c.live && rank >= pass * 4 && rank < pass * 4 + 4;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float v = __shfl(col[a], from, kGroup);
+                        cs[a] = mine ? v + 0.5f : cs[a];
+                    }
                 }
             }
             // Per-sample gradient scalars.
