@@ -256,11 +256,11 @@ def serve_requests(cam_sets, n: int, qps: float, seed: int):
     return reqs
 
 
-def serve_bench(wl: Workload, qps: float = 200.0, n: int = 600):
+def serve_bench(wl: Workload, qps_list=(200.0, 600.0, 1000.0), n: int = 600):
     """The serving loop on four resident grids (256^3 ~8% and 512^3 ~3%
-    occupied, one of each in bf16): the same Poisson request stream served by
-    the continuous-batching server and by one volume_render_image launch per
-    request."""
+    occupied, one of each in bf16): at each rate, the same Poisson request
+    stream served by the continuous-batching server and by one
+    volume_render_image launch per request."""
     from svox2_amd import scenes
     from svox2_amd.serve import RenderServer, serve_one_per_launch
     sc256, cams256, _ = scenes.workload_blobs(256, scenes.SPARSE256, seed=1, device=wl.dev)
@@ -272,15 +272,19 @@ def serve_bench(wl: Workload, qps: float = 200.0, n: int = 600):
     grids = [g256[torch.float32], g256[torch.bfloat16], wl.grid(torch.float32),
              wl.grid(torch.bfloat16)]
     cam_sets = [cams256, cams256, wl.cams, wl.cams]
-    server = RenderServer(grids, hbm_budget=1 << 30, tile_pixels=8192, max_tiles=64,
-                          max_active=64)
-    server.run(serve_requests(cam_sets, 16, qps, seed=100), realtime=False)  # warm-up
-    out = {"qps": qps, "requests": n, "sides": list(SERVE_SIDES),
-           "grids": "256^3 f32, 256^3 bf16, 512^3 f32, 512^3 bf16"}
-    out["batched"] = server.run(serve_requests(cam_sets, n, qps, seed=7))
-    out["batched"]["steps"] = server.steps
-    serve_one_per_launch(grids, serve_requests(cam_sets, 8, qps, seed=101), realtime=False)
-    out["one_per_launch"] = serve_one_per_launch(grids, serve_requests(cam_sets, n, qps, seed=7))
+    server = RenderServer(grids, hbm_budget=1 << 30, tile_pixels=4096, max_tiles=128,
+                          max_active=64, policy="fifo", depth=3)
+    server.run(serve_requests(cam_sets, 16, 100.0, seed=100), realtime=False)  # warm-up
+    serve_one_per_launch(grids, serve_requests(cam_sets, 8, 100.0, seed=101), realtime=False)
+    out = {"requests": n, "sides": list(SERVE_SIDES), "tile_pixels": 4096, "max_tiles": 128,
+           "policy": "fifo", "depth": 3,
+           "grids": "256^3 f32, 256^3 bf16, 512^3 f32, 512^3 bf16", "rates": []}
+    for qps in qps_list:
+        steps0 = server.steps
+        batched = server.run(serve_requests(cam_sets, n, qps, seed=7))
+        batched["steps"] = server.steps - steps0
+        plain = serve_one_per_launch(grids, serve_requests(cam_sets, n, qps, seed=7))
+        out["rates"].append({"qps": qps, "batched": batched, "one_per_launch": plain})
     return out
 
 
@@ -383,12 +387,13 @@ def main():
         extra["stress_1m_fp32"] = stress_1m(wl, g32)
         log(f"1M rays: {extra['stress_1m_fp32']['ms_per_step']:.2f} ms/step, peak HBM "
             f"{extra['stress_1m_fp32']['peak_hbm_gb']:.2f} GB")
-        log("serving loop, 200 QPS")
+        log("serving loop")
         extra["serve"] = serve_bench(wl)
-        sv = extra["serve"]
-        log(f"serve: batched p50 {sv['batched']['p50_ms']:.2f} ms p99 {sv['batched']['p99_ms']:.2f} "
-            f"ms {sv['batched']['rays_per_s']:.3g} rays/s; one-per-launch p50 "
-            f"{sv['one_per_launch']['p50_ms']:.2f} ms p99 {sv['one_per_launch']['p99_ms']:.2f} ms")
+        for rt in extra["serve"]["rates"]:
+            b, p1 = rt["batched"], rt["one_per_launch"]
+            log(f"serve {rt['qps']:.0f} QPS: batched p50 {b['p50_ms']:.2f} ms p99 "
+                f"{b['p99_ms']:.2f} ms {b['rays_per_s']:.3g} rays/s; one-per-launch p50 "
+                f"{p1['p50_ms']:.2f} ms p99 {p1['p99_ms']:.2f} ms {p1['rays_per_s']:.3g} rays/s")
         log("800x800 image forward")
         extra["image_800_fp32"] = image_forward(g32, wl.cams)
         extra["image_800_bf16"] = image_forward(gb, wl.cams)
@@ -437,13 +442,11 @@ def main():
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
             "counters_per_launch": head["counters"],
         },
-        "serve": ({"qps": extra["serve"]["qps"],
-                   "p50_ms": extra["serve"]["batched"]["p50_ms"],
-                   "p99_ms": extra["serve"]["batched"]["p99_ms"],
-                   "rays_per_s": extra["serve"]["batched"]["rays_per_s"],
-                   "one_per_launch_p50_ms": extra["serve"]["one_per_launch"]["p50_ms"],
-                   "one_per_launch_p99_ms": extra["serve"]["one_per_launch"]["p99_ms"]}
-                  if "serve" in extra else None),
+        "serve": ([{"qps": rt["qps"], "p50_ms": rt["batched"]["p50_ms"],
+                    "p99_ms": rt["batched"]["p99_ms"], "rays_per_s": rt["batched"]["rays_per_s"],
+                    "one_per_launch_p50_ms": rt["one_per_launch"]["p50_ms"],
+                    "one_per_launch_p99_ms": rt["one_per_launch"]["p99_ms"]}
+                   for rt in extra["serve"]["rates"]] if "serve" in extra else None),
         "cpu_baseline": cpu,
         "detail": {"fp32_65536": head, **extra},
     }

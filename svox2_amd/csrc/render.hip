@@ -723,6 +723,36 @@ struct GradOut {
     int64_t gss;
 };
 
+#ifndef SVOX2_SCATTER_LDS
+#define SVOX2_SCATTER_LDS 1
+#endif
+
+// Per-sample record a backward chunk stages in LDS for its scatter loop: every
+// lane of the group then reads the current sample's record at one address (a
+// broadcast read) instead of gathering each field from the sample's lane.
+struct __align__(16) SampleRec {
+    float p[3];
+    int base;
+    float val[4];  // colour factors R, G, B and the density gradient
+    int l[3];
+    int pad;
+    int lk[8];
+};
+
+// LDS of one backward workgroup: the sample records of its rays and the
+// accumulators / corner links of the scatter, one entry per lane.
+struct BwdShared {
+    SampleRec rec[kBlock];
+    float acc[kBlock];
+    int link[kBlock];
+};
+
+// Orders this wavefront's LDS writes before its later LDS reads.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Gradient scatter state of one ray.  Lane L of the 32-lane group holds the
 // accumulator of (corner k = L & 7, quantity q = L >> 3) of the cell the ray is
 // in: q = 0, 1, 2 the colour weight factor of corner k for R, G, B (the SH
@@ -741,13 +771,22 @@ struct Scatter {
     float acc = 0.f;     // this lane's (k, q) accumulator
     int link = -1;       // link of corner k (identical for the 4 q lanes)
     uint32_t used = 0;   // corners holding gradient
+    float* s_acc = nullptr;  // this group's LDS accumulator slots (32)
+    int* s_link = nullptr;   // this group's LDS link slots (32)
 
     template <typename DT>
     __device__ __forceinline__ void flush_corner(const GridView<DT>& g, const GradOut& go,
                                                  int lane, float basis_j, int k) const {
-        const int lk = __shfl(link, k, kGroup);
         const int src = lane < C ? k + 8 * (lane / B) : k + 24;
-        float v = __shfl(acc, src, kGroup);
+        int lk;
+        float v;
+        if (SVOX2_SCATTER_LDS) {
+            lk = s_link[k];
+            v = s_acc[src];
+        } else {
+            lk = __shfl(link, k, kGroup);
+            v = __shfl(acc, src, kGroup);
+        }
         if (g.valid(lk)) {
             if (lane < C) {
                 if (go.gs != nullptr)
@@ -764,7 +803,13 @@ struct Scatter {
                                           float basis_j, uint32_t which) {
         which &= used;
         if (which == 0) return;
+        if (SVOX2_SCATTER_LDS) {
+            s_acc[lane] = acc;
+            s_link[lane] = link;
+            wave_lds_fence();
+        }
         for (uint32_t m = which; m; m &= m - 1) flush_corner(g, go, lane, basis_j, __ffs(m) - 1);
+        if (SVOX2_SCATTER_LDS) wave_lds_fence();  // slots are rewritten by the next flush
         if (go.mask != nullptr && lane < 8 && ((which >> lane) & 1u) && g.valid(link))
             go.mask[link] = 1;
         used &= ~which;
@@ -804,13 +849,21 @@ template <typename DT, int B>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                const float* gq, float accum, float sparsity,
-                                               const GradOut& go) {
+                                               const GradOut& go, BwdShared* sh) {
     if (r.tmin > r.tmax) return;
+    const int gbase = threadIdx.x & ~(kGroup - 1);
+    SampleRec* rec = sh->rec + gbase;
     constexpr int C = 3 * B;
-    const float basis_j = lane < C ? rc.basis[lane % B] : 0.f;
+    // basis_j of this lane's SH channel (lane = c * B + j), by selects: a
+    // register array indexed by lane would go to scratch memory.
+    float basis_j = 0.f;
+#pragma unroll
+    for (int j = 0; j < B; ++j) basis_j = (lane < C && lane % B == j) ? rc.basis[j] : basis_j;
     const int my_k = lane & 7;
     const int my_q = lane >> 3;
     Scatter<B> sc;
+    sc.s_acc = sh->acc + gbase;
+    sc.s_link = sh->link + gbase;
 
     float t = r.tmin;
     float log_t = 0.f;
@@ -878,6 +931,49 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
             for (int a = 0; a < 3; ++a)
                 gc[a] = c.weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * gq[a];
             // Accumulate in sample order.
+            if (SVOX2_SCATTER_LDS) {
+                if (c.live) {
+                    SampleRec& R = rec[lane];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        R.p[a] = c.p[a];
+                        R.l[a] = c.l[a];
+                        R.val[a] = gc[a];
+                    }
+                    R.base = c.base;
+                    R.val[3] = gsig;
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) R.lk[kk] = c.lk[kk];
+                }
+                wave_lds_fence();
+                for (uint32_t m = c.live_mask; m; m &= m - 1) {
+                    const SampleRec& R = rec[__ffs(m) - 1];
+                    const int cell = R.base;
+                    if (cell != sc.cur) {
+                        const int l0 = R.l[0], l1 = R.l[1], l2 = R.l[2];
+                        const int d0 = l0 - sc.c0, d1 = l1 - sc.c1, d2 = l2 - sc.c2;
+                        const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 &&
+                                              d1 <= 1 && d2 >= -1 && d2 <= 1;
+                        if (adjacent) {
+                            sc.template shift<4>(d0, g, go, lane, basis_j);
+                            sc.template shift<2>(d1, g, go, lane, basis_j);
+                            sc.template shift<1>(d2, g, go, lane, basis_j);
+                        } else {
+                            sc.flush(g, go, lane, basis_j, 0xFFu);
+                            sc.acc = 0.f;
+                        }
+                        sc.link = R.lk[my_k];
+                        sc.cur = cell;
+                        sc.c0 = l0;
+                        sc.c1 = l1;
+                        sc.c2 = l2;
+                    }
+                    const float p[3] = {R.p[0], R.p[1], R.p[2]};
+                    sc.acc = fmaf(corner_weight(my_k, p), R.val[my_q], sc.acc);
+                    sc.used = 0xFFu;
+                }
+                wave_lds_fence();  // records are rewritten by the next chunk
+            } else
             for (uint32_t m = c.live_mask; m; m &= m - 1) {
                 const int s = __ffs(m) - 1;
                 const int cell = __shfl(c.base, s, kGroup);
@@ -1029,6 +1125,8 @@ SVOX2_FWD_KERNEL void k_render_image(
 
 // Pixels of many image requests per launch.  blockIdx.y selects the tile;
 // tiles past *n_tiles, and pixels past a tile's n_pixels, exit at once.
+// (One workgroup per item lets the hardware balance uneven pixels; a fixed
+// grid striding over the items measured slower.)
 template <typename DT, int B>
 SVOX2_FWD_KERNEL void k_render_tiles(const svox2_grid_t* __restrict__ grids, int n_grids,
                                      const svox2_camera_t* __restrict__ cams, int n_cams,
@@ -1075,7 +1173,8 @@ SVOX2_BWD_KERNEL void k_render_backward(
     const float gq[3] = {grad_out[3 * ray], grad_out[3 * ray + 1], grad_out[3 * ray + 2]};
     const float* cc = color_cache + 3 * ray;
     const float accum = fmaf(cc[0], gq[0], fmaf(cc[1], gq[1], cc[2] * gq[2]));
-    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, 0.f, go);
+    __shared__ BwdShared sh;
+    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, 0.f, go, &sh);
 }
 
 // Backward of the fused MSE step: dL/drgb = (rgb - gt) * 2 / (3n) from the
@@ -1109,7 +1208,8 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
         const float tin = __expf(log_t_fwd[ray]);
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
-    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go);
+    __shared__ BwdShared sh;
+    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go, &sh);
 }
 
 // Counting sort of the rays by descending cost: histogram, scan, scatter.

@@ -12,14 +12,17 @@ How a request flows:
   (H x W x 3 float32) fits in the HBM budget next to the images of the
   requests in flight.
 * Continuous batching: every ``step`` builds one tile table per launch group
-  (grids of one storage kind and basis) from all admitted requests. It hands
-  out one tile of ``tile_pixels`` pixels per request per round, until the
-  table holds ``max_tiles`` tiles. A request admitted during a step joins the
-  next step, and small requests finish in a few steps even while large ones
-  are in flight.
+  (grids of one storage kind and basis) from all admitted requests: their
+  remaining tiles of ``tile_pixels`` pixels in admission order, until the
+  table holds ``max_tiles`` tiles (``policy="round_robin"`` instead hands out
+  one tile per request per round). A request admitted during a step joins the
+  next step, and many requests share one launch.
 * Each step copies the tables to fixed device buffers and replays the group's
   graph. The graph is one ``svox2_volume_render_cuvol_tiles`` launch captured
   once, whose shape depends only on ``max_tiles`` and ``tile_pixels``.
+* Steps are pipelined over two table slots, each with its own buffers and
+  graphs: the host plans and uploads step k+1 while step k runs, and it
+  learns of completions from per-step events.
 * A request completes once all its tiles have run. Its image is then
   bit-identical to ``volume_render_image`` of the same grid and camera,
   because every pixel runs the same march code.
@@ -68,13 +71,26 @@ class RenderRequest:
         return self.done - self.arrival
 
 
-def plan_tiles(active: Sequence[RenderRequest], max_tiles: int,
-               tile_pixels: int) -> List[tuple]:
-    """Round-robin tile plan for one launch group: (request, pixel0, n)
-    entries, one tile per request per round in admission order, at most
-    max_tiles in total.  Advances each request's next_pixel."""
+def plan_tiles(active: Sequence[RenderRequest], max_tiles: int, tile_pixels: int,
+               policy: str = "fifo") -> List[tuple]:
+    """Tile plan for one launch group: (request, pixel0, n) entries, at most
+    max_tiles, advancing each request's next_pixel.  ``fifo`` gives the
+    requests their remaining tiles in admission order (a request waits
+    behind earlier ones, as with one launch per request, but many requests
+    share a launch); ``round_robin`` gives one tile per request per round."""
     plan: List[tuple] = []
     live = [r for r in active if r.next_pixel < r.pixels]
+    if policy == "fifo":
+        for r in live:
+            while r.next_pixel < r.pixels and len(plan) < max_tiles:
+                n = min(tile_pixels, r.pixels - r.next_pixel)
+                plan.append((r, r.next_pixel, n))
+                r.next_pixel += n
+            if len(plan) >= max_tiles:
+                break
+        return plan
+    if policy != "round_robin":
+        raise ValueError(f"unknown tile policy {policy!r}")
     while live and len(plan) < max_tiles:
         nxt = []
         for r in live:
@@ -89,6 +105,11 @@ def plan_tiles(active: Sequence[RenderRequest], max_tiles: int,
     return plan
 
 
+# svox2_tile_t as a numpy record (int32 grid, camera, pixel0, n_pixels; float* rgb_out).
+_TILE_DTYPE = np.dtype([("grid", "<i4"), ("camera", "<i4"), ("pixel0", "<i4"),
+                        ("n_pixels", "<i4"), ("rgb_out", "<u8")])
+
+
 class _Group:
     """Grids of one storage dtype and basis_dim, rendered by one launch."""
 
@@ -98,15 +119,31 @@ class _Group:
         self.grid_ids = grid_ids  # server grid index -> position in this group's table
         self.local: Dict[int, int] = {g: i for i, g in enumerate(grid_ids)}
         self.grids_dev: Optional[torch.Tensor] = None
-        self.tiles_host: Optional[torch.Tensor] = None
-        self.tiles_dev: Optional[torch.Tensor] = None
-        self.count_host: Optional[torch.Tensor] = None
-        self.count_dev: Optional[torch.Tensor] = None
-        self.graph = None
+
+
+class _Slot:
+    """One in-flight step's tables: a device buffer [count | tiles | cameras]
+    per launch group, its pinned staging copy, captured graphs and stream."""
+
+    def __init__(self, device):
+        self.stream = torch.cuda.Stream(device=device)  # default priority
+        self.cam_version = -1  # camera table version last copied to this slot
+        self.host: List[torch.Tensor] = []
+        self.host_np: List[np.ndarray] = []
+        self.dev: List[torch.Tensor] = []
+        self.graphs: list = []  # per launch group: {tile-count class: graph}
+        self.event = None
+        self.busy = False
+        self.finishing: List[RenderRequest] = []
 
 
 class RenderServer:
     """Continuous-batching renderer over resident grids (see module doc).
+
+    Steps are pipelined: ``depth`` table slots, each with its own captured
+    graphs and stream, so the host plans and uploads step k+1 while step k
+    runs; a request completes when the event of the step that ran its last
+    tile has fired.
 
     :param grids: resident SparseGrids (on the GPU; their cell maps are built
         once here).
@@ -114,13 +151,13 @@ class RenderServer:
     :param tile_pixels: pixels per tile.
     :param max_tiles: tiles per launch and group.
     :param max_active: camera slots (requests in flight) at most.
-    :param use_graph: replay a captured graph per launch group (else launch
-        directly).
+    :param use_graph: replay captured graphs (else launch directly).
     """
 
     def __init__(self, grids: Sequence[SparseGrid], opt: Optional[RenderOptions] = None,
-                 hbm_budget: int = 1 << 30, tile_pixels: int = 8192, max_tiles: int = 64,
-                 max_active: int = 64, use_graph: bool = True):
+                 hbm_budget: int = 1 << 30, tile_pixels: int = 4096, max_tiles: int = 128,
+                 max_active: int = 64, use_graph: bool = True, policy: str = "fifo",
+                 depth: int = 3):
         if not grids:
             raise ValueError("RenderServer needs at least one grid")
         self.grids = list(grids)
@@ -133,17 +170,24 @@ class RenderServer:
         self.max_tiles = int(max_tiles)
         self.max_active = int(max_active)
         self.use_graph = use_graph
+        self.policy = policy
+        if depth < 1:
+            raise ValueError("depth must be at least 1")
+        self.depth = int(depth)
         self.waiting: List[RenderRequest] = []
         self.active: List[RenderRequest] = []
-        self.free_slots = list(range(self.max_active))
+        self.free_cams = list(range(self.max_active))
         self.bytes_in_flight = 0
         self.steps = 0
         self.rays_rendered = 0
-        cam_size = ctypes.sizeof(_C._Cam)
-        self.cams_dev = torch.zeros(self.max_active * cam_size, dtype=torch.uint8,
-                                    device=self.device)
-        self.cam_host = torch.zeros(cam_size, dtype=torch.uint8).pin_memory()
-        # Launch groups by (storage, basis_dim).
+        self.cam_size = ctypes.sizeof(_C._Cam)
+        self.tile_size = ctypes.sizeof(_C._Tile)
+        self.cams = bytearray(self.max_active * self.cam_size)  # host master copy
+        self.cam_version = 0
+        # Table layout: [count (16 B) | tiles | cameras].
+        self.tiles_off = 16
+        self.cams_off = self.tiles_off + self.max_tiles * self.tile_size
+        self.table_bytes = self.cams_off + self.max_active * self.cam_size
         groups: Dict[tuple, List[int]] = {}
         for i, g in enumerate(self.grids):
             if g.density_data.device != self.device:
@@ -152,7 +196,6 @@ class RenderServer:
             groups.setdefault(key, []).append(i)
         self.groups = [_Group(k[0], k[1], ids) for k, ids in groups.items()]
         self.group_of = {gid: grp for grp in self.groups for gid in grp.grid_ids}
-        tile_size = ctypes.sizeof(_C._Tile)
         # The device grid tables hold raw pointers: the server keeps every
         # tensor they point at, the cell maps included, alive.
         self._maps = []
@@ -164,29 +207,44 @@ class RenderServer:
                 self._maps.append(spec.cell_map)
                 structs.append(_C.grid_struct(spec))
             grp.grids_dev = _C.pack_structs(structs).to(self.device)
-            grp.tiles_host = torch.zeros(self.max_tiles * tile_size, dtype=torch.uint8).pin_memory()
-            grp.tiles_dev = torch.zeros(self.max_tiles * tile_size, dtype=torch.uint8,
-                                        device=self.device)
-            grp.count_host = torch.zeros(1, dtype=torch.int32).pin_memory()
-            grp.count_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.slots = [_Slot(self.device) for _ in range(self.depth)]
+        for slot in self.slots:
+            for _ in self.groups:
+                slot.host.append(torch.zeros(self.table_bytes, dtype=torch.uint8).pin_memory())
+                slot.host_np.append(slot.host[-1].numpy())
+                slot.dev.append(torch.zeros(self.table_bytes, dtype=torch.uint8,
+                                            device=self.device))
+        # Launch sizes: a step with few tiles replays a graph with a small
+        # grid, since every workgroup of the grid is dispatched even when its
+        # tile is not live.
+        self.size_classes = sorted({c for c in (8, 32, self.max_tiles) if c <= self.max_tiles})
         # The kernel is loaded and run once (empty tables) before any capture.
-        for grp in self.groups:
-            self._launch(grp)
+        for slot in self.slots:
+            with torch.cuda.stream(slot.stream):
+                for gi, grp in enumerate(self.groups):
+                    self._launch(grp, slot.dev[gi], self.max_tiles)
         torch.cuda.synchronize(self.device)
         if self.use_graph:
-            for grp in self.groups:
-                graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph):
-                    self._launch(grp)
-                grp.graph = graph
+            for slot in self.slots:
+                for gi, grp in enumerate(self.groups):
+                    per_size = {}
+                    for size in self.size_classes:
+                        graph = torch.cuda.CUDAGraph()
+                        with torch.cuda.graph(graph):
+                            self._launch(grp, slot.dev[gi], size)
+                        per_size[size] = graph
+                    slot.graphs.append(per_size)
             torch.cuda.synchronize(self.device)
+        self._next_slot = 0
+        self._inflight: List[_Slot] = []
+        self._clock = time.perf_counter
 
     # -- device work ---------------------------------------------------------
-    def _launch(self, grp: _Group) -> None:
+    def _launch(self, grp: _Group, table: torch.Tensor, size: int) -> None:
         _C.volume_render_cuvol_tiles(grp.grids_dev, len(grp.grid_ids), grp.storage,
-                                     grp.basis_dim, self.cams_dev, self.max_active,
-                                     self.opt._to_cpp(), grp.tiles_dev, grp.count_dev,
-                                     self.max_tiles, self.tile_pixels)
+                                     grp.basis_dim, table[self.cams_off:], self.max_active,
+                                     self.opt._to_cpp(), table[self.tiles_off:self.cams_off],
+                                     table[:4].view(torch.int32), size, self.tile_pixels)
 
     # -- scheduling ------------------------------------------------------------
     def submit(self, req: RenderRequest) -> None:
@@ -197,86 +255,145 @@ class RenderServer:
         self.waiting.append(req)
 
     def _admit(self, now: float) -> None:
-        while self.waiting and self.free_slots:
+        while self.waiting and self.free_cams:
             req = self.waiting[0]
             if self.bytes_in_flight + req.image_bytes > self.hbm_budget:
                 break
             self.waiting.pop(0)
-            req.slot = self.free_slots.pop(0)
+            req.slot = self.free_cams.pop(0)
             req.image = torch.empty((int(req.camera.height), int(req.camera.width), 3),
                                     dtype=torch.float32, device=self.device)
             req.next_pixel = 0
             req.admitted = now
             self.bytes_in_flight += req.image_bytes
             cam = _C.camera_struct(req.camera._to_cpp())
-            ctypes.memmove(self.cam_host.data_ptr(), ctypes.addressof(cam), ctypes.sizeof(cam))
-            size = ctypes.sizeof(cam)
-            self.cams_dev[req.slot * size:(req.slot + 1) * size].copy_(self.cam_host,
-                                                                       non_blocking=True)
-            # The staging buffer is reused: wait for this copy to leave it.
-            torch.cuda.current_stream(self.device).synchronize()
+            o = req.slot * self.cam_size
+            self.cams[o:o + self.cam_size] = bytes(memoryview(cam))
+            self.cam_version += 1
             self.active.append(req)
 
-    def step(self, now: float) -> List[RenderRequest]:
-        """Admit what fits, run one multi-request launch per group, wait for
-        it; returns the requests that completed (their ``done`` set to the
-        clock after the wait, via ``clock`` in :meth:`run`)."""
-        self._admit(now)
-        if not self.active:
-            return []
-        tile_size = ctypes.sizeof(_C._Tile)
-        for grp in self.groups:
-            mine = [r for r in self.active if self.group_of[r.grid] is grp]
-            plan = plan_tiles(mine, self.max_tiles, self.tile_pixels)
-            structs = [_C.tile_struct(grp.local[r.grid], r.slot, p0, n, r.image)
-                       for r, p0, n in plan]
-            grp.count_host[0] = len(plan)
-            if plan:
-                blob = _C.pack_structs(structs)
-                grp.tiles_host[:blob.numel()].copy_(blob)
-                grp.tiles_dev[:blob.numel()].copy_(grp.tiles_host[:blob.numel()],
-                                                   non_blocking=True)
-                self.rays_rendered += sum(n for _, _, n in plan)
-            grp.count_dev.copy_(grp.count_host, non_blocking=True)
-            if plan:
-                if grp.graph is not None:
-                    grp.graph.replay()
-                else:
-                    self._launch(grp)
-        self.steps += 1
-        torch.cuda.current_stream(self.device).synchronize()
-        finished = [r for r in self.active if r.next_pixel >= r.pixels]
-        for r in finished:
+    def _retire(self, slot: _Slot) -> List[RenderRequest]:
+        slot.event.synchronize()
+        t = self._clock()
+        slot.busy = False
+        done = slot.finishing
+        slot.finishing = []
+        for r in done:
+            r.done = t
             self.active.remove(r)
-            self.free_slots.append(r.slot)
+            self.free_cams.append(r.slot)
             self.bytes_in_flight -= r.image_bytes
-        return finished
+        return done
+
+    def poll(self) -> List[RenderRequest]:
+        """Requests whose last step has finished on the device (no waiting)."""
+        done: List[RenderRequest] = []
+        while self._inflight and self._inflight[0].event.query():
+            done += self._retire(self._inflight.pop(0))
+        return done
+
+    def step(self, now: float) -> List[RenderRequest]:
+        """Admit what fits and launch one multi-request step (waiting first
+        for the oldest in-flight step if both table slots are busy).
+        Returns the requests that completed meanwhile."""
+        done = self.poll()
+        self._admit(now)
+        pending = [r for r in self.active if r.next_pixel < r.pixels]
+        if not pending:
+            return done
+        slot = self.slots[self._next_slot]
+        if slot.busy:
+            done += self._retire(self._inflight.pop(0))
+        self._next_slot = (self._next_slot + 1) % self.depth
+        finishing: List[RenderRequest] = []
+        # Steps run on the two slots' streams, so a step can start while the
+        # previous one drains.  They share nothing a step writes: each slot
+        # has its own tables, tiles of one image are disjoint, and camera
+        # slots and images are reused only after the event of the step that
+        # last read them.  The slot stream follows the caller's stream, where
+        # the grids were written.
+        stream = slot.stream
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(stream):
+            self._fill_and_launch(slot, pending, finishing)
+        slot.finishing = finishing
+        slot.event = torch.cuda.Event()
+        slot.event.record(stream)
+        slot.busy = True
+        self._inflight.append(slot)
+        self.steps += 1
+        return done
+
+    def _fill_and_launch(self, slot: _Slot, pending: List[RenderRequest],
+                         finishing: List[RenderRequest]) -> None:
+        cams_dirty = slot.cam_version != self.cam_version
+        for gi, grp in enumerate(self.groups):
+            mine = [r for r in pending if self.group_of[r.grid] is grp]
+            plan = plan_tiles(mine, self.max_tiles, self.tile_pixels, self.policy)
+            host = slot.host_np[gi]
+            host[:4].view(np.int32)[0] = len(plan)
+            if plan:
+                tiles = host[self.tiles_off:self.tiles_off + len(plan) * self.tile_size].view(
+                    _TILE_DTYPE)
+                tiles["grid"] = [grp.local[r.grid] for r, _, _ in plan]
+                tiles["camera"] = [r.slot for r, _, _ in plan]
+                tiles["pixel0"] = [p0 for _, p0, _ in plan]
+                tiles["n_pixels"] = [n for _, _, n in plan]
+                tiles["rgb_out"] = [r.image.data_ptr() for r, _, _ in plan]
+                self.rays_rendered += int(tiles["n_pixels"].sum())
+                for r, _, _ in plan:
+                    if r.next_pixel >= r.pixels and r not in finishing:
+                        finishing.append(r)
+            nbytes = self.tiles_off + len(plan) * self.tile_size
+            if cams_dirty:
+                host[self.cams_off:] = np.frombuffer(self.cams, dtype=np.uint8)
+                nbytes = self.table_bytes
+            slot.dev[gi][:nbytes].copy_(slot.host[gi][:nbytes], non_blocking=True)
+            if plan:
+                size = next(c for c in self.size_classes if c >= len(plan))
+                if self.use_graph:
+                    slot.graphs[gi][size].replay()
+                else:
+                    self._launch(grp, slot.dev[gi], size)
+        slot.cam_version = self.cam_version
+
+    def drain(self) -> List[RenderRequest]:
+        done: List[RenderRequest] = []
+        while self._inflight:
+            done += self._retire(self._inflight.pop(0))
+        return done
 
     def run(self, requests: Sequence[RenderRequest], realtime: bool = True,
             clock=time.perf_counter) -> dict:
         """Serves ``requests`` (sorted by arrival); with ``realtime`` each is
         submitted once the clock passes its arrival (seconds from the start),
-        else all are submitted at once.  Returns latency and throughput
-        figures."""
+        else all are submitted at once.  The loop never blocks on the device
+        while requests can still be admitted: it launches whenever a table
+        slot is free and polls step events otherwise.  Returns latency and
+        throughput figures."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = clock()
+        self._clock = lambda: clock() - t0
         i = 0
-        done: List[RenderRequest] = []
-        while len(done) < len(reqs):
+        n_done = 0
+        done_all: List[RenderRequest] = []
+        while n_done < len(reqs):
             now = clock() - t0
             while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
                 self.submit(reqs[i])
                 i += 1
-            if not self.active and not self.waiting:
-                if i < len(reqs):
-                    time.sleep(max(0.0, min(reqs[i].arrival - now, 1e-3)))
-                continue
-            fin = self.step(now)
-            t = clock() - t0
-            for r in fin:
-                r.done = t
-            done.extend(fin)
-        return summarize(done, clock() - t0)
+            fin = self.poll()
+            has_work = bool(self.waiting) or any(r.next_pixel < r.pixels for r in self.active)
+            if has_work and not self.slots[self._next_slot].busy:
+                fin += self.step(now)
+            elif not self._inflight and not has_work and i < len(reqs):
+                # Idle: sleep until the next arrival.
+                time.sleep(max(0.0, reqs[i].arrival - (clock() - t0)))
+            elif not fin:
+                time.sleep(0)  # yield while steps run
+            n_done += len(fin)
+            done_all += fin
+        return summarize(done_all, clock() - t0)
 
 
 def summarize(done: Sequence[RenderRequest], wall: float) -> dict:

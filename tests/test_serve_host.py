@@ -1,6 +1,8 @@
-"""Scheduling logic of the serving loop (no GPU): the round-robin tile plan
-covers every pixel of every request exactly once, in order, within the
-per-launch tile cap, and lets small requests finish before large ones."""
+"""Scheduling logic of the serving loop (no GPU): both tile plans cover every
+pixel of every request exactly once, in order, within the per-launch tile cap;
+FIFO serves requests in admission order, round-robin lets small requests
+finish before large ones."""
+import pytest
 import math
 
 from svox2_amd.serve import RenderRequest, plan_tiles
@@ -15,12 +17,13 @@ def _req(w, h):
     return RenderRequest(grid=0, camera=_Cam(w, h))
 
 
-def test_plan_covers_each_pixel_once():
+@pytest.mark.parametrize("policy", ["fifo", "round_robin"])
+def test_plan_covers_each_pixel_once(policy):
     reqs = [_req(100, 37), _req(5, 5), _req(64, 64), _req(1, 1)]
     seen = {id(r): [] for r in reqs}
     steps = 0
     while any(r.next_pixel < r.pixels for r in reqs):
-        plan = plan_tiles(reqs, max_tiles=3, tile_pixels=500)
+        plan = plan_tiles(reqs, max_tiles=3, tile_pixels=500, policy=policy)
         assert 0 < len(plan) <= 3
         for r, p0, n in plan:
             assert 0 < n <= 500
@@ -40,7 +43,7 @@ def test_round_robin_lets_small_requests_finish_first():
     done_at = {}
     step = 0
     while big.next_pixel < big.pixels or small.next_pixel < small.pixels:
-        plan_tiles([big, small], max_tiles=4, tile_pixels=4096)
+        plan_tiles([big, small], max_tiles=4, tile_pixels=4096, policy="round_robin")
         step += 1
         for name, r in (("big", big), ("small", small)):
             if r.next_pixel >= r.pixels and name not in done_at:
@@ -48,3 +51,21 @@ def test_round_robin_lets_small_requests_finish_first():
     assert done_at["small"] == math.ceil(128 * 128 / 4096 / 2)
     assert done_at["big"] > done_at["small"]
     assert done_at["big"] == math.ceil((1024 * 1024 + 128 * 128) / 4096 / 4)
+
+
+def test_fifo_serves_in_admission_order():
+    first, second = _req(300, 10), _req(10, 10)
+    plan = plan_tiles([first, second], max_tiles=3, tile_pixels=1000)
+    assert [(r is first, p0, n) for r, p0, n in plan] == [(True, 0, 1000), (True, 1000, 1000),
+                                                          (True, 2000, 1000)]
+    plan = plan_tiles([first, second], max_tiles=3, tile_pixels=1000)
+    assert [(r is first, n) for r, _, n in plan] == [(False, 100)]
+
+
+def test_tile_record_matches_c_layout():
+    import ctypes
+    from svox2_amd import _C, serve
+    t = _C._Tile
+    assert serve._TILE_DTYPE.itemsize == ctypes.sizeof(t)
+    for name in ("grid", "camera", "pixel0", "n_pixels", "rgb_out"):
+        assert serve._TILE_DTYPE.fields[name][1] == getattr(t, name).offset
