@@ -147,13 +147,17 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  * rgb_out, then a backward launch that forms the MSE gradient from it and
  * accumulates into grads.  log_transmit_out (n floats, may be NULL unless
  * beta_loss > 0) receives each ray's final log transmittance, which the beta
- * term needs.  With a device workspace of at least
- * svox2_ray_order_workspace_bytes(n_rays) bytes and a batch
- * of at least four times the rays the device holds resident (40960 on a
- * 256-CU MI355X), the forward records each ray's chunk count and the backward
- * processes the rays in descending order of it (same results; the rays of a
- * wave are then of similar length and the longest start first).  Otherwise,
- * or with workspace NULL, the backward takes the rays in input order.
+ * term needs.  The optional device workspace (results are the same with or
+ * without it):
+ * - at least svox2_ray_order_workspace_bytes(n_rays) bytes: for batches of at
+ *   least four times the rays the device holds resident (40960 on a 256-CU
+ *   MI355X) the forward records each ray's chunk count and the backward
+ *   processes the rays in descending order of it;
+ * - at least svox2_volume_render_cuvol_fused_workspace_bytes(n_rays) bytes:
+ *   in addition the forward writes every contributing sample (position,
+ *   corner links, colour, transmittance) to a record stream in the workspace
+ *   and the backward replays it instead of re-marching the ray (a ray whose
+ *   records do not fit is re-marched).
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
@@ -170,6 +174,11 @@ int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
  * svox2_volume_render_cuvol_backward use to order the rays of an n_rays batch
  * (0 for an empty batch).  No counterpart in the reference. */
 size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
+
+/* Device workspace bytes of svox2_volume_render_cuvol_fused with ray ordering
+ * and the record stream (0 for an empty batch): about 4.6 KiB per ray.  No
+ * counterpart in the reference. */
+size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
 /* One tile of a multi-request render launch: n_pixels consecutive pixels
  * (row-major, starting at pixel0) of the image that camera `camera` sees of

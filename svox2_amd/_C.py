@@ -42,7 +42,8 @@ INDEXER_ALL, INDEXER_MASK, INDEXER_INDEX = 0, 1, 2
 EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_image",
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
-    "svox2_ray_order_workspace_bytes", "svox2_cell_map_bytes",
+    "svox2_ray_order_workspace_bytes", "svox2_volume_render_cuvol_fused_workspace_bytes",
+    "svox2_cell_map_bytes",
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_accel_dist_prop_workspace_bytes",
@@ -156,6 +157,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         fn.restype = ctypes.c_int
     lib.svox2_cell_map_bytes.argtypes = [ctypes.POINTER(i32)]
     lib.svox2_cell_map_bytes.restype = ctypes.c_size_t
+    lib.svox2_volume_render_cuvol_fused_workspace_bytes.argtypes = [i64]
+    lib.svox2_volume_render_cuvol_fused_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_ray_order_workspace_bytes.argtypes = [i64]
     lib.svox2_ray_order_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
@@ -567,11 +570,13 @@ def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: Rend
 def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
                               rgb_gt: torch.Tensor, beta_loss: float, sparsity_loss: float,
                               rgb_out: torch.Tensor, grads: GridOutputGrads,
-                              ordered: bool = True) -> None:
+                              ordered: bool = True, record: bool = True) -> None:
     """volume_render_cuvol_fused (render_lerp_kernel_cuvol.cu:1070-1152).
 
     ordered: run the backward over the rays in descending order of the work
     the forward measured (same results, shorter tail); False keeps input order.
+    record (with ordered): the forward records every contributing sample and
+    the backward replays the records instead of re-marching (same results).
     """
     g = _grid_struct(grid)
     o, d, clip = _rays(rays)
@@ -583,8 +588,13 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
              if beta_loss > 0 else None)
     # Workspace for the backward's ray ordering (from torch's caching allocator).
-    nbytes = (load_library().svox2_ray_order_workspace_bytes(o.size(0))
-              if ordered else 0)
+    lib = load_library()
+    if ordered and record:
+        nbytes = lib.svox2_volume_render_cuvol_fused_workspace_bytes(o.size(0))
+    elif ordered:
+        nbytes = lib.svox2_ray_order_workspace_bytes(o.size(0))
+    else:
+        nbytes = 0
     ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
         _call("svox2_volume_render_cuvol_fused", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),

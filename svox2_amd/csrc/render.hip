@@ -503,50 +503,6 @@ __device__ __forceinline__ void load_row(const GridView<uint16_t>& g, int lk, fl
     }
 }
 
-// Colours of up to four live samples of a chunk at once: lane = (sample s, corner
-// k) with s = lane / 8.  Each lane reads its corner's SH row, projects it on
-// the SH basis and weights it trilinearly; the eight corners are summed within
-// the lane's group of 8.  Returns the raw colours (before +0.5 and clamping) of
-// sample pass*4 + lane/8 (identical in its 8 lanes), and that sample's source
-// lane in `src`.
-template <typename DT, int B>
-__device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx<B>& rc,
-                                             const Chunk& c, int lane, int pass, int n_live,
-                                             float* col, int& src, bool& has) {
-    const int q = pass * 4 + (lane >> 3);
-    const int k = lane & 7;
-    has = q < n_live;
-    src = has ? nth_set_bit(c.live_mask, q) : 0;
-    int lk = __shfl(c.lk[0], src, kGroup);
-#pragma unroll
-    for (int kk = 1; kk < 8; ++kk) {
-        const int v = __shfl(c.lk[kk], src, kGroup);
-        lk = (k == kk) ? v : lk;
-    }
-    float p[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) p[a] = __shfl(c.p[a], src, kGroup);
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-    if (has && g.valid(lk)) {
-        constexpr int C = 3 * B;
-        float x[C];
-        load_row<C>(g, lk, x);
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-            v0 = fmaf(rc.basis[j], x[j], v0);
-            v1 = fmaf(rc.basis[j], x[B + j], v1);
-            v2 = fmaf(rc.basis[j], x[2 * B + j], v2);
-        }
-        const float w = corner_weight(k, p);
-        v0 *= w;
-        v1 *= w;
-        v2 *= w;
-    }
-    col[0] = sum8(v0);
-    col[1] = sum8(v1);
-    col[2] = sum8(v2);
-}
-
 // Ray ordering for the backward launch.  A ray's backward costs close to a
 // fixed amount per chunk (the dependent link, density and row round trips),
 // so the forward records its chunk count and, for batches several times
@@ -555,12 +511,12 @@ __device__ __forceinline__ void chunk_colors(const GridView<DT>& g, const RayCtx
 // (no half-wave idles behind a long ray) and the longest rays no longer finish
 // last on an otherwise idle device.  Smaller batches run in one round, where
 // pairing the longest rays in the same waves only slows them, and keep the
-// input order.
+// input order.  When the backward replays recorded samples instead, its
+// work is per record and per recorded chunk, and that count is the key.
 constexpr int kCostBins = 1024;
 
-__device__ __forceinline__ uint32_t backward_cost(uint32_t n_chunks) {
-    return n_chunks < static_cast<uint32_t>(kCostBins) ? n_chunks
-                                                       : static_cast<uint32_t>(kCostBins - 1);
+__device__ __forceinline__ uint32_t backward_cost(uint32_t work) {
+    return work < static_cast<uint32_t>(kCostBins) ? work : static_cast<uint32_t>(kCostBins - 1);
 }
 
 // Colours of a chunk's live samples, grouped by cell.  Consecutive live
@@ -631,20 +587,82 @@ __device__ __forceinline__ uint32_t run_heads(const Chunk& c, int lane) {
     return group_ballot(c.live && !(prev_live && prev_cell == c.base));
 }
 
-#ifndef SVOX2_COLOUR_RUNS
-#define SVOX2_COLOUR_RUNS 1
-#endif
+// Colours of a chunk's live samples: each live lane receives its own sample's
+// colour + 0.5 in cs (before clamping), and, if acc is given, lanes with
+// lane % 8 == 0 add weight * max(colour + 0.5, 0) of their group's samples to
+// acc (the forward's composite, summed over lanes at the end of the ray).
+template <typename DT, int B, bool TO_LANES>
+__device__ __forceinline__ void chunk_colours(const GridView<DT>& g, const RayCtx<B>& rc,
+                                              const Chunk& c, int lane, float* cs, float* acc) {
+    const uint32_t heads = run_heads(c, lane);
+    const uint32_t upto = (lane == 31) ? 0xFFFFFFFFu : ((2u << lane) - 1u);
+    const uint32_t my_heads = heads & upto;
+    // This lane's run (head index) and its place in the run.
+    const int my_run = __popc(my_heads) - 1;
+    const int head_pos = my_heads ? 31 - __clz(my_heads) : 0;
+    const int my_idx = __popc(c.live_mask & ((1u << lane) - 1u) & (0xFFFFFFFFu << head_pos));
+    chunk_run_colors<DT, B>(
+        g, rc, c, lane, heads, [&](int pass, int i, bool act, int s, const float* col) {
+            if (acc != nullptr) {
+                const float w = __shfl(c.weight, s, kGroup);
+                if (act && (lane & 7) == 0) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
+                }
+            }
+            if (TO_LANES) {
+                const int grp = my_run - 4 * pass;
+                const bool mine = c.live && grp >= 0 && grp < 4 && my_idx == i;
+                const int from = mine ? 8 * grp : 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float v = __shfl(col[a], from, kGroup);
+                    cs[a] = mine ? v + 0.5f : cs[a];
+                }
+            }
+        });
+}
+
+// Record stream of the fused call: the forward appends, per ray, a header and
+// one 96-byte record per contributing sample of each chunk; the backward
+// replays them instead of re-marching.  Slot s is rec[6 s .. 6 s + 5]
+// (float4).  Ray r owns slots [r * fixed, (r + 1) * fixed) and continues in
+// overflow blocks of kRecBlock slots taken with one atomic add.  Headers:
+// (n > 0, -) n records follow; (0, next) continue at slot next; (-1, -) end.
+// state[r] = 1 when the ray's stream is complete, 2 when the overflow region
+// ran out (the backward then re-marches the ray).
+struct RecStream {
+    float4* __restrict__ rec;
+    int32_t* __restrict__ state;
+    uint32_t* __restrict__ next_block;  // overflow slots taken so far
+    int64_t fixed;                      // slots per ray in the fixed region
+    int64_t overflow_base;              // first overflow slot
+    int64_t overflow_slots;             // slots in the overflow region
+};
+constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
+
+__device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int64_t next) {
+    rec[6 * slot] = make_float4(__int_as_float(n), __int_as_float(static_cast<int>(next)), 0.f,
+                                0.f);
+}
 
 // Forward march of one ray.  out[3] = composited colour (all lanes),
 // returns the final log transmittance.
-template <typename DT, int B, bool STATS>
+template <typename DT, int B, bool STATS, bool RECORD = false>
 __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                float* out, int32_t* stats,
-                                               uint32_t* cost = nullptr) {
+                                               uint32_t* cost = nullptr,
+                                               const RecStream* rs = nullptr,
+                                               int64_t ray = 0) {
+    // Record stream position of this ray: next free slot and end of block.
+    int64_t cur = RECORD ? ray * rs->fixed : 0;
+    int64_t end = RECORD ? cur + rs->fixed : 0;
+    bool rec_ok = RECORD;
     float acc[3] = {0.f, 0.f, 0.f};  // this lane's share of the composited colour
     uint32_t n_iter = 0, n_eval = 0, n_hit = 0;
     uint32_t n_chunks = 0;
+    uint32_t n_live_all = 0, n_live_chunks = 0;  // replay work (recording only)
     Fnv fnv;
     float log_t = 0.f;
     if (r.tmin <= r.tmax) {
@@ -656,28 +674,44 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
             march_chunk(g, o, r, lane, t, log_t, c);
             const int n_live = __popc(c.live_mask);
             ++n_chunks;
-            if (SVOX2_COLOUR_RUNS) {
-                if (n_live > 0) {
-                    chunk_run_colors<DT, B>(
-                        g, rc, c, lane, run_heads(c, lane),
-                        [&](int, int, bool act, int s, const float* col) {
-                            const float w = __shfl(c.weight, s, kGroup);
-                            if (act && (lane & 7) == 0) {
-#pragma unroll
-                                for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
-                            }
-                        });
-                }
-            } else {
-                for (int pass = 0; pass * 4 < n_live; ++pass) {
-                    float col[3];
-                    int src;
-                    bool has;
-                    chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
-                    const float w = __shfl(c.weight, src, kGroup);
-                    if (has && (lane & 7) == 0) {
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) acc[a] += w * fmaxf(col[a] + 0.5f, 0.f);
+            if (RECORD && n_live > 0) {
+                n_live_all += n_live;
+                ++n_live_chunks;
+            }
+            if (n_live > 0) {
+                float cs[3] = {0.f, 0.f, 0.f};
+                chunk_colours<DT, B, RECORD>(g, rc, c, lane, cs, acc);
+                if (RECORD && rec_ok) {
+                    // Header + n_live records; keep one slot for a jump/end header.
+                    if (cur + n_live + 2 > end) {
+                        uint32_t nb = 0;
+                        if (lane == 0) nb = atomicAdd(rs->next_block, static_cast<uint32_t>(kRecBlock));
+                        nb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), 0, kGroup));
+                        if (static_cast<int64_t>(nb) + kRecBlock > rs->overflow_slots) {
+                            rec_ok = false;
+                        } else {
+                            const int64_t nxt = rs->overflow_base + nb;
+                            if (lane == 0) put_header(rs->rec, cur, 0, nxt);
+                            cur = nxt;
+                            end = nxt + kRecBlock;
+                        }
+                    }
+                    if (rec_ok) {
+                        if (lane == 0) put_header(rs->rec, cur, n_live, 0);
+                        if (c.live) {
+                            const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
+                            float4* q = rs->rec + 6 * (cur + 1 + rank);
+                            q[0] = make_float4(c.p[0], c.p[1], c.p[2], __int_as_float(c.base));
+                            q[1] = make_float4(__int_as_float(c.l[0]), __int_as_float(c.l[1]),
+                                               __int_as_float(c.l[2]), c.weight);
+                            q[2] = make_float4(__int_as_float(c.lk[0]), __int_as_float(c.lk[1]),
+                                               __int_as_float(c.lk[2]), __int_as_float(c.lk[3]));
+                            q[3] = make_float4(__int_as_float(c.lk[4]), __int_as_float(c.lk[5]),
+                                               __int_as_float(c.lk[6]), __int_as_float(c.lk[7]));
+                            q[4] = make_float4(cs[0], cs[1], cs[2], c.lt_after);
+                            q[5] = make_float4(c.ws, c.sigma, 0.f, 0.f);
+                        }
+                        cur += n_live + 1;
                     }
                 }
             }
@@ -704,7 +738,14 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     const float tb = __expf(log_t) * o.bg;
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = __shfl(acc[a], 0, kGroup) + tb;
-    if (cost != nullptr) *cost = backward_cost(n_chunks);
+    // A replayed backward works per record and per recorded chunk, a
+    // re-marching one per chunk.
+    if (cost != nullptr)
+        *cost = backward_cost(RECORD ? n_live_all + 4u * n_live_chunks : n_chunks);
+    if (RECORD && lane == 0) {
+        if (rec_ok) put_header(rs->rec, cur, -1, 0);
+        rs->state[ray] = rec_ok ? 1 : 2;
+    }
     if (STATS && lane == 0) {
         fnv.mix(__float_as_uint(log_t));
         stats[0] = static_cast<int32_t>(n_iter);
@@ -837,34 +878,132 @@ struct Scatter {
     }
 };
 
-// Backward march of one ray given dL/drgb (gq) and accum = rgb . gq (+ beta
-// term); accumulates into the gradient buffers (trace_ray_cuvol_backward,
-// render_lerp_kernel_cuvol.cu:227-383).  Samples are rematerialised chunk by
-// chunk exactly as in the forward.  Each live sample's gradient scalars (the
-// colour factors weight * [colour >= 0] * dL/dcolour and the density gradient
-// ws * (total * T_after - accum)) are formed sample-parallel, with the running
-// `accum` of the reference as a prefix sum over the chunk's samples, then
-// added in sample order into the (corner, quantity) accumulators.
+// Backward state of one ray: dL/drgb (gq), the running accum = rgb . gq (+
+// beta term) of the reference, and the corner accumulators.
+template <int B>
+struct BwdRay {
+    static constexpr int C = 3 * B;
+    float gq[3];
+    float accum;
+    float sparsity;
+    float basis_j;  // basis value of this lane's SH channel (lane = c * B + j)
+    int my_k, my_q;
+    SampleRec* rec;  // this group's LDS records
+    Scatter<B> sc;
+
+    __device__ __forceinline__ BwdRay(const RayCtx<B>& rc, int lane, const float* g3, float acc0,
+                                      float sp, BwdShared* sh) {
+        const int gbase = threadIdx.x & ~(kGroup - 1);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gq[a] = g3[a];
+        accum = acc0;
+        sparsity = sp;
+        // Selects, not a register array indexed by lane (that goes to scratch).
+        basis_j = 0.f;
+#pragma unroll
+        for (int j = 0; j < B; ++j) basis_j = (lane < C && lane % B == j) ? rc.basis[j] : basis_j;
+        my_k = lane & 7;
+        my_q = lane >> 3;
+        rec = sh->rec + gbase;
+        sc.s_acc = sh->acc + gbase;
+        sc.s_link = sh->link + gbase;
+    }
+};
+
+// One chunk's contribution to the gradients (trace_ray_cuvol_backward,
+// render_lerp_kernel_cuvol.cu:227-383).  Lane i holds sample i of the chunk if
+// live: its position (p, l, base), corner links, colour + 0.5 (cs), weight,
+// log transmittance after it, world step and density.  The gradient scalars
+// are formed sample-parallel - the colour factors weight * [colour >= 0] * g
+// and the density gradient ws * (total * T_after - accum), with the running
+// `accum` as a prefix sum over the chunk - staged as LDS records, and added
+// in sample order into the (corner, quantity) accumulators.
+template <typename DT, int B>
+__device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradOut& go,
+                                              BwdRay<B>& st, int lane, bool live,
+                                              uint32_t live_mask, const float* p, const int* l,
+                                              int base, const int* lk, const float* cs,
+                                              float weight, float lt_after, float ws,
+                                              float sigma) {
+    float total = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * st.gq[a];
+    float run = live ? weight * total : 0.f;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const float y = __shfl_up(run, off, kGroup);
+        if (lane >= off) run += y;
+    }
+    const float acc_s = st.accum - run;
+    st.accum -= __shfl(run, 31, kGroup);
+    float gsig = ws * (total * __expf(lt_after) - acc_s);
+    if (st.sparsity > 0.f) gsig += st.sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
+    if (live) {
+        SampleRec& R = st.rec[lane];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            R.p[a] = p[a];
+            R.l[a] = l[a];
+            R.val[a] = weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * st.gq[a];
+        }
+        R.base = base;
+        R.val[3] = gsig;
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) R.lk[kk] = lk[kk];
+    }
+    wave_lds_fence();
+    Scatter<B>& sc = st.sc;
+    // Each iteration reads the next sample's position and value ahead, so
+    // their LDS latency overlaps the current sample's work.
+    uint32_t m = live_mask;
+    int s_next = m ? __ffs(m) - 1 : 0;
+    float4 pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
+    float v_next = st.rec[s_next].val[st.my_q];
+    while (m) {
+        const int s_cur = s_next;
+        const float4 pb = pb_next;
+        const float v = v_next;
+        m &= m - 1;
+        if (m) {
+            s_next = __ffs(m) - 1;
+            pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
+            v_next = st.rec[s_next].val[st.my_q];
+        }
+        const SampleRec& R = st.rec[s_cur];
+        const int cell = __float_as_int(pb.w);
+        if (cell != sc.cur) {
+            const int l0 = R.l[0], l1 = R.l[1], l2 = R.l[2];
+            const int d0 = l0 - sc.c0, d1 = l1 - sc.c1, d2 = l2 - sc.c2;
+            const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 && d1 <= 1 &&
+                                  d2 >= -1 && d2 <= 1;
+            if (adjacent) {
+                sc.template shift<4>(d0, g, go, lane, st.basis_j);
+                sc.template shift<2>(d1, g, go, lane, st.basis_j);
+                sc.template shift<1>(d2, g, go, lane, st.basis_j);
+            } else {
+                sc.flush(g, go, lane, st.basis_j, 0xFFu);
+                sc.acc = 0.f;
+            }
+            sc.link = R.lk[st.my_k];
+            sc.cur = cell;
+            sc.c0 = l0;
+            sc.c1 = l1;
+            sc.c2 = l2;
+        }
+        const float pp[3] = {pb.x, pb.y, pb.z};
+        sc.acc = fmaf(corner_weight(st.my_k, pp), v, sc.acc);
+        sc.used = 0xFFu;
+    }
+    wave_lds_fence();  // records are rewritten by the next chunk
+}
+
+// Backward of one ray by re-marching it: every chunk is rematerialised
+// exactly as in the forward, its colours gathered again.
 template <typename DT, int B>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
-                                               const float* gq, float accum, float sparsity,
-                                               const GradOut& go, BwdShared* sh) {
+                                               BwdRay<B>& st, const GradOut& go) {
     if (r.tmin > r.tmax) return;
-    const int gbase = threadIdx.x & ~(kGroup - 1);
-    SampleRec* rec = sh->rec + gbase;
-    constexpr int C = 3 * B;
-    // basis_j of this lane's SH channel (lane = c * B + j), by selects: a
-    // register array indexed by lane would go to scratch memory.
-    float basis_j = 0.f;
-#pragma unroll
-    for (int j = 0; j < B; ++j) basis_j = (lane < C && lane % B == j) ? rc.basis[j] : basis_j;
-    const int my_k = lane & 7;
-    const int my_q = lane >> 3;
-    Scatter<B> sc;
-    sc.s_acc = sh->acc + gbase;
-    sc.s_link = sh->link + gbase;
-
     float t = r.tmin;
     float log_t = 0.f;
     int guard = 0;
@@ -872,155 +1011,62 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     while (more && ++guard < kMaxIters) {
         Chunk c;
         march_chunk(g, o, r, lane, t, log_t, c);
-        const int n_live = __popc(c.live_mask);
-        if (n_live > 0) {
-            // Colours of the live samples, moved to each sample's own lane.
+        if (c.live_mask != 0u) {
             float cs[3] = {0.f, 0.f, 0.f};
-            if (SVOX2_COLOUR_RUNS) {
-                const uint32_t heads = run_heads(c, lane);
-                const uint32_t upto = (lane == 31) ? 0xFFFFFFFFu : ((2u << lane) - 1u);
-                const uint32_t my_heads = heads & upto;
-                // This lane's run (head index) and its place in the run.
-                const int my_run = __popc(my_heads) - 1;
-                const int head_pos = my_heads ? 31 - __clz(my_heads) : 0;
-                const int my_idx = __popc(c.live_mask & ((1u << lane) - 1u) &
-                                          (0xFFFFFFFFu << head_pos));
-                chunk_run_colors<DT, B>(
-                    g, rc, c, lane, heads, [&](int pass, int i, bool, int, const float* col) {
-                        const int grp = my_run - 4 * pass;
-                        const bool mine = c.live && grp >= 0 && grp < 4 && my_idx == i;
-                        const int from = mine ? 8 * grp : 0;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const float v = __shfl(col[a], from, kGroup);
-                            cs[a] = mine ? v + 0.5f : cs[a];
-                        }
-                    });
-            } else {
-                const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
-                for (int pass = 0; pass * 4 < n_live; ++pass) {
-                    float col[3];
-                    int src;
-                    bool has;
-                    chunk_colors<DT, B>(g, rc, c, lane, pass, n_live, col, src, has);
-                    const int from = ((rank - pass * 4) & 3) * 8;
-                    const bool mine = c.live && rank >= pass * 4 && rank < pass * 4 + 4;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float v = __shfl(col[a], from, kGroup);
-                        cs[a] = mine ? v + 0.5f : cs[a];
-                    }
-                }
-            }
-            // Per-sample gradient scalars.
-            float total = 0.f;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * gq[a];
-            float run = c.live ? c.weight * total : 0.f;
-#pragma unroll
-            for (int off = 1; off < 32; off <<= 1) {
-                const float y = __shfl_up(run, off, kGroup);
-                if (lane >= off) run += y;
-            }
-            const float acc_s = accum - run;
-            accum -= __shfl(run, 31, kGroup);
-            float gsig = c.ws * (total * __expf(c.lt_after) - acc_s);
-            if (sparsity > 0.f) gsig += sparsity * (4.f * c.sigma / (1.f + 2.f * (c.sigma * c.sigma)));
-            float gc[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                gc[a] = c.weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * gq[a];
-            // Accumulate in sample order.
-            if (SVOX2_SCATTER_LDS) {
-                if (c.live) {
-                    SampleRec& R = rec[lane];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        R.p[a] = c.p[a];
-                        R.l[a] = c.l[a];
-                        R.val[a] = gc[a];
-                    }
-                    R.base = c.base;
-                    R.val[3] = gsig;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) R.lk[kk] = c.lk[kk];
-                }
-                wave_lds_fence();
-                for (uint32_t m = c.live_mask; m; m &= m - 1) {
-                    const SampleRec& R = rec[__ffs(m) - 1];
-                    const int cell = R.base;
-                    if (cell != sc.cur) {
-                        const int l0 = R.l[0], l1 = R.l[1], l2 = R.l[2];
-                        const int d0 = l0 - sc.c0, d1 = l1 - sc.c1, d2 = l2 - sc.c2;
-                        const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 &&
-                                              d1 <= 1 && d2 >= -1 && d2 <= 1;
-                        if (adjacent) {
-                            sc.template shift<4>(d0, g, go, lane, basis_j);
-                            sc.template shift<2>(d1, g, go, lane, basis_j);
-                            sc.template shift<1>(d2, g, go, lane, basis_j);
-                        } else {
-                            sc.flush(g, go, lane, basis_j, 0xFFu);
-                            sc.acc = 0.f;
-                        }
-                        sc.link = R.lk[my_k];
-                        sc.cur = cell;
-                        sc.c0 = l0;
-                        sc.c1 = l1;
-                        sc.c2 = l2;
-                    }
-                    const float p[3] = {R.p[0], R.p[1], R.p[2]};
-                    sc.acc = fmaf(corner_weight(my_k, p), R.val[my_q], sc.acc);
-                    sc.used = 0xFFu;
-                }
-                wave_lds_fence();  // records are rewritten by the next chunk
-            } else
-            for (uint32_t m = c.live_mask; m; m &= m - 1) {
-                const int s = __ffs(m) - 1;
-                const int cell = __shfl(c.base, s, kGroup);
-                if (cell != sc.cur) {
-                    int l[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) l[a] = __shfl(c.l[a], s, kGroup);
-                    const int d0 = l[0] - sc.c0, d1 = l[1] - sc.c1, d2 = l[2] - sc.c2;
-                    const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 &&
-                                          d1 <= 1 && d2 >= -1 && d2 <= 1;
-                    if (adjacent) {
-                        sc.template shift<4>(d0, g, go, lane, basis_j);
-                        sc.template shift<2>(d1, g, go, lane, basis_j);
-                        sc.template shift<1>(d2, g, go, lane, basis_j);
-                    } else {
-                        sc.flush(g, go, lane, basis_j, 0xFFu);
-                        sc.acc = 0.f;
-                    }
-                    int nl = __shfl(c.lk[0], s, kGroup);
-#pragma unroll
-                    for (int kk = 1; kk < 8; ++kk) {
-                        const int v = __shfl(c.lk[kk], s, kGroup);
-                        nl = my_k == kk ? v : nl;
-                    }
-                    sc.link = nl;
-                    sc.cur = cell;
-                    sc.c0 = l[0];
-                    sc.c1 = l[1];
-                    sc.c2 = l[2];
-                }
-                float p[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) p[a] = __shfl(c.p[a], s, kGroup);
-                const float v0 = __shfl(gc[0], s, kGroup);
-                const float v1 = __shfl(gc[1], s, kGroup);
-                const float v2 = __shfl(gc[2], s, kGroup);
-                const float v3 = __shfl(gsig, s, kGroup);
-                const float val = my_q == 0 ? v0 : (my_q == 1 ? v1 : (my_q == 2 ? v2 : v3));
-                sc.acc = fmaf(corner_weight(my_k, p), val, sc.acc);
-                sc.used = 0xFFu;
-            }
+            chunk_colours<DT, B, true>(g, rc, c, lane, cs, nullptr);
+            scatter_chunk<DT, B>(g, go, st, lane, c.live, c.live_mask, c.p, c.l, c.base, c.lk,
+                                 cs, c.weight, c.lt_after, c.ws, c.sigma);
         }
         log_t = c.log_t_end;
         t = c.next_t;
         more = c.more;
     }
-    sc.flush(g, go, lane, basis_j, 0xFFu);
+    st.sc.flush(g, go, lane, st.basis_j, 0xFFu);
+}
+
+// Backward of one ray from its record stream (the fused call's forward wrote
+// every contributing sample with its colour and transmittance): no link,
+// density or SH reads, only the records and the gradient scatter.
+template <typename DT, int B>
+__device__ __forceinline__ void replay_backward(const GridView<DT>& g, const RecStream& rs,
+                                                int64_t ray, int lane, BwdRay<B>& st,
+                                                const GradOut& go) {
+    int64_t pos = ray * rs.fixed;
+    for (int guard = 0; guard < kMaxIters; ++guard) {
+        const float4 h = rs.rec[6 * pos];
+        const int n = __float_as_int(h.x);
+        if (n < 0) break;
+        if (n == 0) {
+            pos = __float_as_int(h.y);
+            continue;
+        }
+        const bool live = lane < n;
+        float p[3] = {0.f, 0.f, 0.f}, cs[3] = {0.f, 0.f, 0.f};
+        int l[3] = {0, 0, 0}, lk[8];
+        int base = 0;
+        float weight = 0.f, lt_after = 0.f, ws = 0.f, sigma = 0.f;
+        if (live) {
+            const float4* q = rs.rec + 6 * (pos + 1 + lane);
+            const float4 a = q[0], b = q[1], c2 = q[2], d = q[3], e = q[4], f = q[5];
+            p[0] = a.x; p[1] = a.y; p[2] = a.z; base = __float_as_int(a.w);
+            l[0] = __float_as_int(b.x); l[1] = __float_as_int(b.y); l[2] = __float_as_int(b.z);
+            weight = b.w;
+            lk[0] = __float_as_int(c2.x); lk[1] = __float_as_int(c2.y);
+            lk[2] = __float_as_int(c2.z); lk[3] = __float_as_int(c2.w);
+            lk[4] = __float_as_int(d.x); lk[5] = __float_as_int(d.y);
+            lk[6] = __float_as_int(d.z); lk[7] = __float_as_int(d.w);
+            cs[0] = e.x; cs[1] = e.y; cs[2] = e.z; lt_after = e.w;
+            ws = f.x; sigma = f.y;
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) lk[kk] = -1;
+        }
+        const uint32_t live_mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+        scatter_chunk<DT, B>(g, go, st, lane, live, live_mask, p, l, base, lk, cs, weight,
+                             lt_after, ws, sigma);
+        pos += n + 1;
+    }
+    st.sc.flush(g, go, lane, st.basis_j, 0xFFu);
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,13 +1086,26 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     __attribute__((amdgpu_waves_per_eu(SVOX2_FWD_WAVES)))
 #define SVOX2_BWD_KERNEL __global__ __launch_bounds__(kBlock) \
     __attribute__((amdgpu_waves_per_eu(SVOX2_BWD_WAVES)))
+#ifndef SVOX2_REC_WAVES
+#define SVOX2_REC_WAVES 4
+#endif
+#ifndef SVOX2_REPLAY_WAVES
+#define SVOX2_REPLAY_WAVES 6
+#endif
+#define SVOX2_REPLAY_KERNEL __global__ __launch_bounds__(kBlock) \
+    __attribute__((amdgpu_waves_per_eu(SVOX2_REPLAY_WAVES)))
+#define SVOX2_REC_KERNEL __global__ __launch_bounds__(kBlock) \
+    __attribute__((amdgpu_waves_per_eu(SVOX2_REC_WAVES)))
 
-template <typename DT, int B, bool STATS>
-SVOX2_FWD_KERNEL void k_render_rays(
+// The recording forward of the fused call: a kernel of its own so that it is
+// compiled for the backward's register budget (the record writes and the
+// colour-to-lane moves need more registers than the plain forward has).
+template <typename DT, int B>
+SVOX2_REC_KERNEL void k_render_rays_rec(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         float* __restrict__ rgb_out, float* __restrict__ log_t_out,
-        int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out) {
+        uint32_t* __restrict__ cost_out, RecStream rs) {
     const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (ray >= n) return;
@@ -1057,9 +1116,33 @@ SVOX2_FWD_KERNEL void k_render_rays(
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float out[3];
     uint32_t cost = 0;
-    const float log_t = march_forward<DT, B, STATS>(g, o, r, rc, lane, out,
-                                                    STATS ? stats + 4 * ray : nullptr,
-                                                    cost_out != nullptr ? &cost : nullptr);
+    const float log_t = march_forward<DT, B, false, true>(
+        g, o, r, rc, lane, out, nullptr, cost_out != nullptr ? &cost : nullptr, &rs, ray);
+    if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
+    if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
+    if (cost_out != nullptr && lane == 0) cost_out[ray] = cost;
+}
+
+template <typename DT, int B, bool STATS, bool RECORD = false>
+SVOX2_FWD_KERNEL void k_render_rays(
+        GridView<DT> g, OptView o, const float* __restrict__ origins,
+        const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
+        float* __restrict__ rgb_out, float* __restrict__ log_t_out,
+        int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out, RecStream rs) {
+    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    const RayCtx<B> rc(wd);
+    Ray r;
+    setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
+    float out[3];
+    uint32_t cost = 0;
+    const float log_t = march_forward<DT, B, STATS, RECORD>(g, o, r, rc, lane, out,
+                                                            STATS ? stats + 4 * ray : nullptr,
+                                                            cost_out != nullptr ? &cost : nullptr,
+                                                            &rs, ray);
     if (!STATS) {
         if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
         if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
@@ -1174,7 +1257,40 @@ SVOX2_BWD_KERNEL void k_render_backward(
     const float* cc = color_cache + 3 * ray;
     const float accum = fmaf(cc[0], gq[0], fmaf(cc[1], gq[1], cc[2] * gq[2]));
     __shared__ BwdShared sh;
-    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, 0.f, go, &sh);
+    BwdRay<B> st(rc, lane, gq, accum, 0.f, &sh);
+    march_backward<DT, B>(g, o, r, rc, lane, st, go);
+}
+
+// Backward of the fused MSE step for the rays whose samples the forward
+// recorded (state 1): the same gradient as k_render_backward_mse, from the
+// records, in a kernel without the march code (fewer registers, more waves).
+template <typename DT, int B>
+SVOX2_REPLAY_KERNEL void k_render_backward_replay(
+        GridView<DT> g, const float* __restrict__ dirs, int64_t n,
+        const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
+        const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
+        const uint32_t* __restrict__ order, GradOut go, RecStream rs) {
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (slot >= n) return;
+    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
+    if (rs.state[ray] != 1) return;  // re-marched by k_render_backward_mse
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    const RayCtx<B> rc(wd);
+    float c3[3], gq[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        c3[c] = rgb[3 * ray + c];
+        gq[c] = (c3[c] - rgb_gt[3 * ray + c]) * norm;
+    }
+    float accum = fmaf(c3[0], gq[0], fmaf(c3[1], gq[1], c3[2] * gq[2]));
+    if (beta_per_ray > 0.f) {
+        const float tin = __expf(log_t_fwd[ray]);
+        accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
+    }
+    __shared__ BwdShared sh;
+    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh);
+    replay_backward<DT, B>(g, rs, ray, lane, st, go);
 }
 
 // Backward of the fused MSE step: dL/drgb = (rgb - gt) * 2 / (3n) from the
@@ -1187,11 +1303,12 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
         const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
-        const uint32_t* __restrict__ order, GradOut go) {
+        const uint32_t* __restrict__ order, GradOut go, RecStream rs) {
     const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (slot >= n) return;
     const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
+    if (rs.rec != nullptr && rs.state[ray] == 1) return;  // replayed by k_render_backward_replay
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
@@ -1209,7 +1326,8 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
     __shared__ BwdShared sh;
-    march_backward<DT, B>(g, o, r, rc, lane, gq, accum, sparsity, go, &sh);
+    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh);
+    march_backward<DT, B>(g, o, r, rc, lane, st, go);
 }
 
 // Counting sort of the rays by descending cost: histogram, scan, scatter.
@@ -1292,6 +1410,37 @@ OrderWorkspace order_workspace(void* ws, int64_t n) {
     w.cost = p + 2 * kCostBins;
     w.order = w.cost + n;
     return w;
+}
+
+// Fused-call workspace: the ordering workspace, then the record stream:
+// per-ray state (n words), the overflow counter, and the record slots (n *
+// (kRecFixed + kRecOverflow) slots of 96 bytes).
+constexpr int64_t kRecFixed = 8;      // slots per ray in the fixed region
+constexpr int64_t kRecOverflow = 40;  // overflow slots per ray
+constexpr int64_t kRecSlotBytes = 96;
+
+inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+size_t fused_workspace_bytes(int64_t n) {
+    const int64_t order = static_cast<int64_t>(order_workspace_bytes(n));
+    const int64_t state = align256(4 * n);
+    const int64_t counter = 256;
+    const int64_t recs = n * (kRecFixed + kRecOverflow) * kRecSlotBytes;
+    return static_cast<size_t>(order + state + counter + recs);
+}
+
+RecStream record_stream(void* ws, int64_t n) {
+    char* p = static_cast<char*>(ws) + order_workspace_bytes(n);
+    RecStream rs{};
+    rs.state = reinterpret_cast<int32_t*>(p);
+    p += align256(4 * n);
+    rs.next_block = reinterpret_cast<uint32_t*>(p);
+    p += 256;
+    rs.rec = reinterpret_cast<float4*>(p);
+    rs.fixed = kRecFixed;
+    rs.overflow_base = n * kRecFixed;
+    rs.overflow_slots = n * kRecOverflow;
+    return rs;
 }
 
 int check_order_workspace(const void* workspace, size_t bytes, int64_t n) {
@@ -1409,7 +1558,7 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
-                           log_transmit_out, (int32_t*)nullptr, ray_cost_out);
+                           log_transmit_out, (int32_t*)nullptr, ray_cost_out, RecStream{});
     });
     if (st) return st;
     return check_launch("volume_render_cuvol");
@@ -1432,7 +1581,8 @@ extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const f
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, true>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays,
-                           (float*)nullptr, (float*)nullptr, stats_out, (uint32_t*)nullptr);
+                           (float*)nullptr, (float*)nullptr, stats_out, (uint32_t*)nullptr,
+                           RecStream{});
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_stats");
@@ -1548,6 +1698,10 @@ extern "C" size_t svox2_ray_order_workspace_bytes(int64_t n_rays) {
     return n_rays > 0 ? order_workspace_bytes(n_rays) : 0;
 }
 
+extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
+    return n_rays > 0 ? fused_workspace_bytes(n_rays) : 0;
+}
+
 extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                                const float* origins, const float* dirs,
                                                const float* ray_clip, int64_t n_rays,
@@ -1577,16 +1731,32 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
     if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
+    RecStream rs{};
+    // Records need slot indices below 2^31 (headers hold them as int32).
+    if (workspace != nullptr && workspace_bytes >= fused_workspace_bytes(n_rays) &&
+        n_rays * (kRecFixed + kRecOverflow) < INT32_MAX) {
+        rs = record_stream(workspace, n_rays);
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
+                           reinterpret_cast<uint32_t*>(rs.next_block), 1);
+    }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
-        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s, g, o,
-                           origins, dirs, ray_clip, n_rays, rgb_out, lt, (int32_t*)nullptr,
-                           w.cost);
+        if (rs.rec != nullptr)
+            hipLaunchKernelGGL((k_render_rays_rec<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
+                               o, origins, dirs, ray_clip, n_rays, rgb_out, lt, w.cost, rs);
+        else
+            hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
+                               g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
+                               (int32_t*)nullptr, w.cost, rs);
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s);
+        if (rs.rec != nullptr)
+            hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
+                               s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta, sparsity_loss,
+                               norm, static_cast<const uint32_t*>(w.order), go, rs);
         hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
                            origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, lt, beta,
-                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go);
+                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");
