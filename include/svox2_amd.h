@@ -154,10 +154,10 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  *   MI355X) the forward records each ray's chunk count and the backward
  *   processes the rays in descending order of it;
  * - at least svox2_volume_render_cuvol_fused_workspace_bytes(n_rays) bytes:
- *   in addition the forward writes every contributing sample (position,
- *   corner links, colour, transmittance) to a record stream in the workspace
- *   and the backward replays it instead of re-marching the ray (a ray whose
- *   records do not fit is re-marched).
+ *   for batches below that threshold the forward writes every contributing
+ *   sample (position, corner links, colour, transmittance) to a record
+ *   stream in the workspace and the backward replays it instead of
+ *   re-marching the ray (a ray whose records do not fit is re-marched).
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,

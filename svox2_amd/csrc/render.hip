@@ -1732,9 +1732,13 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     OrderWorkspace w;
     if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
     RecStream rs{};
-    // Records need slot indices below 2^31 (headers hold them as int32).
+    // Records pay off below the ordering threshold: there the backward's
+    // rays run in one round and replaying saves each one's re-march; above
+    // it the recording forward costs what the replay saves (atomics, not the
+    // march, bound the backward).  Slot indices must stay below 2^31
+    // (headers hold them as int32).
     if (workspace != nullptr && workspace_bytes >= fused_workspace_bytes(n_rays) &&
-        n_rays * (kRecFixed + kRecOverflow) < INT32_MAX) {
+        n_rays < order_min_rays() && n_rays * (kRecFixed + kRecOverflow) < INT32_MAX) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
                            reinterpret_cast<uint32_t*>(rs.next_block), 1);

@@ -227,10 +227,11 @@ def test_fused_record_replay_matches_remarch(gpu, dense64, which):
         rays = svox2_amd.Rays(torch.from_numpy(o).to(gpu), torch.from_numpy(d).to(gpu))
         gt_t = torch.from_numpy(gt).to(gpu)
     else:
+        # Below the ordering threshold (40960 rays on 256 CUs): records on.
         gen = torch.Generator().manual_seed(3)
-        oo, dd = scenes.sphere_rays(65536, gen)
+        oo, dd = scenes.sphere_rays(32768, gen)
         rays = svox2_amd.Rays(oo.to(gpu), dd.to(gpu))
-        gt_t = torch.rand(65536, 3, generator=gen).to(gpu)
+        gt_t = torch.rand(32768, 3, generator=gen).to(gpu)
     out = {}
     for record in (True, False):
         gd, gs, _, _ = grid._get_data_grads()
