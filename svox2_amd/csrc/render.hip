@@ -764,28 +764,25 @@ struct GradOut {
     int64_t gss;
 };
 
-#ifndef SVOX2_SCATTER_LDS
-#define SVOX2_SCATTER_LDS 1
-#endif
-
 // Per-sample record a backward chunk stages in LDS for its scatter loop: every
 // lane of the group then reads the current sample's record at one address (a
 // broadcast read) instead of gathering each field from the sample's lane.
+// `move` says how the sample's cell relates to the previous live sample's:
+// 0 the same cell, kMoveJump a cell that shares no corner with it (or the
+// ray's first cell), otherwise kMoveStep | (d0 + 1) << 1 | (d1 + 1) << 3 |
+// (d2 + 1) << 5 for a step of d in {-1, 0, 1}^3.
 struct __align__(16) SampleRec {
     float p[3];
-    int base;
+    int move;
     float val[4];  // colour factors R, G, B and the density gradient
-    int l[3];
-    int pad;
     int lk[8];
 };
+constexpr int kMoveStep = 1;
+constexpr int kMoveJump = 0x80;
 
-// LDS of one backward workgroup: the sample records of its rays and the
-// accumulators / corner links of the scatter, one entry per lane.
+// LDS of one backward workgroup: the sample records of its rays.
 struct BwdShared {
     SampleRec rec[kBlock];
-    float acc[kBlock];
-    int link[kBlock];
 };
 
 // Orders this wavefront's LDS writes before its later LDS reads.
@@ -794,87 +791,106 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Gradient scatter state of one ray.  Lane L of the 32-lane group holds the
-// accumulator of (corner k = L & 7, quantity q = L >> 3) of the cell the ray is
-// in: q = 0, 1, 2 the colour weight factor of corner k for R, G, B (the SH
-// gradient of channel (c, j) is that factor times basis_j) and q = 3 the
-// density gradient.  Consecutive samples in the same cell add into the same
-// accumulators; moving to an adjacent cell keeps the corners the two cells
-// share (a lane permutation) and flushes only the corners left behind, so a
-// grid point is flushed once per visit.  A flush of corner k is one no-return
-// atomic instruction with lane = SH channel (lanes 0..C-1) plus the density
-// (lane C): the same 128-byte row in the interleaved layout.
+// Gradient scatter state of one ray.  Lane L of the 32-lane group owns one
+// gradient column of a row: SH channel L (c = L / B, j = L % B) for L < C,
+// the density for L == C; lanes above are idle.  For each corner k (bits dx
+// dy dz) of the cell the ray is in, the lane keeps in registers the
+// accumulated factor of its quantity - the colour factor of channel c (the SH
+// gradient is that times basis_j) or the density gradient - and the corner's
+// link.  Consecutive samples in one cell add into the same accumulators;
+// moving to an adjacent cell keeps the corners the two cells share (a
+// register permutation) and flushes the face left behind, so a grid point is
+// flushed once per visit.  A flush of corner k is one no-return atomic
+// instruction over lanes 0..C: the row's SH channels and its density, the
+// same 128-byte row in the interleaved layout.
 template <int B>
 struct Scatter {
     static constexpr int C = 3 * B;
-    int cur = -1;        // flat index of the current cell (-1: none)
-    int c0 = 0, c1 = 0, c2 = 0;
-    float acc = 0.f;     // this lane's (k, q) accumulator
-    int link = -1;       // link of corner k (identical for the 4 q lanes)
-    uint32_t used = 0;   // corners holding gradient
-    float* s_acc = nullptr;  // this group's LDS accumulator slots (32)
-    int* s_link = nullptr;   // this group's LDS link slots (32)
+    float acc[8];
+    int link[8];
+    int c0 = 0, c1 = 0, c2 = 0;  // current cell (when `have`)
+    bool have = false;           // a cell has been entered
+    char* col = nullptr;         // this lane's gradient column (null: idle lane)
+    uint32_t col_stride = 0;     // bytes between the column's rows
+    float mult = 0.f;            // basis_j for SH lanes, 1 for the density lane
 
-    template <typename DT>
-    __device__ __forceinline__ void flush_corner(const GridView<DT>& g, const GradOut& go,
-                                                 int lane, float basis_j, int k) const {
-        const int src = lane < C ? k + 8 * (lane / B) : k + 24;
-        int lk;
-        float v;
-        if (SVOX2_SCATTER_LDS) {
-            lk = s_link[k];
-            v = s_acc[src];
-        } else {
-            lk = __shfl(link, k, kGroup);
-            v = __shfl(acc, src, kGroup);
+    __device__ __forceinline__ void init(const GradOut& go, int lane, float basis_j) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            acc[k] = 0.f;
+            link[k] = -1;
         }
-        if (g.valid(lk)) {
-            if (lane < C) {
-                if (go.gs != nullptr)
-                    atomic_add_f32(go.gs + static_cast<int64_t>(lk) * go.gss + lane, basis_j * v);
-            } else if (lane == C) {
-                if (go.gd != nullptr) atomic_add_f32(go.gd + static_cast<int64_t>(lk) * go.gds, v);
-            }
+        if (lane < C && go.gs != nullptr) {
+            col = reinterpret_cast<char*>(go.gs + lane);
+            col_stride = static_cast<uint32_t>(go.gss * 4);
+            mult = basis_j;
+        } else if (lane == C && go.gd != nullptr) {
+            col = reinterpret_cast<char*>(go.gd);
+            col_stride = static_cast<uint32_t>(go.gds * 4);
+            mult = 1.f;
         }
     }
 
-    // Flushes the corners in `which` (bit k = corner k) and marks their rows.
+    // Adds the accumulated factor a of the corner with link lk to its row.
     template <typename DT>
-    __device__ __forceinline__ void flush(const GridView<DT>& g, const GradOut& go, int lane,
-                                          float basis_j, uint32_t which) {
-        which &= used;
-        if (which == 0) return;
-        if (SVOX2_SCATTER_LDS) {
-            s_acc[lane] = acc;
-            s_link[lane] = link;
-            wave_lds_fence();
-        }
-        for (uint32_t m = which; m; m &= m - 1) flush_corner(g, go, lane, basis_j, __ffs(m) - 1);
-        if (SVOX2_SCATTER_LDS) wave_lds_fence();  // slots are rewritten by the next flush
-        if (go.mask != nullptr && lane < 8 && ((which >> lane) & 1u) && g.valid(link))
-            go.mask[link] = 1;
-        used &= ~which;
+    __device__ __forceinline__ void emit(const GridView<DT>& g, int lk, float a) const {
+        if (col != nullptr && g.valid(lk))
+            atomic_add_f32(reinterpret_cast<float*>(
+                               col + static_cast<uint64_t>(static_cast<uint32_t>(lk)) * col_stride),
+                           mult * a);
     }
 
-    // Shift by d in {-1, 0, 1} along the axis of corner bit `bit`.
-    template <int bit, typename DT>
+    // Marks the rows of up to 4 flushed corners: lane C + 1 + i stores row i's.
+    template <typename DT>
+    __device__ __forceinline__ void mark(const GridView<DT>& g, const GradOut& go, int lane,
+                                         const int* lk4) const {
+        int mk = -1;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) mk = (lane == C + 1 + i) ? lk4[i] : mk;
+        if (go.mask != nullptr && g.valid(mk)) go.mask[mk] = 1;
+    }
+
+    // Flushes all 8 corners and clears them.
+    template <typename DT>
+    __device__ __forceinline__ void flush_all(const GridView<DT>& g, const GradOut& go,
+                                              int lane) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) emit(g, link[k], acc[k]);
+        mark(g, go, lane, link);
+        mark(g, go, lane, link + 4);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            acc[k] = 0.f;
+            link[k] = -1;
+        }
+    }
+
+    // Step by d in {-1, 0, 1} along the axis of corner bit BIT: flush the face
+    // left behind, move the shared face's accumulators to their new corner
+    // index and clear the entering face.
+    template <int BIT, typename DT>
     __device__ __forceinline__ void shift(int d, const GridView<DT>& g, const GradOut& go,
-                                          int lane, float basis_j) {
+                                          int lane) {
         if (d == 0) return;
-        constexpr uint32_t lo = bit == 4 ? 0x0Fu : (bit == 2 ? 0x33u : 0x55u);
-        const bool hi_lane = (lane & bit) != 0;
-        flush(g, go, lane, basis_j, d > 0 ? lo : (~lo & 0xFFu));
-        const float a = xor_lanes<bit>(acc);
-        const int lk = __float_as_int(xor_lanes<bit>(__int_as_float(link)));
-        if (d > 0) {
-            acc = hi_lane ? 0.f : a;
-            link = hi_lane ? -1 : lk;
-            used = (used >> bit) & lo;
-        } else {
-            acc = hi_lane ? a : 0.f;
-            link = hi_lane ? lk : -1;
-            used = (used & lo) << bit;
+        const bool up = d > 0;
+        int dep[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // i-th corner without bit BIT, and its partner with it.
+            constexpr int kLow[3][4] = {{0, 2, 4, 6}, {0, 1, 4, 5}, {0, 1, 2, 3}};
+            const int lo = kLow[BIT == 1 ? 0 : (BIT == 2 ? 1 : 2)][i];
+            const int hi = lo | BIT;
+            const float a_out = up ? acc[lo] : acc[hi];
+            dep[i] = up ? link[lo] : link[hi];
+            emit(g, dep[i], a_out);
+            const float a_keep = up ? acc[hi] : acc[lo];
+            const int l_keep = up ? link[hi] : link[lo];
+            acc[lo] = up ? a_keep : 0.f;
+            acc[hi] = up ? 0.f : a_keep;
+            link[lo] = up ? l_keep : -1;
+            link[hi] = up ? -1 : l_keep;
         }
+        mark(g, go, lane, dep);
     }
 };
 
@@ -886,45 +902,42 @@ struct BwdRay {
     float gq[3];
     float accum;
     float sparsity;
-    float basis_j;  // basis value of this lane's SH channel (lane = c * B + j)
-    int my_k, my_q;
+    int my_q;        // quantity of this lane's column: c for SH lanes, 3 otherwise
     SampleRec* rec;  // this group's LDS records
     Scatter<B> sc;
 
     __device__ __forceinline__ BwdRay(const RayCtx<B>& rc, int lane, const float* g3, float acc0,
-                                      float sp, BwdShared* sh) {
+                                      float sp, BwdShared* sh, const GradOut& go) {
         const int gbase = threadIdx.x & ~(kGroup - 1);
 #pragma unroll
         for (int a = 0; a < 3; ++a) gq[a] = g3[a];
         accum = acc0;
         sparsity = sp;
         // Selects, not a register array indexed by lane (that goes to scratch).
-        basis_j = 0.f;
+        float basis_j = 0.f;
 #pragma unroll
         for (int j = 0; j < B; ++j) basis_j = (lane < C && lane % B == j) ? rc.basis[j] : basis_j;
-        my_k = lane & 7;
-        my_q = lane >> 3;
+        my_q = lane < C ? lane / B : 3;
         rec = sh->rec + gbase;
-        sc.s_acc = sh->acc + gbase;
-        sc.s_link = sh->link + gbase;
+        sc.init(go, lane, basis_j);
     }
 };
 
 // One chunk's contribution to the gradients (trace_ray_cuvol_backward,
 // render_lerp_kernel_cuvol.cu:227-383).  Lane i holds sample i of the chunk if
-// live: its position (p, l, base), corner links, colour + 0.5 (cs), weight,
-// log transmittance after it, world step and density.  The gradient scalars
-// are formed sample-parallel - the colour factors weight * [colour >= 0] * g
-// and the density gradient ws * (total * T_after - accum), with the running
-// `accum` as a prefix sum over the chunk - staged as LDS records, and added
-// in sample order into the (corner, quantity) accumulators.
+// live: its position (p, l), corner links, colour + 0.5 (cs), weight, log
+// transmittance after it, world step and density.  The gradient scalars are
+// formed sample-parallel - the colour factors weight * [colour >= 0] * g and
+// the density gradient ws * (total * T_after - accum), with the running
+// `accum` as a prefix sum over the chunk - and staged as LDS records with each
+// sample's cell move; then the samples are added in order into the corner
+// accumulators.
 template <typename DT, int B>
 __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradOut& go,
                                               BwdRay<B>& st, int lane, bool live,
                                               uint32_t live_mask, const float* p, const int* l,
-                                              int base, const int* lk, const float* cs,
-                                              float weight, float lt_after, float ws,
-                                              float sigma) {
+                                              const int* lk, const float* cs, float weight,
+                                              float lt_after, float ws, float sigma) {
     float total = 0.f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * st.gq[a];
@@ -938,25 +951,52 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     st.accum -= __shfl(run, 31, kGroup);
     float gsig = ws * (total * __expf(lt_after) - acc_s);
     if (st.sparsity > 0.f) gsig += st.sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
+    Scatter<B>& sc = st.sc;
+    // Cell of the previous live sample: an earlier lane of this chunk, or the
+    // cell the scatter is in.
+    const uint32_t before = live_mask & ((1u << lane) - 1u);
+    const int prev = before ? 31 - __clz(before) : 0;
+    int pl[3];
+    pl[0] = __shfl(l[0], prev, kGroup);
+    pl[1] = __shfl(l[1], prev, kGroup);
+    pl[2] = __shfl(l[2], prev, kGroup);
+    if (!before) {
+        pl[0] = sc.c0;
+        pl[1] = sc.c1;
+        pl[2] = sc.c2;
+    }
+    const bool had = before != 0u || sc.have;
     if (live) {
+        const int d0 = l[0] - pl[0], d1 = l[1] - pl[1], d2 = l[2] - pl[2];
+        int move;
+        if (!had || d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
+            move = kMoveJump;
+        else if ((d0 | d1 | d2) == 0)
+            move = 0;
+        else
+            move = kMoveStep | ((d0 + 1) << 1) | ((d1 + 1) << 3) | ((d2 + 1) << 5);
         SampleRec& R = st.rec[lane];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             R.p[a] = p[a];
-            R.l[a] = l[a];
             R.val[a] = weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * st.gq[a];
         }
-        R.base = base;
+        R.move = move;
         R.val[3] = gsig;
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) R.lk[kk] = lk[kk];
     }
+    // The scatter ends the chunk in the last live sample's cell.
+    const int last = 31 - __clz(live_mask);
+    sc.c0 = __shfl(l[0], last, kGroup);
+    sc.c1 = __shfl(l[1], last, kGroup);
+    sc.c2 = __shfl(l[2], last, kGroup);
+    sc.have = true;
     wave_lds_fence();
-    Scatter<B>& sc = st.sc;
     // Each iteration reads the next sample's position and value ahead, so
     // their LDS latency overlaps the current sample's work.
     uint32_t m = live_mask;
-    int s_next = m ? __ffs(m) - 1 : 0;
+    int s_next = __ffs(m) - 1;
     float4 pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
     float v_next = st.rec[s_next].val[st.my_q];
     while (m) {
@@ -969,30 +1009,28 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
             pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
             v_next = st.rec[s_next].val[st.my_q];
         }
-        const SampleRec& R = st.rec[s_cur];
-        const int cell = __float_as_int(pb.w);
-        if (cell != sc.cur) {
-            const int l0 = R.l[0], l1 = R.l[1], l2 = R.l[2];
-            const int d0 = l0 - sc.c0, d1 = l1 - sc.c1, d2 = l2 - sc.c2;
-            const bool adjacent = sc.cur >= 0 && d0 >= -1 && d0 <= 1 && d1 >= -1 && d1 <= 1 &&
-                                  d2 >= -1 && d2 <= 1;
-            if (adjacent) {
-                sc.template shift<4>(d0, g, go, lane, st.basis_j);
-                sc.template shift<2>(d1, g, go, lane, st.basis_j);
-                sc.template shift<1>(d2, g, go, lane, st.basis_j);
+        const int move = __float_as_int(pb.w);
+        if (move != 0) {
+            const SampleRec& R = st.rec[s_cur];
+            const int4 la = *reinterpret_cast<const int4*>(R.lk);
+            const int4 lb = *reinterpret_cast<const int4*>(R.lk + 4);
+            if (move & kMoveJump) {
+                sc.flush_all(g, go, lane);
             } else {
-                sc.flush(g, go, lane, st.basis_j, 0xFFu);
-                sc.acc = 0.f;
+                sc.template shift<4>(((move >> 1) & 3) - 1, g, go, lane);
+                sc.template shift<2>(((move >> 3) & 3) - 1, g, go, lane);
+                sc.template shift<1>(((move >> 5) & 3) - 1, g, go, lane);
             }
-            sc.link = R.lk[st.my_k];
-            sc.cur = cell;
-            sc.c0 = l0;
-            sc.c1 = l1;
-            sc.c2 = l2;
+            sc.link[0] = la.x; sc.link[1] = la.y; sc.link[2] = la.z; sc.link[3] = la.w;
+            sc.link[4] = lb.x; sc.link[5] = lb.y; sc.link[6] = lb.z; sc.link[7] = lb.w;
         }
-        const float pp[3] = {pb.x, pb.y, pb.z};
-        sc.acc = fmaf(corner_weight(st.my_k, pp), v, sc.acc);
-        sc.used = 0xFFu;
+        // Trilinear weights (wy * wz) * wx of the 8 corners.
+        const float wx[2] = {1.f - pb.x, pb.x};
+        const float wy[2] = {1.f - pb.y, pb.y};
+        const float wz[2] = {1.f - pb.z, pb.z};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            sc.acc[k] = fmaf(wy[(k >> 1) & 1] * wz[k & 1] * wx[k >> 2], v, sc.acc[k]);
     }
     wave_lds_fence();  // records are rewritten by the next chunk
 }
@@ -1014,14 +1052,14 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
         if (c.live_mask != 0u) {
             float cs[3] = {0.f, 0.f, 0.f};
             chunk_colours<DT, B, true>(g, rc, c, lane, cs, nullptr);
-            scatter_chunk<DT, B>(g, go, st, lane, c.live, c.live_mask, c.p, c.l, c.base, c.lk,
-                                 cs, c.weight, c.lt_after, c.ws, c.sigma);
+            scatter_chunk<DT, B>(g, go, st, lane, c.live, c.live_mask, c.p, c.l, c.lk, cs,
+                                 c.weight, c.lt_after, c.ws, c.sigma);
         }
         log_t = c.log_t_end;
         t = c.next_t;
         more = c.more;
     }
-    st.sc.flush(g, go, lane, st.basis_j, 0xFFu);
+    st.sc.flush_all(g, go, lane);
 }
 
 // Backward of one ray from its record stream (the fused call's forward wrote
@@ -1043,12 +1081,11 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
         const bool live = lane < n;
         float p[3] = {0.f, 0.f, 0.f}, cs[3] = {0.f, 0.f, 0.f};
         int l[3] = {0, 0, 0}, lk[8];
-        int base = 0;
         float weight = 0.f, lt_after = 0.f, ws = 0.f, sigma = 0.f;
         if (live) {
             const float4* q = rs.rec + 6 * (pos + 1 + lane);
             const float4 a = q[0], b = q[1], c2 = q[2], d = q[3], e = q[4], f = q[5];
-            p[0] = a.x; p[1] = a.y; p[2] = a.z; base = __float_as_int(a.w);
+            p[0] = a.x; p[1] = a.y; p[2] = a.z;
             l[0] = __float_as_int(b.x); l[1] = __float_as_int(b.y); l[2] = __float_as_int(b.z);
             weight = b.w;
             lk[0] = __float_as_int(c2.x); lk[1] = __float_as_int(c2.y);
@@ -1062,11 +1099,11 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
             for (int kk = 0; kk < 8; ++kk) lk[kk] = -1;
         }
         const uint32_t live_mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-        scatter_chunk<DT, B>(g, go, st, lane, live, live_mask, p, l, base, lk, cs, weight,
-                             lt_after, ws, sigma);
+        scatter_chunk<DT, B>(g, go, st, lane, live, live_mask, p, l, lk, cs, weight, lt_after,
+                             ws, sigma);
         pos += n + 1;
     }
-    st.sc.flush(g, go, lane, st.basis_j, 0xFFu);
+    st.sc.flush_all(g, go, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1257,7 +1294,7 @@ SVOX2_BWD_KERNEL void k_render_backward(
     const float* cc = color_cache + 3 * ray;
     const float accum = fmaf(cc[0], gq[0], fmaf(cc[1], gq[1], cc[2] * gq[2]));
     __shared__ BwdShared sh;
-    BwdRay<B> st(rc, lane, gq, accum, 0.f, &sh);
+    BwdRay<B> st(rc, lane, gq, accum, 0.f, &sh, go);
     march_backward<DT, B>(g, o, r, rc, lane, st, go);
 }
 
@@ -1289,7 +1326,7 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
     __shared__ BwdShared sh;
-    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh);
+    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
     replay_backward<DT, B>(g, rs, ray, lane, st, go);
 }
 
@@ -1326,7 +1363,7 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
     __shared__ BwdShared sh;
-    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh);
+    BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
     march_backward<DT, B>(g, o, r, rc, lane, st, go);
 }
 
@@ -1441,6 +1478,20 @@ RecStream record_stream(void* ws, int64_t n) {
     rs.overflow_base = n * kRecFixed;
     rs.overflow_slots = n * kRecOverflow;
     return rs;
+}
+
+// Gradient descriptor checks of the backward entry points: the scatter
+// addresses a row as link * stride with a 32-bit byte stride.
+int check_grads(const svox2_grads_t* grads, int sh_dim, GradOut& go) {
+    SVOX2_REQUIRE(grads != nullptr, "grads is NULL");
+    SVOX2_REQUIRE(grads->grad_density_stride >= 0 && grads->grad_sh_stride >= 0,
+                  "negative gradient stride");
+    go = GradOut{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
+                 grad_dstride(*grads), grad_sstride(*grads, sh_dim)};
+    SVOX2_REQUIRE(go.gss >= sh_dim, "grad_sh_stride below sh_data_dim");
+    SVOX2_REQUIRE(go.gds < (int64_t{1} << 30) && go.gss < (int64_t{1} << 30),
+                  "gradient row strides must be below 2^30 elements");
+    return ok();
 }
 
 int check_order_workspace(const void* workspace, size_t bytes, int64_t n) {
@@ -1669,13 +1720,12 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     OptView o;
     if ((st = make_opt(opt, o))) return st;
     if ((st = check_rays(origins, dirs, n_rays))) return st;
-    SVOX2_REQUIRE(grads != nullptr, "grads is NULL");
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(grad_out != nullptr && color_cache != nullptr,
                   "grad_out / color_cache are NULL");
     if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
-    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
-                     grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
@@ -1716,14 +1766,13 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     OptView o;
     if ((st = make_opt(opt, o))) return st;
     if ((st = check_rays(origins, dirs, n_rays))) return st;
-    SVOX2_REQUIRE(grads != nullptr, "grads is NULL");
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
     SVOX2_REQUIRE(!(beta_loss > 0.f) || log_transmit_out != nullptr,
                   "beta_loss > 0 needs log_transmit_out (n floats)");
     if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
-    const GradOut go{grads->grad_density_out, grads->grad_sh_out, grads->mask_out,
-                     grad_dstride(*grads), grad_sstride(*grads, grid->sh_data_dim)};
     const float norm = 2.f / static_cast<float>(3 * n_rays);
     const float beta = beta_loss / static_cast<float>(n_rays);
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
