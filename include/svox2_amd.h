@@ -151,8 +151,9 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  * without it):
  * - at least svox2_ray_order_workspace_bytes(n_rays) bytes: for batches of at
  *   least four times the rays the device holds resident (40960 on a 256-CU
- *   MI355X) the forward records each ray's chunk count and the backward
- *   processes the rays in descending order of it;
+ *   MI355X) the forward processes the rays in descending order of their
+ *   segment length in the grid, records each ray's chunk count, and the
+ *   backward processes the rays in descending order of that count;
  * - at least svox2_volume_render_cuvol_fused_workspace_bytes(n_rays) bytes:
  *   for batches below that threshold the forward writes every contributing
  *   sample (position, corner links, colour, transmittance) to a record
@@ -175,9 +176,10 @@ int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
  * (0 for an empty batch).  No counterpart in the reference. */
 size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 
-/* Device workspace bytes of svox2_volume_render_cuvol_fused with ray ordering
- * and the record stream (0 for an empty batch): about 4.6 KiB per ray.  No
- * counterpart in the reference. */
+/* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
+ * fastest for an n_rays batch (0 for an empty batch): the record stream
+ * (about 4.6 KiB per ray) below the ordering threshold, the ray-ordering
+ * workspace (8 bytes per ray) above it.  No counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
 /* One tile of a multi-request render launch: n_pixels consecutive pixels

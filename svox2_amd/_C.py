@@ -587,7 +587,8 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     gs = _grads_struct(grads, g.rows, g.sh_data_dim)
     log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
              if beta_loss > 0 else None)
-    # Workspace for the backward's ray ordering (from torch's caching allocator).
+    # Workspace for the ray ordering and, for small batches, the record
+    # stream (from torch's caching allocator).
     lib = load_library()
     if ordered and record:
         nbytes = lib.svox2_volume_render_cuvol_fused_workspace_bytes(o.size(0))

@@ -1123,6 +1123,11 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
     __attribute__((amdgpu_waves_per_eu(SVOX2_FWD_WAVES)))
 #define SVOX2_BWD_KERNEL __global__ __launch_bounds__(kBlock) \
     __attribute__((amdgpu_waves_per_eu(SVOX2_BWD_WAVES)))
+// Record the fused call's samples at every batch size (a tuning switch; by
+// default only batches below the ordering threshold record).
+#ifndef SVOX2_RECORD_ALL
+#define SVOX2_RECORD_ALL 0
+#endif
 #ifndef SVOX2_REC_WAVES
 #define SVOX2_REC_WAVES 4
 #endif
@@ -1142,10 +1147,11 @@ SVOX2_REC_KERNEL void k_render_rays_rec(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         float* __restrict__ rgb_out, float* __restrict__ log_t_out,
-        uint32_t* __restrict__ cost_out, RecStream rs) {
-    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+        uint32_t* __restrict__ cost_out, RecStream rs, const uint32_t* __restrict__ order) {
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
-    if (ray >= n) return;
+    if (slot >= n) return;
+    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
@@ -1165,10 +1171,12 @@ SVOX2_FWD_KERNEL void k_render_rays(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         float* __restrict__ rgb_out, float* __restrict__ log_t_out,
-        int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out, RecStream rs) {
-    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+        int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out, RecStream rs,
+        const uint32_t* __restrict__ order) {
+    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
-    if (ray >= n) return;
+    if (slot >= n) return;
+    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
     const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
@@ -1427,6 +1435,30 @@ __global__ __launch_bounds__(kSortBlock) void k_zero_words(uint32_t* __restrict_
     if (i < n) p[i] = 0u;
 }
 
+// Sort key of the forward: the length of the ray's [tmin, tmax] segment in
+// voxels, scaled to the cost bins (bins_per_voxel = bins / grid diagonal).
+// The forward's work per ray is not known before it runs; this length ranks
+// rays close to their measured chunk counts (empty space is skipped in a few
+// chunks, but a longer segment crosses more of it and more of the occupied
+// shell), and ordering by it pairs rays of similar length in a wave and
+// starts the long ones first.
+__global__ __launch_bounds__(kSortBlock) void k_ray_length_key(
+        GridView<float> g, OptView o, const float* __restrict__ origins,
+        const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
+        float bins_per_voxel, uint32_t* __restrict__ key) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
+    if (i >= n) return;
+    const float wo[3] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
+    const float wd[3] = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    Ray r;
+    setup_ray(g, o, wo, wd, clip ? clip + 2 * i : nullptr, r);
+    const float len = r.tmax - r.tmin;
+    uint32_t b = 0;
+    if (len > 0.f)  // NaN-safe: a NaN length keeps bin 0
+        b = static_cast<uint32_t>(fminf(len * bins_per_voxel, static_cast<float>(kCostBins - 1)));
+    key[i] = b;
+}
+
 struct OrderWorkspace {
     uint32_t* hist = nullptr;
     uint32_t* offsets = nullptr;
@@ -1609,7 +1641,8 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
         using DT = std::remove_pointer_t<decltype(dtp)>;
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
-                           log_transmit_out, (int32_t*)nullptr, ray_cost_out, RecStream{});
+                           log_transmit_out, (int32_t*)nullptr, ray_cost_out, RecStream{},
+                           (const uint32_t*)nullptr);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol");
@@ -1633,7 +1666,7 @@ extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const f
         hipLaunchKernelGGL((k_render_rays<DT, B, true>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays,
                            (float*)nullptr, (float*)nullptr, stats_out, (uint32_t*)nullptr,
-                           RecStream{});
+                           RecStream{}, (const uint32_t*)nullptr);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_stats");
@@ -1748,8 +1781,19 @@ extern "C" size_t svox2_ray_order_workspace_bytes(int64_t n_rays) {
     return n_rays > 0 ? order_workspace_bytes(n_rays) : 0;
 }
 
+// Whether the fused call records its samples for a batch of n rays (given a
+// workspace of fused_workspace_bytes(n)).  Records pay off below the ordering
+// threshold: there the backward's rays run in one round and replaying saves
+// each one's re-march.  Slot indices must stay below 2^31 (headers hold them
+// as int32).
+bool fused_records(int64_t n) {
+    return (SVOX2_RECORD_ALL || n < order_min_rays()) &&
+           n * (kRecFixed + kRecOverflow) < INT32_MAX;
+}
+
 extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
-    return n_rays > 0 ? fused_workspace_bytes(n_rays) : 0;
+    if (n_rays <= 0) return 0;
+    return fused_records(n_rays) ? fused_workspace_bytes(n_rays) : order_workspace_bytes(n_rays);
 }
 
 extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
@@ -1781,13 +1825,8 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     OrderWorkspace w;
     if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
     RecStream rs{};
-    // Records pay off below the ordering threshold: there the backward's
-    // rays run in one round and replaying saves each one's re-march; above
-    // it the recording forward costs what the replay saves (atomics, not the
-    // march, bound the backward).  Slot indices must stay below 2^31
-    // (headers hold them as int32).
-    if (workspace != nullptr && workspace_bytes >= fused_workspace_bytes(n_rays) &&
-        n_rays < order_min_rays() && n_rays * (kRecFixed + kRecOverflow) < INT32_MAX) {
+    if (workspace != nullptr && fused_records(n_rays) &&
+        workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
                            reinterpret_cast<uint32_t*>(rs.next_block), 1);
@@ -1795,13 +1834,28 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
+        // Large batches: the forward runs in order of segment length, then
+        // the backward in order of the forward's work counts (both sorts go
+        // through w.cost / w.order; the forward rewrites w.cost by ray).
+        if (w.order != nullptr) {
+            const double diag = sqrt(static_cast<double>(grid->size[0]) * grid->size[0] +
+                                     static_cast<double>(grid->size[1]) * grid->size[1] +
+                                     static_cast<double>(grid->size[2]) * grid->size[2]);
+            hipLaunchKernelGGL(k_ray_length_key, dim3(blocks_for(n_rays, kSortBlock)),
+                               dim3(kSortBlock), 0, s, make_view<float>(*grid), o, origins, dirs,
+                               ray_clip, n_rays, static_cast<float>((kCostBins - 1) / diag),
+                               w.cost);
+            launch_cost_order(w.cost, w, n_rays, s);
+        }
+        const uint32_t* order = w.order;
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_rays_rec<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
-                               o, origins, dirs, ray_clip, n_rays, rgb_out, lt, w.cost, rs);
+                               o, origins, dirs, ray_clip, n_rays, rgb_out, lt, w.cost, rs,
+                               order);
         else
             hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
                                g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
-                               (int32_t*)nullptr, w.cost, rs);
+                               (int32_t*)nullptr, w.cost, rs, order);
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s);
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
