@@ -212,6 +212,25 @@ __device__ __forceinline__ float sum8(float v) {
     return v;
 }
 
+// Inclusive prefix sum over each 32-lane group by DPP moves: shifts by 1, 2,
+// 4 and 8 within each 16-lane row (lanes shifted in read 0), then the second
+// row of each group adds the first row's total (row broadcast of lane 15).
+__device__ __forceinline__ float group_scan(float v) {
+    v += dpp<0x111>(v);  // row_shr:1
+    v += dpp<0x112>(v);  // row_shr:2
+    v += dpp<0x114>(v);  // row_shr:4
+    v += dpp<0x118>(v);  // row_shr:8
+    // row_bcast:15 into rows 1 and 3; rows 0 and 2 keep 0.
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
+    return v;
+}
+
+// Value of lane - 1 (wave shift right by one; lane 0 of each group must not
+// use it: it reads the previous group's last lane or nothing).
+__device__ __forceinline__ int prev_lane(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
+}
+
 // Value of lane ^ BIT (BIT = 1, 2 or 4): quad permutes for 1 and 2, a
 // bit-mode swizzle (xor mask 4 within 32 lanes) for 4.
 template <int BIT>
@@ -219,6 +238,39 @@ __device__ __forceinline__ float xor_lanes(float v) {
     if constexpr (BIT == 1) return dpp<0xB1>(v);
     else if constexpr (BIT == 2) return dpp<0x4E>(v);
     else return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+}
+
+// Lane j of this lane's 32-lane group, j the same in every lane of the
+// wavefront (both groups read their own lane j): two scalar lane reads, no
+// LDS round trip, whatever lanes are active.
+__device__ __forceinline__ float group_lane(float v, int j) {
+    const int bits = __float_as_int(v);
+    const int a = __builtin_amdgcn_readlane(bits, j);
+    const int b = __builtin_amdgcn_readlane(bits, j + 32);
+    return __int_as_float((threadIdx.x & 32u) ? b : a);
+}
+
+// Lane j of this lane's group, j the same across each group (it may differ
+// between the two groups): lanes 0 and 32 give each group's j.  A group that
+// is not active reads some lane of its own half, unused.
+__device__ __forceinline__ float group_bcast(float v, int j) {
+    const int ja = __builtin_amdgcn_readlane(j, 0) & 31;
+    const int jb = __builtin_amdgcn_readlane(j, 32) & 31;
+    const int bits = __float_as_int(v);
+    const int a = __builtin_amdgcn_readlane(bits, ja);
+    const int b = __builtin_amdgcn_readlane(bits, jb + 32);
+    return __int_as_float((threadIdx.x & 32u) ? b : a);
+}
+__device__ __forceinline__ int group_bcast(int v, int j) {
+    return __float_as_int(group_bcast(__int_as_float(v), j));
+}
+
+// 1 + the highest set lane of pred in either group (0 if none), the same in
+// every lane; inactive lanes count as unset.
+__device__ __forceinline__ int wave_last_set(bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const uint32_t a = static_cast<uint32_t>(m), b = static_cast<uint32_t>(m >> 32);
+    return max(a ? 32 - static_cast<int>(__clz(a)) : 0, b ? 32 - static_cast<int>(__clz(b)) : 0);
 }
 
 // Position (0-based) of the n-th set bit of m; m must have more than n bits set.
@@ -272,16 +324,21 @@ __device__ __forceinline__ float corner_weight(int k, const float* p) {
     return wy * wz * wx;
 }
 
-// One chunk of up to 32 consecutive samples of a ray, one sample per lane.
+// One chunk of up to 32 evaluated samples of a ray, one sample per lane, in
+// march order.
 //
-// Lane i holds the sample at the i-th step from the chunk start t0, with t
-// advanced by the same sequence of float additions the sequential march
-// performs, so positions, cells, skips and densities are those of the
-// reference march (render_lerp_kernel_cuvol.cu:54-86).  The chunk ends at the
-// first lane whose cell is skipped (that lane's skip sets the next chunk start)
-// or at tmax.  The log transmittance before each sample is replayed in sample
-// order with the same float subtractions, so the stop decision (:103-106) and
-// every later sample agree bit for bit with the sequential march.
+// The chunk is filled segment by segment: a segment is a run of consecutive
+// steps t, t + step, ... with t advanced by the same sequence of float
+// additions the sequential march performs, and it ends at the first sample
+// whose cell is skipped.  That sample is not evaluated: the skip sets the start
+// of the next segment (render_lerp_kernel_cuvol.cu:54-86), which continues on
+// the skipped sample's lane.  Positions, cells, skips and densities are those
+// of the reference march.  The log transmittance before each sample is
+// replayed in sample order with the same float subtractions, so the stop
+// decision (:103-106) and every later sample agree bit for bit with the
+// sequential march.  Filling a chunk across skips keeps its lanes busy where
+// the ray crawls along a surface (a skip every few samples): each skip costs
+// one cell-map round trip instead of a chunk of its own.
 struct Chunk {
     float t;          // this lane's sample t
     float p[3];       // fractional position in the cell
@@ -296,13 +353,17 @@ struct Chunk {
     bool contrib;     // evaluated and sigma > sigma_thresh
     bool live;        // contributes and lies at or before the stop sample
     uint32_t live_mask;
-    int n_eval;       // evaluated samples (before the first skip / tmax / stop)
+    int n_eval;       // evaluated samples (lanes 0 .. n_eval - 1)
     int n_iter;       // march iterations consumed (evaluated + one per skip)
     bool stopped;     // the stop test fired in this chunk
     bool more;        // the march continues after this chunk
     float next_t;     // start of the next chunk
     float log_t_end;  // log transmittance after the chunk
 };
+
+// Segments (skips + 1) at most per chunk: a chunk that reaches it ends early
+// and the next chunk continues after the last skip.
+constexpr int kMaxSegments = 16;
 
 // True when t0, t0 + step, ..., t0 + 31 step are all exactly representable in
 // the binade of t0: then the running sum t += step never rounds.  t0 must be
@@ -325,87 +386,106 @@ template <typename DT>
 __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView& o,
                                             const Ray& r, int lane, float t0, float log_t,
                                             Chunk& c) {
-    // t of the lane-th step, as the sequential float additions t += step
-    // give it.  When every t of the chunk lies in the binade of t0 (t0 > 0
-    // normal) and step is a whole multiple of that binade's ulp, each of those
-    // additions is exact, so t0 + lane * step (one fma, exact) is the same
-    // float; otherwise the chain is replayed.
-    float ti;
-    if (sum_chain_exact(t0, o.step)) {
-        ti = fmaf(static_cast<float>(lane), o.step, t0);
-    } else {
-        ti = t0;
-#pragma unroll
-        for (int k = 1; k < 32; ++k) {
-            const float nx = ti + o.step;
-            ti = (k <= lane) ? nx : ti;
-        }
-    }
-    c.t = ti;
-    const bool in_range = ti <= r.tmax;
-    c.base = sample_cell(g, r, ti, c.p, c.l);
     const int sx = g.stride_x, sy = g.stride_y;
-    float skip = 0.f;
-    bool is_skip = false;
+    // Phase 1: place the samples.  Each round gives the lanes from seg_lane on
+    // the steps of the segment starting at seg_t, reads their cells' skip
+    // levels and finds the segment's end.
+    int seg_lane = 0;      // first lane of the current segment
+    float seg_t = t0;      // t of its first sample
+    int n_skips = 0;       // skips before the current segment
+    int my_skips = 0;      // skips before this lane's sample
     bool occupied = true;  // the cell may have a corner link that addresses a row
-    if (g.cmap != nullptr) {
-        // Skip level and occupancy from the cell map: a cache-resident read
-        // in place of the 8 link reads for cells whose corners are all empty.
-        uint32_t info = 0;
-        if (in_range)
-            info = g.cmap[(c.l[0] >> 1) * g.mstride_x + (c.l[1] >> 1) * g.mstride_y +
-                          (c.l[2] >> 1)];
-        int level = static_cast<int>(info & 0xFFu);
-        if (level == 255) {  // block mixes levels: read this cell's own link
-            const int l0 = g.links[c.base];
-            level = l0 < -1 ? -l0 - 1 : 0;
+    float ti = t0;
+    int n_place;           // evaluated lanes 0 .. n_place - 1
+    bool more;
+    float next_t = 0.f;
+    for (int seg = 1;; ++seg) {
+        const bool mine = lane >= seg_lane;
+        if (mine) {
+            // t of the k-th step of the segment, as the sequential float
+            // additions t += step give it.  When every t of the segment lies
+            // in the binade of seg_t (seg_t > 0 normal) and step is a whole
+            // multiple of that binade's ulp, each of those additions is exact,
+            // so seg_t + k * step (one fma, exact) is the same float;
+            // otherwise the chain is replayed.
+            const int k = lane - seg_lane;
+            if (sum_chain_exact(seg_t, o.step)) {
+                ti = fmaf(static_cast<float>(k), o.step, seg_t);
+            } else {
+                ti = seg_t;
+#pragma unroll
+                for (int kk = 1; kk < 32; ++kk) {
+                    const float nx = ti + o.step;
+                    ti = (kk <= k) ? nx : ti;
+                }
+            }
+            c.base = sample_cell(g, r, ti, c.p, c.l);
+            my_skips = n_skips;
         }
-        occupied = ((info >> (8 + ((c.l[0] & 1) << 2) + ((c.l[1] & 1) << 1) + (c.l[2] & 1))) &
-                    1u) != 0;
-        if (in_range && level > 0) {
-            skip = skip_dist(-1 - level, c.l, c.p, r);
-            is_skip = skip >= o.step;
-        }
-    } else {
+        const bool in_range = mine && ti <= r.tmax;
+        float skip = 0.f;
+        bool is_skip = false;
         if (in_range) {
-            c.lk[0] = g.links[c.base];
-            c.lk[1] = g.links[c.base + 1];
-            c.lk[2] = g.links[c.base + sy];
-            c.lk[3] = g.links[c.base + sy + 1];
-            c.lk[4] = g.links[c.base + sx];
-            c.lk[5] = g.links[c.base + sx + 1];
-            c.lk[6] = g.links[c.base + sx + sy];
-            c.lk[7] = g.links[c.base + sx + sy + 1];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) c.lk[k] = -1;
+            int level;
+            if (g.cmap != nullptr) {
+                // Skip level and occupancy from the cell map: a cache-resident
+                // read in place of the 8 link reads for cells whose corners are
+                // all empty.
+                const uint32_t info = g.cmap[(c.l[0] >> 1) * g.mstride_x +
+                                             (c.l[1] >> 1) * g.mstride_y + (c.l[2] >> 1)];
+                level = static_cast<int>(info & 0xFFu);
+                if (level == 255) {  // block mixes levels: read this cell's own link
+                    const int l0 = g.links[c.base];
+                    level = l0 < -1 ? -l0 - 1 : 0;
+                }
+                occupied = ((info >> (8 + ((c.l[0] & 1) << 2) + ((c.l[1] & 1) << 1) +
+                                      (c.l[2] & 1))) & 1u) != 0;
+            } else {
+                const int l0 = g.links[c.base];
+                level = l0 < -1 ? -l0 - 1 : 0;
+            }
+            if (level > 0) {
+                skip = skip_dist(-1 - level, c.l, c.p, r);
+                is_skip = skip >= o.step;
+            }
         }
-        if (in_range && c.lk[0] < -1) {
-            skip = skip_dist(c.lk[0], c.l, c.p, r);
-            is_skip = skip >= o.step;
+        const uint32_t skip_mask = group_ballot(is_skip);
+        // t grows along the segment, so its in-range lanes are a prefix.
+        const int range_end = seg_lane + __popc(group_ballot(in_range));
+        const int first_skip = skip_mask ? (__ffs(skip_mask) - 1) : 32;
+        if (first_skip < range_end) {
+            const float ts = group_bcast(ti, first_skip);
+            const float sk = group_bcast(skip, first_skip);
+            next_t = ts + ceilf(sk / o.step) * o.step;
+            ++n_skips;
+            seg_lane = first_skip;
+            seg_t = next_t;
+            if (seg < kMaxSegments) continue;
+            n_place = seg_lane;
+            more = true;
+            break;
         }
+        n_place = range_end;
+        more = range_end == 32;
+        if (more) next_t = group_lane(ti, 31) + o.step;
+        break;
     }
-    const uint32_t skip_mask = group_ballot(is_skip);
-    const int n_range = __popc(group_ballot(in_range));
-    const int first_skip = skip_mask ? (__ffs(skip_mask) - 1) : 32;
-    const int n_eval = min(first_skip, n_range);
-    const bool eval = lane < n_eval;
-    if (g.cmap != nullptr) {
-        if (eval && occupied) {
-            c.lk[0] = g.links[c.base];
-            c.lk[1] = g.links[c.base + 1];
-            c.lk[2] = g.links[c.base + sy];
-            c.lk[3] = g.links[c.base + sy + 1];
-            c.lk[4] = g.links[c.base + sx];
-            c.lk[5] = g.links[c.base + sx + 1];
-            c.lk[6] = g.links[c.base + sx + sy];
-            c.lk[7] = g.links[c.base + sx + sy + 1];
-        } else {
+    const bool eval = lane < n_place;
+    c.t = ti;
+    // Phase 2: corner links and densities of the evaluated samples.
+    if (eval && (g.cmap == nullptr || occupied)) {
+        c.lk[0] = g.links[c.base];
+        c.lk[1] = g.links[c.base + 1];
+        c.lk[2] = g.links[c.base + sy];
+        c.lk[3] = g.links[c.base + sy + 1];
+        c.lk[4] = g.links[c.base + sx];
+        c.lk[5] = g.links[c.base + sx + 1];
+        c.lk[6] = g.links[c.base + sx + sy];
+        c.lk[7] = g.links[c.base + sx + sy + 1];
+    } else {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) c.lk[k] = -1;
-        }
+        for (int k = 0; k < 8; ++k) c.lk[k] = -1;
     }
-
     float dn[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -421,11 +501,12 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
     // the last contributing one subtract anything (x - 0 == x), so the replay
     // stops there, and chunks without a contribution (empty-space crawls)
     // skip it.
+    // The replay runs to the later of the two groups' last contributions
+    // (the other group subtracts zeros there), with a wave-uniform j.
     float lt = log_t;
-    const uint32_t contrib_mask = group_ballot(c.contrib);
-    const int n_rep = contrib_mask ? 32 - __clz(contrib_mask) : 0;
+    const int n_rep = wave_last_set(c.contrib);
     for (int j = 0; j < n_rep; ++j) {
-        const float pj = __shfl(pcnt, j, kGroup);
+        const float pj = group_lane(pcnt, j);
         if (j < lane) lt -= pj;
     }
     c.lt_before = lt;
@@ -439,25 +520,17 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
 
     if (c.stopped) {
         c.n_eval = first_stop + 1;
-        c.n_iter = first_stop + 1;
+        c.n_iter = first_stop + 1 + group_bcast(my_skips, first_stop);
         c.more = false;
         c.log_t_end = -1e3f;
         c.next_t = 0.f;
         return;
     }
-    c.n_eval = n_eval;
-    c.log_t_end = n_eval > 0 ? __shfl(c.lt_after, n_eval - 1, kGroup) : log_t;
-    if (first_skip < n_range) {
-        c.n_iter = n_eval + 1;
-        const float ts = __shfl(ti, first_skip, kGroup);
-        const float sk = __shfl(skip, first_skip, kGroup);
-        c.next_t = ts + ceilf(sk / o.step) * o.step;
-        c.more = true;
-    } else {
-        c.n_iter = n_eval;
-        c.more = n_range == 32;
-        c.next_t = __shfl(ti, 31, kGroup) + o.step;
-    }
+    c.n_eval = n_place;
+    c.n_iter = n_place + n_skips;
+    c.log_t_end = n_place > 0 ? group_bcast(c.lt_after, n_place - 1) : log_t;
+    c.more = more;
+    c.next_t = next_t;
 }
 
 // Loads the C SH channels of row lk.  Rows that are 16-byte aligned with a
@@ -582,7 +655,7 @@ __device__ __forceinline__ void chunk_run_colors(const GridView<DT>& g, const Ra
 // Run heads of a chunk: live samples whose previous lane is not a live
 // sample of the same cell.
 __device__ __forceinline__ uint32_t run_heads(const Chunk& c, int lane) {
-    const int prev_cell = __shfl_up(c.base, 1, kGroup);
+    const int prev_cell = prev_lane(c.base);
     const bool prev_live = lane > 0 && ((c.live_mask >> (lane - 1)) & 1u);
     return group_ballot(c.live && !(prev_live && prev_cell == c.base));
 }
@@ -737,7 +810,7 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     }
     const float tb = __expf(log_t) * o.bg;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[a] = __shfl(acc[a], 0, kGroup) + tb;
+    for (int a = 0; a < 3; ++a) out[a] = group_lane(acc[a], 0) + tb;
     // A replayed backward works per record and per recorded chunk, a
     // re-marching one per chunk.
     if (cost != nullptr)
@@ -941,14 +1014,9 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     float total = 0.f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) total += fmaxf(cs[a], 0.f) * st.gq[a];
-    float run = live ? weight * total : 0.f;
-#pragma unroll
-    for (int off = 1; off < 32; off <<= 1) {
-        const float y = __shfl_up(run, off, kGroup);
-        if (lane >= off) run += y;
-    }
+    const float run = group_scan(live ? weight * total : 0.f);
     const float acc_s = st.accum - run;
-    st.accum -= __shfl(run, 31, kGroup);
+    st.accum -= group_lane(run, 31);
     float gsig = ws * (total * __expf(lt_after) - acc_s);
     if (st.sparsity > 0.f) gsig += st.sparsity * (4.f * sigma / (1.f + 2.f * (sigma * sigma)));
     Scatter<B>& sc = st.sc;
@@ -988,9 +1056,9 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     }
     // The scatter ends the chunk in the last live sample's cell.
     const int last = 31 - __clz(live_mask);
-    sc.c0 = __shfl(l[0], last, kGroup);
-    sc.c1 = __shfl(l[1], last, kGroup);
-    sc.c2 = __shfl(l[2], last, kGroup);
+    sc.c0 = group_bcast(l[0], last);
+    sc.c1 = group_bcast(l[1], last);
+    sc.c2 = group_bcast(l[2], last);
     sc.have = true;
     wave_lds_fence();
     // Each iteration reads the next sample's position and value ahead, so
