@@ -59,6 +59,23 @@ __device__ __forceinline__ float ld(const uint16_t* p, int64_t i) {
     return __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
 }
 
+// Raw storage bits of element i (float bits, or bf16 bits in the low half),
+// and their float value: loads behind a predicate are written as raw loads
+// followed by the conversions, so that the conversions (and the waits they
+// need) come after all the loads instead of inside each predicated load.
+__device__ __forceinline__ uint32_t ld_bits(const float* p, int64_t i) {
+    return __float_as_uint(p[i]);
+}
+__device__ __forceinline__ uint32_t ld_bits(const uint16_t* p, int64_t i) {
+    return static_cast<uint32_t>(p[i]);
+}
+template <typename DT>
+__device__ __forceinline__ float bits_value(uint32_t b);
+template <>
+__device__ __forceinline__ float bits_value<float>(uint32_t b) { return __uint_as_float(b); }
+template <>
+__device__ __forceinline__ float bits_value<uint16_t>(uint32_t b) { return __uint_as_float(b << 16); }
+
 // Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
     uint32_t u = __float_as_uint(f);

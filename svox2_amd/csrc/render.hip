@@ -486,11 +486,23 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
 #pragma unroll
         for (int k = 0; k < 8; ++k) c.lk[k] = -1;
     }
+    // Corners without a row read row 0 and are zeroed after: the 8 loads
+    // issue back to back without a branch each (a grid with rows reads in
+    // bounds; one without any has no valid corner and reads nothing).
+    uint32_t db[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    if (g.rows > 0) {
+        bool ok[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ok[k] = eval && g.valid(c.lk[k]);
+            db[k] = ld_bits(g.density, static_cast<int64_t>(ok[k] ? c.lk[k] : 0) * g.dstride);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) db[k] = ok[k] ? db[k] : 0u;
+    }
     float dn[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        dn[k] = (eval && g.valid(c.lk[k])) ? ld(g.density, static_cast<int64_t>(c.lk[k]) * g.dstride)
-                                           : 0.f;
+    for (int k = 0; k < 8; ++k) dn[k] = bits_value<DT>(db[k]);
     c.sigma = trilerp8(dn, c.p);
     c.ws = (o.last_opaque && ti + o.step > r.tmax) ? 1e9f : r.ws;
     c.contrib = eval && c.sigma > o.sigma_thresh;
@@ -533,30 +545,46 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
     c.next_t = next_t;
 }
 
-// Loads the C SH channels of row lk.  Rows that are 16-byte aligned with a
-// stride of whole 16-byte units are read with 16-byte loads (one aligned
-// 128-byte line for a float32 SH-2 row of stride 32, 64 bytes in bf16).
-template <int C>
-__device__ __forceinline__ void load_row(const GridView<float>& g, int lk, float* x) {
+// Projections of row lk on the SH basis: col[c] = sum_j basis[j] * sh[c * B +
+// j], accumulated in j order.  Rows that are 16-byte aligned with a stride of
+// whole 16-byte units are read with 16-byte loads (one aligned 128-byte line
+// for a float32 SH-2 row of stride 32, 64 bytes in bf16), and each loaded
+// vector is consumed as it arrives (bf16 channels are widened one by one
+// into the products), so the row is never held as C floats.
+template <int B>
+__device__ __forceinline__ void project_channel(int ch, float x, const float* basis, float& c0,
+                                                float& c1, float& c2) {
+    const int c = ch / B, j = ch % B;
+    if (c == 0) c0 = fmaf(basis[j], x, c0);
+    else if (c == 1) c1 = fmaf(basis[j], x, c1);
+    else c2 = fmaf(basis[j], x, c2);
+}
+
+template <int B>
+__device__ __forceinline__ void project_row(const GridView<float>& g, int lk, const float* basis,
+                                            float& c0, float& c1, float& c2) {
+    constexpr int C = 3 * B;
     const float* row = g.sh + static_cast<int64_t>(lk) * g.sstride;
     if (g.vec_rows) {
         const float4* r4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(row, 16));
 #pragma unroll
         for (int i = 0; i < (C + 3) / 4; ++i) {
             const float4 v = r4[i];
-            if (4 * i + 0 < C) x[4 * i + 0] = v.x;
-            if (4 * i + 1 < C) x[4 * i + 1] = v.y;
-            if (4 * i + 2 < C) x[4 * i + 2] = v.z;
-            if (4 * i + 3 < C) x[4 * i + 3] = v.w;
+            const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+                if (4 * i + h < C) project_channel<B>(4 * i + h, e[h], basis, c0, c1, c2);
         }
     } else {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) x[ch] = row[ch];
+        for (int ch = 0; ch < C; ++ch) project_channel<B>(ch, row[ch], basis, c0, c1, c2);
     }
 }
 
-template <int C>
-__device__ __forceinline__ void load_row(const GridView<uint16_t>& g, int lk, float* x) {
+template <int B>
+__device__ __forceinline__ void project_row(const GridView<uint16_t>& g, int lk,
+                                            const float* basis, float& c0, float& c1, float& c2) {
+    constexpr int C = 3 * B;
     const uint16_t* row = g.sh + static_cast<int64_t>(lk) * g.sstride;
     if (g.vec_rows) {
         const uint4* r4 = reinterpret_cast<const uint4*>(__builtin_assume_aligned(row, 16));
@@ -566,13 +594,16 @@ __device__ __forceinline__ void load_row(const GridView<uint16_t>& g, int lk, fl
             const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                if (8 * i + 2 * h < C) x[8 * i + 2 * h] = __uint_as_float(w[h] << 16);
-                if (8 * i + 2 * h + 1 < C) x[8 * i + 2 * h + 1] = __uint_as_float(w[h] & 0xffff0000u);
+                if (8 * i + 2 * h < C)
+                    project_channel<B>(8 * i + 2 * h, __uint_as_float(w[h] << 16), basis, c0, c1, c2);
+                if (8 * i + 2 * h + 1 < C)
+                    project_channel<B>(8 * i + 2 * h + 1, __uint_as_float(w[h] & 0xffff0000u), basis,
+                                       c0, c1, c2);
             }
         }
     } else {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) x[ch] = ld(row, ch);
+        for (int ch = 0; ch < C; ++ch) project_channel<B>(ch, ld(row, ch), basis, c0, c1, c2);
     }
 }
 
@@ -623,17 +654,7 @@ __device__ __forceinline__ void chunk_run_colors(const GridView<DT>& g, const Ra
             lk = (k == kk) ? v : lk;
         }
         float c0 = 0.f, c1 = 0.f, c2 = 0.f;
-        if (has && g.valid(lk)) {
-            constexpr int C = 3 * B;
-            float x[C];
-            load_row<C>(g, lk, x);
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                c0 = fmaf(rc.basis[j], x[j], c0);
-                c1 = fmaf(rc.basis[j], x[B + j], c1);
-                c2 = fmaf(rc.basis[j], x[2 * B + j], c2);
-            }
-        }
+        if (has && g.valid(lk)) project_row<B>(g, lk, rc.basis, c0, c1, c2);
         for (int i = 0;; ++i) {
             const bool act = run != 0u;
             if (!__any(act)) break;
