@@ -195,6 +195,12 @@ __device__ __forceinline__ uint32_t group_ballot(bool pred) {
     return static_cast<uint32_t>(m >> (threadIdx.x & 32u));
 }
 
+// Orders this wavefront's LDS writes before its later LDS reads.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum over each aligned group of 8 lanes, in every lane of the group, by
 // data-parallel-primitive moves (VALU, no LDS round trip): lane ^ 1 and
 // lane ^ 2 by quad permutes, then the other quad by the half-row mirror
@@ -353,6 +359,8 @@ struct Chunk {
     bool contrib;     // evaluated and sigma > sigma_thresh
     bool live;        // contributes and lies at or before the stop sample
     uint32_t live_mask;
+    int n_place;      // lanes given a sample to evaluate (0 .. n_place - 1)
+    uint32_t load_mask;  // lanes whose corner links are read (evaluated, cell may hold rows)
     int n_eval;       // evaluated samples (lanes 0 .. n_eval - 1)
     int n_iter;       // march iterations consumed (evaluated + one per skip)
     bool stopped;     // the stop test fired in this chunk
@@ -382,12 +390,20 @@ __device__ __forceinline__ bool sum_chain_exact(float t0, float step) {
     return isfinite(q) && q == floorf(q);
 }
 
+// Where a chunk's placement leaves the march: whether it continues after
+// the chunk and where, and the skips taken (for the iteration counters).
+struct Placement {
+    bool more = false;
+    float next_t = 0.f;
+    int n_skips = 0;   // skips inside the chunk
+    int my_skips = 0;  // skips before this lane's sample
+};
+
 template <typename DT>
-__device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView& o,
-                                            const Ray& r, int lane, float t0, float log_t,
-                                            Chunk& c) {
-    const int sx = g.stride_x, sy = g.stride_y;
-    // Phase 1: place the samples.  Each round gives the lanes from seg_lane on
+__device__ __forceinline__ void place_chunk(const GridView<DT>& g, const OptView& o,
+                                            const Ray& r, int lane, float t0, Chunk& c,
+                                            Placement& pl) {
+    // Place the samples.  Each round gives the lanes from seg_lane on
     // the steps of the segment starting at seg_t, reads their cells' skip
     // levels and finds the segment's end.
     int seg_lane = 0;      // first lane of the current segment
@@ -470,22 +486,49 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
         if (more) next_t = group_lane(ti, 31) + o.step;
         break;
     }
-    const bool eval = lane < n_place;
     c.t = ti;
-    // Phase 2: corner links and densities of the evaluated samples.
-    if (eval && (g.cmap == nullptr || occupied)) {
-        c.lk[0] = g.links[c.base];
-        c.lk[1] = g.links[c.base + 1];
-        c.lk[2] = g.links[c.base + sy];
-        c.lk[3] = g.links[c.base + sy + 1];
-        c.lk[4] = g.links[c.base + sx];
-        c.lk[5] = g.links[c.base + sx + 1];
-        c.lk[6] = g.links[c.base + sx + sy];
-        c.lk[7] = g.links[c.base + sx + sy + 1];
+    c.n_place = n_place;
+    c.load_mask = group_ballot(lane < n_place && (g.cmap == nullptr || occupied));
+    pl.more = more;
+    pl.next_t = next_t;
+    pl.n_skips = n_skips;
+    pl.my_skips = my_skips;
+}
+
+// Corner links of the cell at flat index base for the lanes in load_mask
+// (-1 for the others).
+template <typename DT>
+__device__ __forceinline__ void load_corner_links(const GridView<DT>& g, int lane, int base,
+                                                  uint32_t load_mask, int* lk) {
+    const int sx = g.stride_x, sy = g.stride_y;
+    if ((load_mask >> lane) & 1u) {
+        lk[0] = g.links[base];
+        lk[1] = g.links[base + 1];
+        lk[2] = g.links[base + sy];
+        lk[3] = g.links[base + sy + 1];
+        lk[4] = g.links[base + sx];
+        lk[5] = g.links[base + sx + 1];
+        lk[6] = g.links[base + sx + sy];
+        lk[7] = g.links[base + sx + sy + 1];
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) c.lk[k] = -1;
+        for (int k = 0; k < 8; ++k) lk[k] = -1;
     }
+}
+
+// Evaluates a placed chunk (c.t, c.base, c.p, c.l, c.n_place, c.load_mask):
+// corner links (unless links_ready: c.lk already holds them) and densities,
+// transmittance, stop and weights.
+template <typename DT>
+__device__ __forceinline__ void eval_chunk(const GridView<DT>& g, const OptView& o,
+                                           const Ray& r, int lane, float log_t,
+                                           const Placement& pl, Chunk& c,
+                                           bool links_ready = false) {
+    const int n_place = c.n_place;
+    const bool eval = lane < n_place;
+    const float ti = c.t;
+    // Corner links and densities of the evaluated samples.
+    if (!links_ready) load_corner_links(g, lane, c.base, c.load_mask, c.lk);
     // Corners without a row read row 0 and are zeroed after: the 8 loads
     // issue back to back without a branch each (a grid with rows reads in
     // bounds; one without any has no valid corner and reads nothing).
@@ -532,18 +575,28 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
 
     if (c.stopped) {
         c.n_eval = first_stop + 1;
-        c.n_iter = first_stop + 1 + group_bcast(my_skips, first_stop);
+        c.n_iter = first_stop + 1 + group_bcast(pl.my_skips, first_stop);
         c.more = false;
         c.log_t_end = -1e3f;
         c.next_t = 0.f;
         return;
     }
     c.n_eval = n_place;
-    c.n_iter = n_place + n_skips;
+    c.n_iter = n_place + pl.n_skips;
     c.log_t_end = n_place > 0 ? group_bcast(c.lt_after, n_place - 1) : log_t;
-    c.more = more;
-    c.next_t = next_t;
+    c.more = pl.more;
+    c.next_t = pl.next_t;
 }
+
+template <typename DT>
+__device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView& o,
+                                            const Ray& r, int lane, float t0, float log_t,
+                                            Chunk& c) {
+    Placement pl;
+    place_chunk(g, o, r, lane, t0, c, pl);
+    eval_chunk(g, o, r, lane, log_t, pl, c);
+}
+
 
 // Projections of row lk on the SH basis: col[c] = sum_j basis[j] * sh[c * B +
 // j], accumulated in j order.  Rows that are 16-byte aligned with a stride of
@@ -717,6 +770,20 @@ __device__ __forceinline__ void chunk_colours(const GridView<DT>& g, const RayCt
         });
 }
 
+// Chunk log of the fused call: the forward writes the placement of each of a
+// ray's first kLogChunks chunks (every lane's t, the placed-lane count and the
+// link-load mask), and the backward evaluates those chunks from it instead of
+// placing them again (no cell-map reads or skip distances).  Chunk ci of ray
+// r is rec[(r * kLogChunks + ci) * kLogWords + w]: w = 0..31 the lanes' t,
+// 32 the placed-lane count, 33 the mask.  head[r] = (logged chunks, whether
+// the march continued after the last of them, the t it continued at, 0).
+struct ChunkLog {
+    float* __restrict__ rec;
+    int4* __restrict__ head;
+};
+constexpr int kLogChunks = 8;
+constexpr int kLogWords = 36;
+
 // Record stream of the fused call: the forward appends, per ray, a header and
 // one 96-byte record per contributing sample of each chunk; the backward
 // replays them instead of re-marching.  Slot s is rec[6 s .. 6 s + 5]
@@ -742,13 +809,17 @@ __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int
 
 // Forward march of one ray.  out[3] = composited colour (all lanes),
 // returns the final log transmittance.
-template <typename DT, int B, bool STATS, bool RECORD = false>
+template <typename DT, int B, bool STATS, bool RECORD = false, bool LOG = false>
 __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                float* out, int32_t* stats,
                                                uint32_t* cost = nullptr,
                                                const RecStream* rs = nullptr,
-                                               int64_t ray = 0) {
+                                               int64_t ray = 0,
+                                               const ChunkLog* log = nullptr,
+                                               float* log_lds = nullptr) {
+    bool log_more = false;   // the march continued after the last logged chunk
+    float log_next_t = 0.f;  // and where
     // Record stream position of this ray: next free slot and end of block.
     int64_t cur = RECORD ? ray * rs->fixed : 0;
     int64_t end = RECORD ? cur + rs->fixed : 0;
@@ -767,6 +838,19 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
             Chunk c;
             march_chunk(g, o, r, lane, t, log_t, c);
             const int n_live = __popc(c.live_mask);
+            if (LOG && n_chunks < static_cast<uint32_t>(kLogChunks)) {
+                // Staged in LDS and written out when the ray ends: global
+                // stores here would hold up the next chunk's loads (a load's
+                // wait also waits for every earlier store).
+                float* rec = log_lds + n_chunks * kLogWords;
+                rec[lane] = c.t;
+                if (lane == 0) {
+                    rec[32] = __int_as_float(c.n_place);
+                    rec[33] = __uint_as_float(c.load_mask);
+                }
+                log_more = c.more;
+                log_next_t = c.next_t;
+            }
             ++n_chunks;
             if (RECORD && n_live > 0) {
                 n_live_all += n_live;
@@ -840,6 +924,18 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
         if (rec_ok) put_header(rs->rec, cur, -1, 0);
         rs->state[ray] = rec_ok ? 1 : 2;
     }
+    if (LOG) {
+        const int n_logged = static_cast<int>(min(n_chunks, static_cast<uint32_t>(kLogChunks)));
+        wave_lds_fence();
+        for (int ci = 0; ci < n_logged; ++ci) {
+            const float* src = log_lds + ci * kLogWords;
+            float* dst = log->rec + (ray * kLogChunks + ci) * kLogWords;
+            dst[lane] = src[lane];
+            if (lane < 2) dst[32 + lane] = src[32 + lane];
+        }
+        if (lane == 0)
+            log->head[ray] = make_int4(n_logged, log_more ? 1 : 0, __float_as_int(log_next_t), 0);
+    }
     if (STATS && lane == 0) {
         fnv.mix(__float_as_uint(log_t));
         stats[0] = static_cast<int32_t>(n_iter);
@@ -878,12 +974,6 @@ constexpr int kMoveJump = 0x80;
 struct BwdShared {
     SampleRec rec[kBlock];
 };
-
-// Orders this wavefront's LDS writes before its later LDS reads.
-__device__ __forceinline__ void wave_lds_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Gradient scatter state of one ray.  Lane L of the 32-lane group owns one
 // gradient column of a row: SH channel L (c = L / B, j = L % B) for L < C,
@@ -1129,15 +1219,68 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
 template <typename DT, int B>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
-                                               BwdRay<B>& st, const GradOut& go) {
+                                               BwdRay<B>& st, const GradOut& go,
+                                               const ChunkLog* log = nullptr,
+                                               int64_t ray = 0) {
     if (r.tmin > r.tmax) return;
+    // Chunks the forward logged are evaluated from their placement.
+    int n_logged = 0;
+    bool log_more = false;
+    float log_next_t = 0.f;
+    if (log != nullptr) {
+        const int4 h = log->head[ray];
+        n_logged = min(h.x, kLogChunks);
+        log_more = h.y != 0;
+        log_next_t = __int_as_float(h.z);
+    }
+    // The next logged chunk's placement is read, and its corner links
+    // requested, while the current chunk is evaluated, so a logged chunk
+    // starts with its links on the way.
+    float nt = 0.f;
+    int nplace = 0;
+    uint32_t nmask = 0u;
+    int nlk[8];
+    if (n_logged > 0) {
+        const float* rec = log->rec + ray * kLogChunks * kLogWords;
+        nt = rec[lane];
+        nplace = __float_as_int(rec[32]);
+        nmask = __float_as_uint(rec[33]);
+        float np[3];
+        int nl[3];
+        load_corner_links(g, lane, sample_cell(g, r, nt, np, nl), nmask, nlk);
+    }
     float t = r.tmin;
     float log_t = 0.f;
     int guard = 0;
     bool more = true;
-    while (more && ++guard < kMaxIters) {
+    for (int ci = 0; more && ++guard < kMaxIters; ++ci) {
         Chunk c;
-        march_chunk(g, o, r, lane, t, log_t, c);
+        if (ci < n_logged) {
+            c.t = nt;
+            c.n_place = nplace;
+            c.load_mask = nmask;
+            c.base = sample_cell(g, r, c.t, c.p, c.l);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c.lk[k] = nlk[k];
+            const bool next = ci + 1 < n_logged;
+            if (next) {
+                const float* rec = log->rec + (ray * kLogChunks + ci + 1) * kLogWords;
+                nt = rec[lane];
+                nplace = __float_as_int(rec[32]);
+                nmask = __float_as_uint(rec[33]);
+            }
+            Placement pl;
+            pl.more = next || log_more;
+            pl.next_t = log_next_t;  // read only after the last logged chunk
+            eval_chunk(g, o, r, lane, log_t, pl, c, true);
+            if (next && c.more) {
+                float np[3];
+                int nl[3];
+                load_corner_links(g, lane, sample_cell(g, r, nt, np, nl), nmask, nlk);
+            }
+        } else {
+            march_chunk(g, o, r, lane, t, log_t, c);
+        }
         if (c.live_mask != 0u) {
             float cs[3] = {0.f, 0.f, 0.f};
             chunk_colours<DT, B, true>(g, rc, c, lane, cs, nullptr);
@@ -1200,13 +1343,15 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
 // ---------------------------------------------------------------------------
 
 // Minimum waves per SIMD the march kernels are compiled for (the register
-// budget is 512 / waves VGPRs).  The marches are latency bound: occupancy
-// hides the dependent link -> row loads better than unrolled passes do.
+// budget is 512 / waves VGPRs).  The forward is latency bound: occupancy
+// hides its dependent link -> row loads.  The backward measured fastest at 3
+// waves, whose 170 registers hold the next chunk's prefetched placement and
+// links without spilling.
 #ifndef SVOX2_FWD_WAVES
 #define SVOX2_FWD_WAVES 5
 #endif
 #ifndef SVOX2_BWD_WAVES
-#define SVOX2_BWD_WAVES 4
+#define SVOX2_BWD_WAVES 3
 #endif
 #define SVOX2_FWD_KERNEL __global__ __launch_bounds__(kBlock) \
     __attribute__((amdgpu_waves_per_eu(SVOX2_FWD_WAVES)))
@@ -1255,13 +1400,13 @@ SVOX2_REC_KERNEL void k_render_rays_rec(
     if (cost_out != nullptr && lane == 0) cost_out[ray] = cost;
 }
 
-template <typename DT, int B, bool STATS, bool RECORD = false>
+template <typename DT, int B, bool STATS, bool LOG = false>
 SVOX2_FWD_KERNEL void k_render_rays(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         float* __restrict__ rgb_out, float* __restrict__ log_t_out,
         int32_t* __restrict__ stats, uint32_t* __restrict__ cost_out, RecStream rs,
-        const uint32_t* __restrict__ order) {
+        const uint32_t* __restrict__ order, ChunkLog log) {
     const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (slot >= n) return;
@@ -1273,10 +1418,12 @@ SVOX2_FWD_KERNEL void k_render_rays(
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float out[3];
     uint32_t cost = 0;
-    const float log_t = march_forward<DT, B, STATS, RECORD>(g, o, r, rc, lane, out,
-                                                            STATS ? stats + 4 * ray : nullptr,
-                                                            cost_out != nullptr ? &cost : nullptr,
-                                                            &rs, ray);
+    // LDS staging of this group's chunk log (used when LOG).
+    __shared__ float log_lds[LOG ? (kBlock / kGroup) * kLogChunks * kLogWords : 1];
+    const float log_t = march_forward<DT, B, STATS, false, LOG>(
+        g, o, r, rc, lane, out, STATS ? stats + 4 * ray : nullptr,
+        cost_out != nullptr ? &cost : nullptr, &rs, ray, &log,
+        log_lds + (threadIdx.x / kGroup) * kLogChunks * kLogWords);
     if (!STATS) {
         if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
         if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
@@ -1437,7 +1584,7 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
         const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
-        const uint32_t* __restrict__ order, GradOut go, RecStream rs) {
+        const uint32_t* __restrict__ order, GradOut go, RecStream rs, ChunkLog log) {
     const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     if (slot >= n) return;
@@ -1461,7 +1608,7 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
     }
     __shared__ BwdShared sh;
     BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
-    march_backward<DT, B>(g, o, r, rc, lane, st, go);
+    march_backward<DT, B>(g, o, r, rc, lane, st, go, log.rec != nullptr ? &log : nullptr, ray);
 }
 
 // Counting sort of the rays by descending cost: histogram, scan, scatter.
@@ -1585,6 +1732,24 @@ size_t fused_workspace_bytes(int64_t n) {
     const int64_t counter = 256;
     const int64_t recs = n * (kRecFixed + kRecOverflow) * kRecSlotBytes;
     return static_cast<size_t>(order + state + counter + recs);
+}
+
+// Fused-call workspace with the chunk log (batches that re-march): the
+// ordering workspace, then the log records (n * kLogChunks * kLogWords
+// words) and the per-ray heads (n int4).
+size_t logged_workspace_bytes(int64_t n) {
+    const int64_t order = static_cast<int64_t>(order_workspace_bytes(n));
+    const int64_t recs = align256(n * kLogChunks * kLogWords * 4);
+    const int64_t heads = align256(16 * n);
+    return static_cast<size_t>(order + recs + heads);
+}
+
+ChunkLog chunk_log(void* ws, int64_t n) {
+    char* p = static_cast<char*>(ws) + order_workspace_bytes(n);
+    ChunkLog log;
+    log.rec = reinterpret_cast<float*>(p);
+    log.head = reinterpret_cast<int4*>(p + align256(n * kLogChunks * kLogWords * 4));
+    return log;
 }
 
 RecStream record_stream(void* ws, int64_t n) {
@@ -1731,7 +1896,7 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
         hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
                            log_transmit_out, (int32_t*)nullptr, ray_cost_out, RecStream{},
-                           (const uint32_t*)nullptr);
+                           (const uint32_t*)nullptr, ChunkLog{});
     });
     if (st) return st;
     return check_launch("volume_render_cuvol");
@@ -1755,7 +1920,7 @@ extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const f
         hipLaunchKernelGGL((k_render_rays<DT, B, true>), dim3(blocks), dim3(kBlock), 0,
                            as_stream(stream), g, o, origins, dirs, ray_clip, n_rays,
                            (float*)nullptr, (float*)nullptr, stats_out, (uint32_t*)nullptr,
-                           RecStream{}, (const uint32_t*)nullptr);
+                           RecStream{}, (const uint32_t*)nullptr, ChunkLog{});
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_stats");
@@ -1882,7 +2047,7 @@ bool fused_records(int64_t n) {
 
 extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
     if (n_rays <= 0) return 0;
-    return fused_records(n_rays) ? fused_workspace_bytes(n_rays) : order_workspace_bytes(n_rays);
+    return fused_records(n_rays) ? fused_workspace_bytes(n_rays) : logged_workspace_bytes(n_rays);
 }
 
 extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
@@ -1914,6 +2079,12 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     OrderWorkspace w;
     if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
     RecStream rs{};
+    ChunkLog log{};
+    // Batches that re-march log the forward's chunk placements when the
+    // workspace has room for them.
+    if (workspace != nullptr && !fused_records(n_rays) &&
+        workspace_bytes >= logged_workspace_bytes(n_rays))
+        log = chunk_log(workspace, n_rays);
     if (workspace != nullptr && fused_records(n_rays) &&
         workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
@@ -1941,10 +2112,14 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
             hipLaunchKernelGGL((k_render_rays_rec<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
                                o, origins, dirs, ray_clip, n_rays, rgb_out, lt, w.cost, rs,
                                order);
+        else if (log.rec != nullptr)
+            hipLaunchKernelGGL((k_render_rays<DT, B, false, true>), dim3(blocks), dim3(kBlock), 0,
+                               s, g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
+                               (int32_t*)nullptr, w.cost, rs, order, log);
         else
             hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
                                g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
-                               (int32_t*)nullptr, w.cost, rs, order);
+                               (int32_t*)nullptr, w.cost, rs, order, log);
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s);
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
@@ -1952,7 +2127,8 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                norm, static_cast<const uint32_t*>(w.order), go, rs);
         hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
                            origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, lt, beta,
-                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs);
+                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs,
+                           log);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");

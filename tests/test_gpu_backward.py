@@ -201,6 +201,16 @@ def test_fused_ray_order(gpu, dense64):
 
     rgb_a, gd_a, gs_a = run(ws)
     rgb_b, gd_b, gs_b = run(None)
+    # The workspace the call asks for at this size adds the chunk log: the
+    # backward evaluates the forward's logged chunk placements.
+    nbytes_log = lib.svox2_volume_render_cuvol_fused_workspace_bytes(n)
+    assert nbytes_log > nbytes
+    ws_log = torch.zeros(nbytes_log // 4, dtype=torch.int32, device=gpu)
+    nbytes, nbytes_ord = nbytes_log, nbytes
+    rgb_c, gd_c, gs_c = run(ws_log)
+    nbytes = nbytes_ord
+    assert np.array_equal(rgb_c, rgb_b)
+    assert _rel(gd_c, gd_b) < 1e-5 and _rel(gs_c, gs_b) < 1e-5
     w = ws.cpu().numpy().view(np.uint32)
     hist, cost, order = w[:1024], w[2048:2048 + n], w[2048 + n:2048 + 2 * n]
     assert int(hist.sum()) == n
