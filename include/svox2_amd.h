@@ -116,6 +116,20 @@ int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
                               float* rgb_out, float* log_transmit_out,
                               uint32_t* ray_cost_out, void* stream);
 
+/* svox2_volume_render_cuvol with an optional device workspace of
+ * svox2_ray_order_workspace_bytes(n_rays) bytes: batches of at least four
+ * times the rays the device holds resident (40960 on a 256-CU MI355X) are
+ * processed in descending order of each ray's segment length in the grid
+ * (same results); smaller batches, or a NULL workspace, in input order.
+ * Replaces volume_render_cuvol (svox2.cpp:28; render_lerp_kernel_cuvol.cu:918)
+ * for callers that can lend the workspace (the autograd forward). */
+int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const float* origins,
+                                      const float* dirs, const float* ray_clip,
+                                      int64_t n_rays, const svox2_render_options_t* opt,
+                                      float* rgb_out, float* log_transmit_out,
+                                      uint32_t* ray_cost_out, void* workspace,
+                                      size_t workspace_bytes, void* stream);
+
 /* Forward render of a pinhole image; rgb_out [height][width][3].
  * Replaces volume_render_cuvol_image (svox2.cpp:31; render_lerp_kernel_cuvol.cu:960). */
 int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
@@ -158,7 +172,10 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  *   for batches below that threshold the forward writes every contributing
  *   sample (position, corner links, colour, transmittance) to a record
  *   stream in the workspace and the backward replays it instead of
- *   re-marching the ray (a ray whose records do not fit is re-marched).
+ *   re-marching the ray (a ray whose records do not fit is re-marched);
+ *   for batches at or above it the forward also logs where it placed the
+ *   samples of each ray's first 8 chunks, and the backward evaluates those
+ *   chunks from the log instead of placing them again.
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
@@ -179,7 +196,8 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 /* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
  * fastest for an n_rays batch (0 for an empty batch): the record stream
  * (about 4.6 KiB per ray) below the ordering threshold, the ray-ordering
- * workspace (8 bytes per ray) above it.  No counterpart in the reference. */
+ * workspace plus the chunk log (about 1.2 KiB per ray) at or above it.  No
+ * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
 /* One tile of a multi-request render launch: n_pixels consecutive pixels

@@ -40,7 +40,8 @@ INDEXER_ALL, INDEXER_MASK, INDEXER_INDEX = 0, 1, 2
 # Every function the C header declares (checked against include/svox2_amd.h by
 # the tests).
 EXPORTED_SYMBOLS = (
-    "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_image",
+    "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_ordered",
+    "svox2_volume_render_cuvol_image",
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
     "svox2_ray_order_workspace_bytes", "svox2_volume_render_cuvol_fused_workspace_bytes",
     "svox2_cell_map_bytes",
@@ -134,6 +135,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
                     ctypes.POINTER(_Grads))
     sig = {
         "svox2_volume_render_cuvol": [G, vp, vp, vp, i64, O, vp, vp, vp, vp],
+        "svox2_volume_render_cuvol_ordered": [G, vp, vp, vp, i64, O, vp, vp, vp, vp,
+                                              ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_image": [G, Cm, O, vp, vp, vp],
         "svox2_volume_render_cuvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp, vp,
                                                ctypes.c_size_t, vp],
@@ -472,11 +475,14 @@ def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
 # ---------------------------------------------------------------------------
 def volume_render_cuvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
                         log_transmit_out: Optional[torch.Tensor] = None,
-                        ray_cost_out: Optional[torch.Tensor] = None) -> torch.Tensor:
+                        ray_cost_out: Optional[torch.Tensor] = None,
+                        ordered: bool = False) -> torch.Tensor:
     """volume_render_cuvol (render_lerp_kernel_cuvol.cu:918-958): (n, 3) colours.
 
     ray_cost_out: optional (n,) int32 tensor receiving each ray's march chunk
     count, for ``volume_render_cuvol_backward(..., ray_cost=...)``.
+    ordered: large batches run in descending order of segment length, with a
+    workspace from torch's caching allocator (same results).
     """
     g = _grid_struct(grid)
     o, d, clip = _rays(rays)
@@ -486,10 +492,12 @@ def volume_render_cuvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions
             _check(t, name, (torch.float32,) if t is log_transmit_out else (torch.int32,))
             if t.numel() != o.size(0):
                 raise RuntimeError(f"{name} must have one value per ray")
+    nbytes = int(load_library().svox2_ray_order_workspace_bytes(o.size(0))) if ordered else 0
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
-        _call("svox2_volume_render_cuvol", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
-              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _ptr(log_transmit_out),
-              _ptr(ray_cost_out), _stream(o))
+        _call("svox2_volume_render_cuvol_ordered", ctypes.byref(g), _ptr(o), _ptr(d),
+              _ptr(clip), o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out),
+              _ptr(log_transmit_out), _ptr(ray_cost_out), _ptr(ws), nbytes, _stream(o))
     return out
 
 

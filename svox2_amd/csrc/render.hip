@@ -1216,7 +1216,7 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
 
 // Backward of one ray by re-marching it: every chunk is rematerialised
 // exactly as in the forward, its colours gathered again.
-template <typename DT, int B>
+template <typename DT, int B, bool PREFETCH = true>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                BwdRay<B>& st, const GradOut& go,
@@ -1233,14 +1233,15 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
         log_more = h.y != 0;
         log_next_t = __int_as_float(h.z);
     }
-    // The next logged chunk's placement is read, and its corner links
-    // requested, while the current chunk is evaluated, so a logged chunk
-    // starts with its links on the way.
+    // With PREFETCH the next logged chunk's placement is read, and its corner
+    // links requested, while the current chunk is evaluated, so a logged
+    // chunk starts with its links on the way (it needs the registers of a
+    // 3-wave build).
     float nt = 0.f;
     int nplace = 0;
     uint32_t nmask = 0u;
     int nlk[8];
-    if (n_logged > 0) {
+    if (PREFETCH && n_logged > 0) {
         const float* rec = log->rec + ray * kLogChunks * kLogWords;
         nt = rec[lane];
         nplace = __float_as_int(rec[32]);
@@ -1255,7 +1256,17 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     bool more = true;
     for (int ci = 0; more && ++guard < kMaxIters; ++ci) {
         Chunk c;
-        if (ci < n_logged) {
+        if (ci < n_logged && !PREFETCH) {
+            const float* rec = log->rec + (ray * kLogChunks + ci) * kLogWords;
+            c.t = rec[lane];
+            c.n_place = __float_as_int(rec[32]);
+            c.load_mask = __float_as_uint(rec[33]);
+            c.base = sample_cell(g, r, c.t, c.p, c.l);
+            Placement pl;
+            pl.more = ci + 1 < n_logged || log_more;
+            pl.next_t = log_next_t;  // read only after the last logged chunk
+            eval_chunk(g, o, r, lane, log_t, pl, c);
+        } else if (ci < n_logged) {
             c.t = nt;
             c.n_place = nplace;
             c.load_mask = nmask;
@@ -1344,14 +1355,19 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
 
 // Minimum waves per SIMD the march kernels are compiled for (the register
 // budget is 512 / waves VGPRs).  The forward is latency bound: occupancy
-// hides its dependent link -> row loads.  The backward measured fastest at 3
-// waves, whose 170 registers hold the next chunk's prefetched placement and
-// links without spilling.
+// hides its dependent link -> row loads.  The fused backward is compiled
+// twice: at 3 waves, whose 170 registers hold the next chunk's prefetched
+// placement and links without spilling (fastest while a batch fills the
+// device a few times over), and at 4 waves for larger batches, where
+// throughput counts more than each ray's latency.
 #ifndef SVOX2_FWD_WAVES
 #define SVOX2_FWD_WAVES 5
 #endif
 #ifndef SVOX2_BWD_WAVES
-#define SVOX2_BWD_WAVES 3
+#define SVOX2_BWD_WAVES 4
+#endif
+#ifndef SVOX2_BWD_WAVES_FEW
+#define SVOX2_BWD_WAVES_FEW 3
 #endif
 #define SVOX2_FWD_KERNEL __global__ __launch_bounds__(kBlock) \
     __attribute__((amdgpu_waves_per_eu(SVOX2_FWD_WAVES)))
@@ -1578,8 +1594,9 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
 // forward colours (render_ray_backward_kernel with grad_out_is_rgb,
 // render_lerp_kernel_cuvol.cu:735-741), plus the beta term from the forward's
 // final log transmittance (:257-262) and the Cauchy sparsity term (:345-348).
-template <typename DT, int B>
-SVOX2_BWD_KERNEL void k_render_backward_mse(
+template <typename DT, int B, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
+void k_render_backward_mse(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
         const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
@@ -1608,7 +1625,8 @@ SVOX2_BWD_KERNEL void k_render_backward_mse(
     }
     __shared__ BwdShared sh;
     BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
-    march_backward<DT, B>(g, o, r, rc, lane, st, go, log.rec != nullptr ? &log : nullptr, ray);
+    march_backward<DT, B, (WAVES <= 3)>(g, o, r, rc, lane, st, go,
+                                         log.rec != nullptr ? &log : nullptr, ray);
 }
 
 // Counting sort of the rays by descending cost: histogram, scan, scatter.
@@ -1818,6 +1836,20 @@ void launch_cost_order(const uint32_t* cost, const OrderWorkspace& w, int64_t n,
                        cost, n, w.offsets, w.order);
 }
 
+// Orders the rays of a forward by descending segment length (keys into
+// w.cost, then the counting sort into w.order).
+void launch_length_order(const svox2_grid_t* grid, const OptView& o, const float* origins,
+                         const float* dirs, const float* clip, int64_t n,
+                         const OrderWorkspace& w, hipStream_t s) {
+    const double diag = sqrt(static_cast<double>(grid->size[0]) * grid->size[0] +
+                             static_cast<double>(grid->size[1]) * grid->size[1] +
+                             static_cast<double>(grid->size[2]) * grid->size[2]);
+    hipLaunchKernelGGL(k_ray_length_key, dim3(blocks_for(n, kSortBlock)), dim3(kSortBlock), 0, s,
+                       make_view<float>(*grid), o, origins, dirs, clip, n,
+                       static_cast<float>((kCostBins - 1) / diag), w.cost);
+    launch_cost_order(w.cost, w, n, s);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1877,11 +1909,12 @@ int dispatch(const svox2_grid_t* grid, Fn&& fn) {
 
 using namespace svox2amd;
 
-extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
-                                         const float* dirs, const float* ray_clip,
-                                         int64_t n_rays, const svox2_render_options_t* opt,
-                                         float* rgb_out, float* log_transmit_out,
-                                         uint32_t* ray_cost_out, void* stream) {
+// Forward render of a ray batch; with an ordering workspace, batches at or
+// above the ordering threshold run in descending order of segment length.
+int render_rays(const svox2_grid_t* grid, const float* origins, const float* dirs,
+                const float* ray_clip, int64_t n_rays, const svox2_render_options_t* opt,
+                float* rgb_out, float* log_transmit_out, uint32_t* ray_cost_out,
+                void* workspace, size_t workspace_bytes, void* stream, const char* what) {
     int st = check_grid(grid, true);
     if (st) return st;
     OptView o;
@@ -1889,17 +1922,45 @@ extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* 
     if ((st = check_rays(origins, dirs, n_rays))) return st;
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(rgb_out != nullptr, "rgb_out is NULL");
+    if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    const hipStream_t s = as_stream(stream);
+    OrderWorkspace w;
+    if (workspace != nullptr && n_rays >= order_min_rays()) {
+        w = order_workspace(workspace, n_rays);
+        launch_length_order(grid, o, origins, dirs, ray_clip, n_rays, w, s);
+    }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
-        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0,
-                           as_stream(stream), g, o, origins, dirs, ray_clip, n_rays, rgb_out,
-                           log_transmit_out, (int32_t*)nullptr, ray_cost_out, RecStream{},
-                           (const uint32_t*)nullptr, ChunkLog{});
+        hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s, g, o,
+                           origins, dirs, ray_clip, n_rays, rgb_out, log_transmit_out,
+                           (int32_t*)nullptr, ray_cost_out, RecStream{},
+                           static_cast<const uint32_t*>(w.order), ChunkLog{});
     });
     if (st) return st;
-    return check_launch("volume_render_cuvol");
+    return check_launch(what);
+}
+
+extern "C" int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
+                                         const float* dirs, const float* ray_clip,
+                                         int64_t n_rays, const svox2_render_options_t* opt,
+                                         float* rgb_out, float* log_transmit_out,
+                                         uint32_t* ray_cost_out, void* stream) {
+    return render_rays(grid, origins, dirs, ray_clip, n_rays, opt, rgb_out, log_transmit_out,
+                       ray_cost_out, nullptr, 0, stream, "volume_render_cuvol");
+}
+
+extern "C" int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const float* origins,
+                                                 const float* dirs, const float* ray_clip,
+                                                 int64_t n_rays,
+                                                 const svox2_render_options_t* opt,
+                                                 float* rgb_out, float* log_transmit_out,
+                                                 uint32_t* ray_cost_out, void* workspace,
+                                                 size_t workspace_bytes, void* stream) {
+    return render_rays(grid, origins, dirs, ray_clip, n_rays, opt, rgb_out, log_transmit_out,
+                       ray_cost_out, workspace, workspace_bytes, stream,
+                       "volume_render_cuvol_ordered");
 }
 
 extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const float* origins,
@@ -2097,16 +2158,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         // Large batches: the forward runs in order of segment length, then
         // the backward in order of the forward's work counts (both sorts go
         // through w.cost / w.order; the forward rewrites w.cost by ray).
-        if (w.order != nullptr) {
-            const double diag = sqrt(static_cast<double>(grid->size[0]) * grid->size[0] +
-                                     static_cast<double>(grid->size[1]) * grid->size[1] +
-                                     static_cast<double>(grid->size[2]) * grid->size[2]);
-            hipLaunchKernelGGL(k_ray_length_key, dim3(blocks_for(n_rays, kSortBlock)),
-                               dim3(kSortBlock), 0, s, make_view<float>(*grid), o, origins, dirs,
-                               ray_clip, n_rays, static_cast<float>((kCostBins - 1) / diag),
-                               w.cost);
-            launch_cost_order(w.cost, w, n_rays, s);
-        }
+        if (w.order != nullptr) launch_length_order(grid, o, origins, dirs, ray_clip, n_rays, w, s);
         const uint32_t* order = w.order;
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_rays_rec<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
@@ -2125,10 +2177,17 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
             hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
                                s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta, sparsity_loss,
                                norm, static_cast<const uint32_t*>(w.order), go, rs);
-        hipLaunchKernelGGL((k_render_backward_mse<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
-                           origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, lt, beta,
-                           sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs,
-                           log);
+        // Batches of up to 4 device fills use the 3-wave build.
+        if (n_rays <= 4 * order_min_rays())
+            hipLaunchKernelGGL((k_render_backward_mse<DT, B, SVOX2_BWD_WAVES_FEW>), dim3(blocks),
+                               dim3(kBlock), 0, s, g, o, origins, dirs, ray_clip, n_rays, rgb_gt,
+                               rgb_out, lt, beta, sparsity_loss, norm,
+                               static_cast<const uint32_t*>(w.order), go, rs, log);
+        else
+            hipLaunchKernelGGL((k_render_backward_mse<DT, B, SVOX2_BWD_WAVES>), dim3(blocks),
+                               dim3(kBlock), 0, s, g, o, origins, dirs, ray_clip, n_rays, rgb_gt,
+                               rgb_out, lt, beta, sparsity_loss, norm,
+                               static_cast<const uint32_t*>(w.order), go, rs, log);
     });
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");

@@ -128,3 +128,27 @@ def test_512_chunk_log(gpu, big):
           f"{rel(gd_a, gd_b):.2e} / {rel(gs_a, gs_b):.2e}")
     assert rel(gd_a, gd_b) < 1e-5 and rel(gs_a, gs_b) < 1e-5
     assert np.array_equal(m_a, m_b)
+
+
+def test_512_ordered_forward(gpu, big):
+    """The forward with an ordering workspace (svox2_volume_render_cuvol_ordered,
+    used by the autograd forward) on a full 65,536-ray batch, with per-ray
+    clips: rays run in descending order of segment length, and every ray's
+    colour, final log transmittance and chunk count equal those of the
+    input-order forward, bit for bit."""
+    sc, grid, links, o, d, gt = big
+    n = 65536
+    gen = torch.Generator().manual_seed(11)
+    clip = torch.stack([torch.rand(n, generator=gen) * 0.5, 2 + 2 * torch.rand(n, generator=gen)], 1)
+    rays = svox2_amd.Rays(o.to(gpu), d.to(gpu), clip=clip.to(gpu))
+    out = {}
+    for ordered in (False, True):
+        lt = torch.empty(n, device=gpu)
+        cost = torch.empty(n, dtype=torch.int32, device=gpu)
+        rgb = _C.volume_render_cuvol(grid._to_cpp(), rays._to_cpp(), grid.opt._to_cpp(),
+                                     log_transmit_out=lt, ray_cost_out=cost, ordered=ordered)
+        torch.cuda.synchronize()
+        out[ordered] = (rgb.cpu().numpy(), lt.cpu().numpy(), cost.cpu().numpy())
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(a, b)
+    assert out[True][2].max() > out[True][2].min()
