@@ -1220,15 +1220,15 @@ template <typename DT, int B, bool PREFETCH = true>
 __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                BwdRay<B>& st, const GradOut& go,
-                                               const ChunkLog* log = nullptr,
+                                               ChunkLog log_v = ChunkLog{},
                                                int64_t ray = 0) {
     if (r.tmin > r.tmax) return;
     // Chunks the forward logged are evaluated from their placement.
     int n_logged = 0;
     bool log_more = false;
     float log_next_t = 0.f;
-    if (log != nullptr) {
-        const int4 h = log->head[ray];
+    if (log_v.rec != nullptr) {
+        const int4 h = log_v.head[ray];
         n_logged = min(h.x, kLogChunks);
         log_more = h.y != 0;
         log_next_t = __int_as_float(h.z);
@@ -1242,7 +1242,7 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     uint32_t nmask = 0u;
     int nlk[8];
     if (PREFETCH && n_logged > 0) {
-        const float* rec = log->rec + ray * kLogChunks * kLogWords;
+        const float* rec = log_v.rec + ray * kLogChunks * kLogWords;
         nt = rec[lane];
         nplace = __float_as_int(rec[32]);
         nmask = __float_as_uint(rec[33]);
@@ -1257,7 +1257,7 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
     for (int ci = 0; more && ++guard < kMaxIters; ++ci) {
         Chunk c;
         if (ci < n_logged && !PREFETCH) {
-            const float* rec = log->rec + (ray * kLogChunks + ci) * kLogWords;
+            const float* rec = log_v.rec + (ray * kLogChunks + ci) * kLogWords;
             c.t = rec[lane];
             c.n_place = __float_as_int(rec[32]);
             c.load_mask = __float_as_uint(rec[33]);
@@ -1275,7 +1275,7 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
             for (int k = 0; k < 8; ++k) c.lk[k] = nlk[k];
             const bool next = ci + 1 < n_logged;
             if (next) {
-                const float* rec = log->rec + (ray * kLogChunks + ci + 1) * kLogWords;
+                const float* rec = log_v.rec + (ray * kLogChunks + ci + 1) * kLogWords;
                 nt = rec[lane];
                 nplace = __float_as_int(rec[32]);
                 nmask = __float_as_uint(rec[33]);
@@ -1625,8 +1625,7 @@ void k_render_backward_mse(
     }
     __shared__ BwdShared sh;
     BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
-    march_backward<DT, B, (WAVES <= 3)>(g, o, r, rc, lane, st, go,
-                                         log.rec != nullptr ? &log : nullptr, ray);
+    march_backward<DT, B, (WAVES <= 3)>(g, o, r, rc, lane, st, go, log, ray);
 }
 
 // Counting sort of the rays by descending cost: histogram, scan, scatter.
