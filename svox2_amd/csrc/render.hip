@@ -1663,6 +1663,12 @@ __global__ __launch_bounds__(kCostBins) void k_cost_scan(const uint32_t* __restr
     offsets[b] = a[t] - hist[b];  // exclusive
 }
 
+// Each workgroup places kScatterRays consecutive rays: it counts them per bin
+// in LDS, reserves its range of every bin with one global atomic, then ranks
+// them.  Few workgroups per batch keep the reservations on a popular bin (rays
+// that miss the grid share bin 0) from queueing on one address.
+constexpr int64_t kScatterRays = 4096;
+
 __global__ __launch_bounds__(kSortBlock) void k_cost_scatter(const uint32_t* __restrict__ cost,
                                                              int64_t n,
                                                              uint32_t* __restrict__ offsets,
@@ -1670,17 +1676,15 @@ __global__ __launch_bounds__(kSortBlock) void k_cost_scatter(const uint32_t* __r
     __shared__ uint32_t h[kCostBins];
     for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) h[b] = 0u;
     __syncthreads();
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
-    uint32_t bin = 0, rank = 0;
-    if (i < n) {
-        bin = cost[i];
-        rank = atomicAdd(&h[bin], 1u);
-    }
+    const int64_t begin = static_cast<int64_t>(blockIdx.x) * kScatterRays;
+    const int64_t end = min(n, begin + kScatterRays);
+    for (int64_t i = begin + threadIdx.x; i < end; i += kSortBlock) atomicAdd(&h[cost[i]], 1u);
     __syncthreads();
     for (int b = threadIdx.x; b < kCostBins; b += kSortBlock)
         if (h[b] != 0u) h[b] = atomicAdd(&offsets[b], h[b]);
     __syncthreads();
-    if (i < n) order[h[bin] + rank] = static_cast<uint32_t>(i);
+    for (int64_t i = begin + threadIdx.x; i < end; i += kSortBlock)
+        order[atomicAdd(&h[cost[i]], 1u)] = static_cast<uint32_t>(i);
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_zero_words(uint32_t* __restrict__ p, int n) {
@@ -1826,13 +1830,13 @@ int64_t order_min_rays() {
 // Sorts the rays by the costs the forward wrote (w.cost) into w.order.
 void launch_cost_order(const uint32_t* cost, const OrderWorkspace& w, int64_t n,
                        hipStream_t s) {
-    const unsigned hb = static_cast<unsigned>(std::min<int64_t>(blocks_for(n, kSortBlock), 1024));
+    const unsigned hb = static_cast<unsigned>((n + kScatterRays - 1) / kScatterRays);
     hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)), dim3(kSortBlock), 0,
                        s, w.hist, kCostBins);
     hipLaunchKernelGGL(k_cost_hist, dim3(hb), dim3(kSortBlock), 0, s, cost, n, w.hist);
     hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(kCostBins), 0, s, w.hist, w.offsets);
-    hipLaunchKernelGGL(k_cost_scatter, dim3(blocks_for(n, kSortBlock)), dim3(kSortBlock), 0, s,
-                       cost, n, w.offsets, w.order);
+    hipLaunchKernelGGL(k_cost_scatter, dim3(static_cast<unsigned>((n + kScatterRays - 1) / kScatterRays)),
+                       dim3(kSortBlock), 0, s, cost, n, w.offsets, w.order);
 }
 
 // Orders the rays of a forward by descending segment length (keys into
