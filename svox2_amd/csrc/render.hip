@@ -35,6 +35,7 @@
 
 #include <math.h>
 #include <limits.h>
+#include <string.h>
 #include <type_traits>
 
 namespace svox2amd {
@@ -50,6 +51,7 @@ constexpr int kMaxIters = 1 << 22;  // hard per-ray iteration cap (termination g
 struct OptView {
     float bg;
     float step;
+    int step_ulp_max;  // largest float exponent field e with step a whole multiple of 2^(e - 150)
     float sigma_thresh;
     float log_stop;  // stop when log transmittance < log_stop
     float near_clip;
@@ -61,6 +63,7 @@ struct Ray {
     float o[3];    // grid-space origin
     float d[3];    // unit grid-space direction
     float inv[3];  // (float)(1.0 / (double)d[i])
+    float hi[3];   // grid size - 1 per axis (the position clamp)
     float ws;      // world_step: world distance per step
     float tmin, tmax;
 };
@@ -97,6 +100,7 @@ __device__ __forceinline__ void setup_ray(const GridView<DT>& g, const OptView& 
     for (int i = 0; i < 3; ++i) {
         r.o[i] = fmaf(wo[i], g.scaling[i], g.offset[i]);
         d[i] = wd[i] * g.scaling[i];
+        r.hi[i] = static_cast<float>(g.size[i]) - 1.f;
     }
     const float delta_scale = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 #pragma unroll
@@ -315,7 +319,7 @@ __device__ __forceinline__ int sample_cell(const GridView<DT>& g, const Ray& r, 
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float x = fmaf(t, r.d[i], r.o[i]);
-        x = fminf(fmaxf(x, 0.f), static_cast<float>(g.size[i]) - 1.f);
+        x = fminf(fmaxf(x, 0.f), r.hi[i]);
         l[i] = min(static_cast<int>(x), g.size[i] - 2);
         p[i] = x - static_cast<float>(l[i]);
     }
@@ -375,19 +379,15 @@ constexpr int kMaxSegments = 16;
 
 // True when t0, t0 + step, ..., t0 + 31 step are all exactly representable in
 // the binade of t0: then the running sum t += step never rounds.  t0 must be
-// a positive normal float; the binade test uses the rounded end point, which
-// reaches the next power of two whenever the exact one does (rounding is
-// monotone and powers of two are representable), so it is conservative.
-__device__ __forceinline__ bool sum_chain_exact(float t0, float step) {
-    const uint32_t e = (__float_as_uint(t0) >> 23) & 0xFFu;
-    if (!(t0 > 0.f) || e == 0u || e >= 254u || !(step > 0.f)) return false;
-    const float t_end = fmaf(31.f, step, t0);
-    if (((__float_as_uint(t_end) >> 23) & 0xFFu) != e) return false;
-    // step / ulp(t0), ulp(t0) = 2^(e - 150): an exact power-of-two scaling.
-    const int sh = 150 - static_cast<int>(e);
-    if (sh > 100 || sh < -100) return false;
-    const float q = ldexpf(step, sh);
-    return isfinite(q) && q == floorf(q);
+// a positive normal float (exponent field e in 1..253; a sign bit, zero,
+// infinity or NaN fails that), step a whole multiple of the binade's ulp
+// 2^(e - 150) (e <= step_ulp_max), and the rounded end point in the same
+// binade (it reaches the next power of two whenever the exact one does:
+// rounding is monotone and powers of two are representable).
+__device__ __forceinline__ bool sum_chain_exact(float t0, float step, int step_ulp_max) {
+    const uint32_t e = __float_as_uint(t0) >> 23;
+    const uint32_t e_end = __float_as_uint(fmaf(31.f, step, t0)) >> 23;
+    return (e - 1u) < 253u && static_cast<int>(e) <= step_ulp_max && e_end == e;
 }
 
 // Where a chunk's placement leaves the march: whether it continues after
@@ -425,7 +425,7 @@ __device__ __forceinline__ void place_chunk(const GridView<DT>& g, const OptView
             // so seg_t + k * step (one fma, exact) is the same float;
             // otherwise the chain is replayed.
             const int k = lane - seg_lane;
-            if (sum_chain_exact(seg_t, o.step)) {
+            if (sum_chain_exact(seg_t, o.step, o.step_ulp_max)) {
                 ti = fmaf(static_cast<float>(k), o.step, seg_t);
             } else {
                 ti = seg_t;
@@ -1873,6 +1873,14 @@ int make_opt(const svox2_render_options_t* opt, OptView& o) {
                   "step_size must be in [1e-3, 1e3] voxels, got %g", (double)opt->step_size);
     o.bg = opt->background_brightness;
     o.step = opt->step_size;
+    {
+        // step = m * 2^(e_step - 150) with a 24-bit m: a whole multiple of
+        // 2^(e - 150) exactly when e <= e_step + (trailing zeros of m).
+        uint32_t bits;
+        memcpy(&bits, &o.step, 4);
+        const uint32_t m = (bits & 0x7FFFFFu) | 0x800000u;
+        o.step_ulp_max = static_cast<int>((bits >> 23) & 0xFFu) + __builtin_ctz(m);
+    }
     o.sigma_thresh = opt->sigma_thresh;
     o.log_stop = log_stop_threshold(opt->stop_thresh);
     o.near_clip = opt->near_clip;
