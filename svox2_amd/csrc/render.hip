@@ -1628,15 +1628,24 @@ void k_render_backward_mse(
     march_backward<DT, B, (WAVES <= 3)>(g, o, r, rc, lane, st, go, log, ray);
 }
 
-// Counting sort of the rays by descending cost: histogram, scan, scatter.
-// The workspace holds [hist | offsets] (kCostBins words each), the per-ray
-// costs and the resulting order (n words each).
+// Counting sort of the rays by descending cost in two launches: a histogram,
+// then a scatter whose workgroups each derive the bins' start offsets from the
+// histogram.  The workspace holds two histograms (a call that sorts twice
+// uses one each, so both can be cleared by its first kernel), the per-bin
+// cursors the scatter reserves ranges with, then the per-ray costs and the
+// resulting order (n words each).
 constexpr int kSortBlock = 256;
+constexpr int kOrderHeadWords = 3 * kCostBins;  // hist 0, hist 1, cursors
 
+// LDS histogram per workgroup, added into hist; workgroup 0 also clears the
+// cursors the scatter will use.
 __global__ __launch_bounds__(kSortBlock) void k_cost_hist(const uint32_t* __restrict__ cost,
-                                                          int64_t n, uint32_t* __restrict__ hist) {
+                                                          int64_t n, uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ cursor) {
     __shared__ uint32_t h[kCostBins];
     for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) h[b] = 0u;
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) cursor[b] = 0u;
     __syncthreads();
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kSortBlock)
@@ -1646,44 +1655,54 @@ __global__ __launch_bounds__(kSortBlock) void k_cost_hist(const uint32_t* __rest
         if (h[b] != 0u) atomicAdd(&hist[b], h[b]);
 }
 
-// offsets[b] = number of rays with a cost above b (one block of kCostBins).
-__global__ __launch_bounds__(kCostBins) void k_cost_scan(const uint32_t* __restrict__ hist,
-                                                         uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t a[kCostBins];
-    const int t = threadIdx.x;
-    const int b = kCostBins - 1 - t;  // position t holds bin b: descending bins
-    a[t] = hist[b];
-    __syncthreads();
-    for (int off = 1; off < kCostBins; off <<= 1) {
-        const uint32_t v = t >= off ? a[t - off] : 0u;
-        __syncthreads();
-        a[t] += v;
-        __syncthreads();
-    }
-    offsets[b] = a[t] - hist[b];  // exclusive
-}
-
-// Each workgroup places kScatterRays consecutive rays: it counts them per bin
-// in LDS, reserves its range of every bin with one global atomic, then ranks
-// them.  Few workgroups per batch keep the reservations on a popular bin (rays
-// that miss the grid share bin 0) from queueing on one address.
+// Each workgroup places kScatterRays consecutive rays.  It computes where
+// every bin starts (the number of rays in higher bins: an exclusive scan of
+// the histogram over descending bins, in LDS), counts its rays per bin,
+// reserves its range of every bin with one atomic on that bin's cursor, then
+// ranks them.  Few workgroups per batch keep the reservations on a popular
+// bin (rays that miss the grid share bin 0) from queueing on one address.
 constexpr int64_t kScatterRays = 4096;
 
 __global__ __launch_bounds__(kSortBlock) void k_cost_scatter(const uint32_t* __restrict__ cost,
                                                              int64_t n,
-                                                             uint32_t* __restrict__ offsets,
+                                                             const uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ cursor,
                                                              uint32_t* __restrict__ order) {
+    constexpr int kPer = kCostBins / kSortBlock;
+    __shared__ uint32_t start[kCostBins];
+    __shared__ uint32_t part[kSortBlock];
     __shared__ uint32_t h[kCostBins];
-    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock) h[b] = 0u;
+    const int t = threadIdx.x;
+    // Thread t holds descending positions kPer * t .. kPer * t + kPer - 1.
+    uint32_t c[kPer], sum = 0u;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        c[j] = hist[kCostBins - 1 - (kPer * t + j)];
+        sum += c[j];
+    }
+    part[t] = sum;
+    for (int b = t; b < kCostBins; b += kSortBlock) h[b] = 0u;
     __syncthreads();
+    for (int off = 1; off < kSortBlock; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t at = part[t] - sum;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        start[kCostBins - 1 - (kPer * t + j)] = at;
+        at += c[j];
+    }
     const int64_t begin = static_cast<int64_t>(blockIdx.x) * kScatterRays;
     const int64_t end = min(n, begin + kScatterRays);
-    for (int64_t i = begin + threadIdx.x; i < end; i += kSortBlock) atomicAdd(&h[cost[i]], 1u);
+    for (int64_t i = begin + t; i < end; i += kSortBlock) atomicAdd(&h[cost[i]], 1u);
     __syncthreads();
-    for (int b = threadIdx.x; b < kCostBins; b += kSortBlock)
-        if (h[b] != 0u) h[b] = atomicAdd(&offsets[b], h[b]);
+    for (int b = t; b < kCostBins; b += kSortBlock)
+        if (h[b] != 0u) h[b] = start[b] + atomicAdd(&cursor[b], h[b]);
     __syncthreads();
-    for (int64_t i = begin + threadIdx.x; i < end; i += kSortBlock)
+    for (int64_t i = begin + t; i < end; i += kSortBlock)
         order[atomicAdd(&h[cost[i]], 1u)] = static_cast<uint32_t>(i);
 }
 
@@ -1702,7 +1721,10 @@ __global__ __launch_bounds__(kSortBlock) void k_zero_words(uint32_t* __restrict_
 __global__ __launch_bounds__(kSortBlock) void k_ray_length_key(
         GridView<float> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
-        float bins_per_voxel, uint32_t* __restrict__ key) {
+        float bins_per_voxel, uint32_t* __restrict__ key, uint32_t* __restrict__ clear) {
+    // Workgroup 0 clears both histograms of the ordering workspace.
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b < 2 * kCostBins; b += kSortBlock) clear[b] = 0u;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
     if (i >= n) return;
     const float wo[3] = {origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]};
@@ -1717,14 +1739,14 @@ __global__ __launch_bounds__(kSortBlock) void k_ray_length_key(
 }
 
 struct OrderWorkspace {
-    uint32_t* hist = nullptr;
-    uint32_t* offsets = nullptr;
+    uint32_t* hist = nullptr;    // two histograms of kCostBins words
+    uint32_t* cursor = nullptr;  // kCostBins words
     uint32_t* cost = nullptr;
     uint32_t* order = nullptr;
 };
 
 size_t order_workspace_bytes(int64_t n) {
-    const size_t words = 2 * static_cast<size_t>(kCostBins) + 2 * static_cast<size_t>(n);
+    const size_t words = static_cast<size_t>(kOrderHeadWords) + 2 * static_cast<size_t>(n);
     return (words * 4 + 255) / 256 * 256;
 }
 
@@ -1732,8 +1754,8 @@ OrderWorkspace order_workspace(void* ws, int64_t n) {
     OrderWorkspace w;
     uint32_t* p = static_cast<uint32_t*>(ws);
     w.hist = p;
-    w.offsets = p + kCostBins;
-    w.cost = p + 2 * kCostBins;
+    w.cursor = p + 2 * kCostBins;
+    w.cost = p + kOrderHeadWords;
     w.order = w.cost + n;
     return w;
 }
@@ -1828,15 +1850,18 @@ int64_t order_min_rays() {
 }
 
 // Sorts the rays by the costs the forward wrote (w.cost) into w.order.
-void launch_cost_order(const uint32_t* cost, const OrderWorkspace& w, int64_t n,
-                       hipStream_t s) {
-    const unsigned hb = static_cast<unsigned>((n + kScatterRays - 1) / kScatterRays);
-    hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)), dim3(kSortBlock), 0,
-                       s, w.hist, kCostBins);
-    hipLaunchKernelGGL(k_cost_hist, dim3(hb), dim3(kSortBlock), 0, s, cost, n, w.hist);
-    hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(kCostBins), 0, s, w.hist, w.offsets);
-    hipLaunchKernelGGL(k_cost_scatter, dim3(static_cast<unsigned>((n + kScatterRays - 1) / kScatterRays)),
-                       dim3(kSortBlock), 0, s, cost, n, w.offsets, w.order);
+// Sorts rays by descending cost into w.order with histogram `which` (0 or 1),
+// which must be clear when `cleared` says so (else it is cleared here).
+void launch_cost_order(const uint32_t* cost, const OrderWorkspace& w, int64_t n, hipStream_t s,
+                       int which = 0, bool cleared = false) {
+    uint32_t* hist = w.hist + which * kCostBins;
+    if (!cleared)
+        hipLaunchKernelGGL(k_zero_words, dim3(blocks_for(kCostBins, kSortBlock)), dim3(kSortBlock), 0,
+                           s, hist, kCostBins);
+    const unsigned blocks = static_cast<unsigned>((n + kScatterRays - 1) / kScatterRays);
+    hipLaunchKernelGGL(k_cost_hist, dim3(blocks), dim3(kSortBlock), 0, s, cost, n, hist, w.cursor);
+    hipLaunchKernelGGL(k_cost_scatter, dim3(blocks), dim3(kSortBlock), 0, s, cost, n, hist,
+                       w.cursor, w.order);
 }
 
 // Orders the rays of a forward by descending segment length (keys into
@@ -1849,8 +1874,8 @@ void launch_length_order(const svox2_grid_t* grid, const OptView& o, const float
                              static_cast<double>(grid->size[2]) * grid->size[2]);
     hipLaunchKernelGGL(k_ray_length_key, dim3(blocks_for(n, kSortBlock)), dim3(kSortBlock), 0, s,
                        make_view<float>(*grid), o, origins, dirs, clip, n,
-                       static_cast<float>((kCostBins - 1) / diag), w.cost);
-    launch_cost_order(w.cost, w, n, s);
+                       static_cast<float>((kCostBins - 1) / diag), w.cost, w.hist);
+    launch_cost_order(w.cost, w, n, s, 0, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2183,7 +2208,8 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
             hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
                                g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
                                (int32_t*)nullptr, w.cost, rs, order, log);
-        if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s);
+        // The key kernel cleared both histograms; this sort uses the second.
+        if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s, 1, true);
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
                                s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta, sparsity_loss,

@@ -160,10 +160,11 @@ def test_training_step_facade(gpu):
 
 
 def test_fused_ray_order(gpu, dense64):
-    """The fused call's backward ordering on a 65536-ray batch (above the
-    ordering threshold of an MI355X, 40960 rays): with a caller-owned
-    workspace the order is a permutation of the rays sorted by descending
-    chunk count, and colours and gradients match the input-order backward."""
+    """The fused call's ray orders on a 65536-ray batch (above the ordering
+    threshold of an MI355X, 40960 rays): with a caller-owned workspace the
+    backward's order is a permutation of the rays sorted by descending chunk
+    count (its histogram is the workspace's second), and colours and
+    gradients match the input-order call."""
     import ctypes
     from svox2_amd import _C
     _, _, _, _, sc = dense64
@@ -175,7 +176,8 @@ def test_fused_ray_order(gpu, dense64):
     gt_t = torch.rand(n, 3, generator=gen).to(gpu)
     lib = _C.load_library()
     nbytes = lib.svox2_ray_order_workspace_bytes(n)
-    assert nbytes >= 4 * (2 * 1024 + 2 * n)
+    # [forward-order histogram | backward-order histogram | cursors | cost | order]
+    assert nbytes >= 4 * (3 * 1024 + 2 * n)
     ws = torch.zeros(nbytes // 4, dtype=torch.int32, device=gpu)
 
     def run(workspace):
@@ -212,7 +214,7 @@ def test_fused_ray_order(gpu, dense64):
     assert np.array_equal(rgb_c, rgb_b)
     assert _rel(gd_c, gd_b) < 1e-5 and _rel(gs_c, gs_b) < 1e-5
     w = ws.cpu().numpy().view(np.uint32)
-    hist, cost, order = w[:1024], w[2048:2048 + n], w[2048 + n:2048 + 2 * n]
+    hist, cost, order = w[1024:2048], w[3072:3072 + n], w[3072 + n:3072 + 2 * n]
     assert int(hist.sum()) == n
     assert cost.max() > cost.min()  # rays of different lengths
     assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
