@@ -105,6 +105,35 @@ def plan_tiles(active: Sequence[RenderRequest], max_tiles: int, tile_pixels: int
     return plan
 
 
+def plan_runs(active: Sequence[RenderRequest], max_tiles: int, tile_pixels: int,
+              policy: str = "fifo") -> List[tuple]:
+    """The tile plan of ``plan_tiles`` as runs of consecutive tiles of one
+    request: (request, pixel0 array, n_pixels array) entries whose
+    concatenation is plan_tiles' list, with the same next_pixel updates.
+    The FIFO plan is built with array arithmetic per request rather than a
+    Python step per tile (it is on the serving loop's critical path)."""
+    if policy != "fifo":
+        runs: List[tuple] = []
+        for r, p0, n in plan_tiles(active, max_tiles, tile_pixels, policy):
+            runs.append((r, np.array([p0], np.int64), np.array([n], np.int64)))
+        return runs
+    runs = []
+    budget = max_tiles
+    for r in active:
+        if budget <= 0:
+            break
+        rem = r.pixels - r.next_pixel
+        if rem <= 0:
+            continue
+        k = min(budget, -(-rem // tile_pixels))
+        p0 = r.next_pixel + tile_pixels * np.arange(k, dtype=np.int64)
+        n = np.minimum(tile_pixels, r.pixels - p0)
+        runs.append((r, p0, n))
+        r.next_pixel = int(p0[-1] + n[-1])
+        budget -= k
+    return runs
+
+
 # svox2_tile_t as a numpy record (int32 grid, camera, pixel0, n_pixels; float* rgb_out).
 _TILE_DTYPE = np.dtype([("grid", "<i4"), ("camera", "<i4"), ("pixel0", "<i4"),
                         ("n_pixels", "<i4"), ("rgb_out", "<u8")])
@@ -127,7 +156,7 @@ class _Slot:
 
     def __init__(self, device):
         self.stream = torch.cuda.Stream(device=device)  # default priority
-        self.cam_version = -1  # camera table version last copied to this slot
+        self.cam_version: List[int] = []  # per launch group: camera table version copied
         self.host: List[torch.Tensor] = []
         self.host_np: List[np.ndarray] = []
         self.dev: List[torch.Tensor] = []
@@ -210,6 +239,7 @@ class RenderServer:
         self.slots = [_Slot(self.device) for _ in range(self.depth)]
         for slot in self.slots:
             for _ in self.groups:
+                slot.cam_version.append(-1)
                 slot.host.append(torch.zeros(self.table_bytes, dtype=torch.uint8).pin_memory())
                 slot.host_np.append(slot.host[-1].numpy())
                 slot.dev.append(torch.zeros(self.table_bytes, dtype=torch.uint8,
@@ -326,36 +356,39 @@ class RenderServer:
 
     def _fill_and_launch(self, slot: _Slot, pending: List[RenderRequest],
                          finishing: List[RenderRequest]) -> None:
-        cams_dirty = slot.cam_version != self.cam_version
         for gi, grp in enumerate(self.groups):
             mine = [r for r in pending if self.group_of[r.grid] is grp]
-            plan = plan_tiles(mine, self.max_tiles, self.tile_pixels, self.policy)
+            runs = plan_runs(mine, self.max_tiles, self.tile_pixels, self.policy)
+            if not runs:
+                continue  # nothing launched: this group's table is not read
             host = slot.host_np[gi]
-            host[:4].view(np.int32)[0] = len(plan)
-            if plan:
-                tiles = host[self.tiles_off:self.tiles_off + len(plan) * self.tile_size].view(
-                    _TILE_DTYPE)
-                tiles["grid"] = [grp.local[r.grid] for r, _, _ in plan]
-                tiles["camera"] = [r.slot for r, _, _ in plan]
-                tiles["pixel0"] = [p0 for _, p0, _ in plan]
-                tiles["n_pixels"] = [n for _, _, n in plan]
-                tiles["rgb_out"] = [r.image.data_ptr() for r, _, _ in plan]
-                self.rays_rendered += int(tiles["n_pixels"].sum())
-                for r, _, _ in plan:
-                    if r.next_pixel >= r.pixels and r not in finishing:
-                        finishing.append(r)
-            nbytes = self.tiles_off + len(plan) * self.tile_size
-            if cams_dirty:
+            n_tiles = sum(len(p0) for _, p0, _ in runs)
+            host[:4].view(np.int32)[0] = n_tiles
+            tiles = host[self.tiles_off:self.tiles_off + n_tiles * self.tile_size].view(_TILE_DTYPE)
+            at = 0
+            for r, p0, n in runs:
+                k = len(p0)
+                t = tiles[at:at + k]
+                t["grid"] = grp.local[r.grid]
+                t["camera"] = r.slot
+                t["pixel0"] = p0
+                t["n_pixels"] = n
+                t["rgb_out"] = r.image.data_ptr()
+                at += k
+                self.rays_rendered += int(n.sum())
+                if r.next_pixel >= r.pixels and r not in finishing:
+                    finishing.append(r)
+            nbytes = self.tiles_off + n_tiles * self.tile_size
+            if slot.cam_version[gi] != self.cam_version:
                 host[self.cams_off:] = np.frombuffer(self.cams, dtype=np.uint8)
                 nbytes = self.table_bytes
+                slot.cam_version[gi] = self.cam_version
             slot.dev[gi][:nbytes].copy_(slot.host[gi][:nbytes], non_blocking=True)
-            if plan:
-                size = next(c for c in self.size_classes if c >= len(plan))
-                if self.use_graph:
-                    slot.graphs[gi][size].replay()
-                else:
-                    self._launch(grp, slot.dev[gi], size)
-        slot.cam_version = self.cam_version
+            size = next(c for c in self.size_classes if c >= n_tiles)
+            if self.use_graph:
+                slot.graphs[gi][size].replay()
+            else:
+                self._launch(grp, slot.dev[gi], size)
 
     def drain(self) -> List[RenderRequest]:
         done: List[RenderRequest] = []

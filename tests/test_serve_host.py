@@ -1,11 +1,14 @@
 """Scheduling logic of the serving loop (no GPU): both tile plans cover every
 pixel of every request exactly once, in order, within the per-launch tile cap;
 FIFO serves requests in admission order, round-robin lets small requests
-finish before large ones."""
+finish before large ones; the array-built plan the server uses equals the
+per-tile plan."""
 import pytest
 import math
 
-from svox2_amd.serve import RenderRequest, plan_tiles
+import numpy as np
+
+from svox2_amd.serve import RenderRequest, plan_runs, plan_tiles
 
 
 class _Cam:
@@ -69,3 +72,22 @@ def test_tile_record_matches_c_layout():
     assert serve._TILE_DTYPE.itemsize == ctypes.sizeof(t)
     for name in ("grid", "camera", "pixel0", "n_pixels", "rgb_out"):
         assert serve._TILE_DTYPE.fields[name][1] == getattr(t, name).offset
+
+
+@pytest.mark.parametrize("policy", ["fifo", "round_robin"])
+def test_plan_runs_equal_plan_tiles(policy):
+    """The array-built plan (runs of one request's consecutive tiles) is the
+    per-tile plan, with the same request progress."""
+    g = np.random.default_rng(5)
+    for trial in range(40):
+        sides = g.integers(1, 900, size=g.integers(1, 6))
+        a = [_req(int(s), int(s)) for s in sides]
+        b = [_req(int(s), int(s)) for s in sides]
+        for r, q in zip(a, b):
+            r.next_pixel = q.next_pixel = int(g.integers(0, r.pixels))
+        tp, mt = int(g.integers(100, 5000)), int(g.integers(1, 40))
+        plan = plan_tiles(a, max_tiles=mt, tile_pixels=tp, policy=policy)
+        runs = plan_runs(b, max_tiles=mt, tile_pixels=tp, policy=policy)
+        flat = [(b.index(r), int(p), int(n)) for r, p0, nn in runs for p, n in zip(p0, nn)]
+        assert flat == [(a.index(r), p, n) for r, p, n in plan]
+        assert [r.next_pixel for r in a] == [r.next_pixel for r in b]
