@@ -190,26 +190,50 @@ class MarchRef:
         """Every evaluated sample, in ray order: dict of ray (S,), t (S,),
         l (S,3) int, p (S,3) float32, ws (S,), flag (S,) (bit 0: FP32 density
         above sigma_thresh; bit 1: the FP32 march has not stopped before it)."""
+        parts = list(self.sample_blocks(g, origins, dirs, opt, clip=clip, skip=skip))
+        out = {k: np.concatenate([q[2][k] for q in parts]) for k in parts[0][2]}
+        out["ray"] = np.concatenate([q[2]["ray"] + q[0] for q in parts])
+        return out
+
+    def sample_blocks(self, g: Grid, origins, dirs, opt: Opts, clip=None, skip: bool = True,
+                      block: int = 2048, post=None):
+        """The samples of ``samples`` block by block: yields (r0, r1, samples of
+        rays [r0, r1) with ray ids relative to r0) in ray order, or (r0, r1,
+        *post(samples)) with ``post``.  Rays are independent, so blocks are
+        marched on worker threads (ctypes releases the GIL), ``post`` too."""
         keep: list = []
         gs = self._grid(g, keep)
         o, d, c = self._rays(origins, dirs, clip, keep)
         n = o.shape[0]
         ops = self._opt(opt, skip)
-        cap = max(1024, n * 64)
-        while True:
-            ray = np.empty(cap, np.int32)
-            t = np.empty(cap, np.float32)
-            l = np.empty((cap, 3), np.int32)
-            p = np.empty((cap, 3), np.float32)
-            ws = np.empty(cap, np.float32)
-            flag = np.empty(cap, np.uint8)
-            cnt = self.lib.ref_samples(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), n,
-                                       ctypes.byref(ops), cap, _c_ptr(ray), _c_ptr(t), _c_ptr(l),
-                                       _c_ptr(p), _c_ptr(ws), _c_ptr(flag))
-            if cnt <= cap:
-                return dict(ray=ray[:cnt], t=t[:cnt], l=l[:cnt], p=p[:cnt], ws=ws[:cnt],
-                            flag=flag[:cnt])
-            cap = int(cnt)
+
+        def run(b):
+            r0, r1 = b
+            nb = r1 - r0
+            oc, dc = _c_ptr(o[r0:r1]), _c_ptr(d[r0:r1])
+            cc = None if c is None else _c_ptr(c[r0:r1])
+            cap = max(1024, nb * 64)
+            while True:
+                out = dict(ray=np.empty(cap, np.int32), t=np.empty(cap, np.float32),
+                           l=np.empty((cap, 3), np.int32), p=np.empty((cap, 3), np.float32),
+                           ws=np.empty(cap, np.float32), flag=np.empty(cap, np.uint8))
+                cnt = self.lib.ref_samples(ctypes.byref(gs), oc, dc, cc, nb, ctypes.byref(ops),
+                                           cap, *(_c_ptr(out[k]) for k in
+                                                  ("ray", "t", "l", "p", "ws", "flag")))
+                if cnt <= cap:
+                    break
+                cap = int(cnt)
+            out = {k: v[:cnt] for k, v in out.items()}
+            return (r0, r1, out) if post is None else (r0, r1, *post(out))
+
+        bounds = [(r0, min(n, r0 + block)) for r0 in range(0, n, block)] or [(0, 0)]
+        if len(bounds) == 1:
+            yield run(bounds[0])
+            return
+        from concurrent.futures import ThreadPoolExecutor
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        with ThreadPoolExecutor(workers) as ex:
+            yield from ex.map(run, bounds)
 
     def ray_setup(self, g: Grid, origins, dirs, opt: Opts, clip=None) -> np.ndarray:
         keep: list = []
@@ -309,8 +333,11 @@ _CORNERS = [(dx, dy, dz) for dx in (0, 1) for dy in (0, 1) for dz in (0, 1)]
 
 def corner_gather(links: np.ndarray, l: np.ndarray) -> np.ndarray:
     """(S, 8) corner links of cells l (S, 3), corner order (dx, dy, dz)."""
-    return np.stack([links[l[:, 0] + dx, l[:, 1] + dy, l[:, 2] + dz] for dx, dy, dz in _CORNERS],
-                    axis=1)
+    links = np.ascontiguousarray(links)
+    _, n1, n2 = links.shape
+    base = (l[:, 0].astype(np.int64) * n1 + l[:, 1]) * n2 + l[:, 2]
+    offs = np.array([(dx * n1 + dy) * n2 + dz for dx, dy, dz in _CORNERS], np.int64)
+    return links.reshape(-1)[base[:, None] + offs[None, :]]
 
 
 def corner_weights(p: torch.Tensor) -> torch.Tensor:
@@ -342,7 +369,8 @@ def _composite(grid: Grid, dirs: np.ndarray, opt: Opts, density: torch.Tensor,
                          counted=torch.zeros(0, dtype=torch.bool))
         return torch.full((n, 3), bg, dtype=dt) + 0 * (density.sum() + sh.sum())
     basis = eval_sh_bases(B, torch.from_numpy(dirs).to(dt))  # at the raw dirs, as the kernel
-    lk = torch.from_numpy(corner_gather(grid.links, smp["l"]).astype(np.int64))
+    lk = smp["lk"] if "lk" in smp else corner_gather(grid.links, smp["l"])
+    lk = torch.from_numpy(lk.astype(np.int64))
     rows = density.shape[0]
     valid = (lk >= 0) & (lk < rows)
     lk0 = torch.where(valid, lk, torch.zeros_like(lk))
@@ -404,6 +432,26 @@ def _ray_chunks(smp: dict, n: int, max_samples: int):
         r0 = r1
 
 
+def _compact_rows(grid: Grid, smp: dict, rows: int):
+    """Drops the samples none of whose 8 corners has a row and renumbers the
+    others' corner links to the rows the chunk uses.  A dropped sample has
+    sigma = 0 exactly, so with sigma_thresh >= 0 it neither contributes nor
+    moves the transmittance: the chunk's colours and gradients are unchanged,
+    and autograd works on the used rows instead of the whole grid.  Returns
+    the compacted samples (with "lk", local links, -1 for none) and the used
+    global rows."""
+    lk = corner_gather(grid.links, smp["l"]).astype(np.int64)
+    valid = (lk >= 0) & (lk < rows)
+    keep = valid.any(1)
+    smp = {k: v[keep] for k, v in smp.items()}
+    lk, valid = lk[keep], valid[keep]
+    used, inv = np.unique(lk[valid], return_inverse=True)
+    local = np.full(lk.shape, -1, np.int64)
+    local[valid] = inv.reshape(-1)
+    smp["lk"] = local
+    return smp, torch.from_numpy(used)
+
+
 def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
                 sh: torch.Tensor, clip=None, samples=None, thresholds: bool = True,
                 max_samples: int = 2_000_000):
@@ -421,12 +469,29 @@ def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
     origins = np.asarray(origins, np.float32)
     dirs = np.asarray(dirs, np.float32)
     n = origins.shape[0]
-    if samples is None:
-        samples = march().samples(grid, origins, dirs, opt, clip=clip)
+    compact = thresholds and opt.sigma_thresh >= 0
     outs = []
-    for r0, r1, sub in _ray_chunks(samples, n, max_samples):
-        outs.append(_composite(grid, dirs[r0:r1], opt, density, sh, sub, r1 - r0, thresholds))
+    for b0, b1, bsub, used in _blocks(grid, origins, dirs, opt, clip, samples, max_samples,
+                                      density.shape[0], compact):
+        dens, shb = (density[used], sh[used]) if compact else (density, sh)
+        for c0, c1, sub in _ray_chunks(bsub, b1 - b0, max_samples):
+            r0, r1 = b0 + c0, b0 + c1
+            outs.append(_composite(grid, dirs[r0:r1], opt, dens, shb, sub, r1 - r0, thresholds))
     return torch.cat(outs, 0) if outs else torch.zeros((0, 3), dtype=torch.float64)
+
+
+def _blocks(grid: Grid, origins, dirs, opt: Opts, clip, samples, max_samples: int, rows: int,
+            compact: bool):
+    """(r0, r1, samples of rays [r0, r1) relative to r0, used rows) in ray
+    order: marched block by block when ``samples`` is None, else cut from it;
+    compacted by ``_compact_rows`` when ``compact``, else with every row."""
+    every = torch.arange(rows)
+    post = (lambda smp: _compact_rows(grid, smp, rows)) if compact else (lambda smp: (smp, every))
+    if samples is None:
+        yield from march().sample_blocks(grid, origins, dirs, opt, clip=clip, post=post)
+    else:
+        for r0, r1, sub in _ray_chunks(samples, np.asarray(origins).shape[0], max_samples):
+            yield (r0, r1, *post(sub))
 
 
 def touched_rows(grid: Grid, samples: dict) -> np.ndarray:
@@ -454,27 +519,34 @@ def mse_grads_fp64(grid: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, samp
     dirs = np.asarray(dirs, np.float32)
     n = origins.shape[0]
     gt = torch.as_tensor(np.asarray(rgb_gt, np.float64))
-    dens = torch.from_numpy(np.asarray(grid.density, np.float64).reshape(-1)).requires_grad_(True)
-    sh = torch.from_numpy(np.asarray(grid.sh, np.float64)).requires_grad_(True)
-    if samples is None:
-        samples = march().samples(grid, origins, dirs, opt, clip=clip)
+    dens_all = torch.from_numpy(np.asarray(grid.density, np.float64).reshape(-1))
+    sh_all = torch.from_numpy(np.asarray(grid.sh, np.float64))
+    gd = torch.zeros_like(dens_all)
+    gs = torch.zeros_like(sh_all)
     rgbs = []
-    for r0, r1, sub in _ray_chunks(samples, n, max_samples):
-        extra = {}
-        rgb = _composite(grid, dirs[r0:r1], opt, dens, sh, sub, r1 - r0, True, extra)
-        loss = ((rgb - gt[r0:r1]) ** 2).sum() / (3.0 * n)
-        if beta_loss > 0:
-            t_end = extra["t_end"].detach()
-            b = beta_loss / n * (1.0 - t_end / (1.0 - t_end + 1e-3))
-            loss = loss - (b * extra["depth"]).sum()
-        if sparsity_loss > 0:
-            s_ = extra["sigma"][extra["counted"]]
-            loss = loss + sparsity_loss * torch.log1p(2.0 * s_ * s_).sum()
-        loss.backward()
-        rgbs.append(rgb.detach())
+    for b0, b1, bsub, used in _blocks(grid, origins, dirs, opt, clip, samples, max_samples,
+                                      dens_all.shape[0], opt.sigma_thresh >= 0):
+        dens = dens_all[used].requires_grad_(True)
+        sh = sh_all[used].requires_grad_(True)
+        for c0, c1, sub in _ray_chunks(bsub, b1 - b0, max_samples):
+            r0, r1 = b0 + c0, b0 + c1
+            extra = {}
+            rgb = _composite(grid, dirs[r0:r1], opt, dens, sh, sub, r1 - r0, True, extra)
+            loss = ((rgb - gt[r0:r1]) ** 2).sum() / (3.0 * n)
+            if beta_loss > 0:
+                t_end = extra["t_end"].detach()
+                b = beta_loss / n * (1.0 - t_end / (1.0 - t_end + 1e-3))
+                loss = loss - (b * extra["depth"]).sum()
+            if sparsity_loss > 0:
+                s_ = extra["sigma"][extra["counted"]]
+                loss = loss + sparsity_loss * torch.log1p(2.0 * s_ * s_).sum()
+            loss.backward()
+            rgbs.append(rgb.detach())
+        if dens.grad is not None:
+            gd.index_add_(0, used, dens.grad)
+        if sh.grad is not None:
+            gs.index_add_(0, used, sh.grad)
     rgb = torch.cat(rgbs, 0) if rgbs else torch.zeros((0, 3), dtype=torch.float64)
-    gd = dens.grad if dens.grad is not None else torch.zeros_like(dens)
-    gs = sh.grad if sh.grad is not None else torch.zeros_like(sh)
     return rgb.numpy(), gd.numpy(), gs.numpy()
 
 

@@ -962,10 +962,11 @@ struct GradOut {
 // ray's first cell), otherwise kMoveStep | (d0 + 1) << 1 | (d1 + 1) << 3 |
 // (d2 + 1) << 5 for a step of d in {-1, 0, 1}^3.
 struct __align__(16) SampleRec {
-    float p[3];
-    int move;
+    float w[8];    // trilinear weights (wy * wz) * wx of corners k = dx dy dz
     float val[4];  // colour factors R, G, B and the density gradient
     int lk[8];
+    int move;
+    int pad[3];
 };
 constexpr int kMoveStep = 1;
 constexpr int kMoveJump = 0x80;
@@ -1114,8 +1115,8 @@ struct BwdRay {
 // formed sample-parallel - the colour factors weight * [colour >= 0] * g and
 // the density gradient ws * (total * T_after - accum), with the running
 // `accum` as a prefix sum over the chunk - and staged as LDS records with each
-// sample's cell move; then the samples are added in order into the corner
-// accumulators.
+// sample's 8 trilinear corner weights and cell move; then the samples are
+// added in order into the corner accumulators, one fma per corner each.
 template <typename DT, int B>
 __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradOut& go,
                                               BwdRay<B>& st, int lane, bool live,
@@ -1155,11 +1156,14 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
         else
             move = kMoveStep | ((d0 + 1) << 1) | ((d1 + 1) << 3) | ((d2 + 1) << 5);
         SampleRec& R = st.rec[lane];
+        const float wx[2] = {1.f - p[0], p[0]};
+        const float wy[2] = {1.f - p[1], p[1]};
+        const float wz[2] = {1.f - p[2], p[2]};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            R.p[a] = p[a];
+        for (int k = 0; k < 8; ++k) R.w[k] = wy[(k >> 1) & 1] * wz[k & 1] * wx[k >> 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
             R.val[a] = weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * st.gq[a];
-        }
         R.move = move;
         R.val[3] = gsig;
 #pragma unroll
@@ -1172,23 +1176,27 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     sc.c2 = group_bcast(l[2], last);
     sc.have = true;
     wave_lds_fence();
-    // Each iteration reads the next sample's position and value ahead, so
-    // their LDS latency overlaps the current sample's work.
+    // Each iteration reads the next sample's weights, value and move ahead,
+    // so their LDS latency overlaps the current sample's work.
     uint32_t m = live_mask;
     int s_next = __ffs(m) - 1;
-    float4 pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
+    float4 wa_next = *reinterpret_cast<const float4*>(st.rec[s_next].w);
+    float4 wb_next = *reinterpret_cast<const float4*>(st.rec[s_next].w + 4);
     float v_next = st.rec[s_next].val[st.my_q];
+    int mv_next = st.rec[s_next].move;
     while (m) {
         const int s_cur = s_next;
-        const float4 pb = pb_next;
+        const float4 wa = wa_next, wb = wb_next;
         const float v = v_next;
+        const int move = mv_next;
         m &= m - 1;
         if (m) {
             s_next = __ffs(m) - 1;
-            pb_next = *reinterpret_cast<const float4*>(st.rec[s_next].p);
+            wa_next = *reinterpret_cast<const float4*>(st.rec[s_next].w);
+            wb_next = *reinterpret_cast<const float4*>(st.rec[s_next].w + 4);
             v_next = st.rec[s_next].val[st.my_q];
+            mv_next = st.rec[s_next].move;
         }
-        const int move = __float_as_int(pb.w);
         if (move != 0) {
             const SampleRec& R = st.rec[s_cur];
             const int4 la = *reinterpret_cast<const int4*>(R.lk);
@@ -1203,13 +1211,9 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
             sc.link[0] = la.x; sc.link[1] = la.y; sc.link[2] = la.z; sc.link[3] = la.w;
             sc.link[4] = lb.x; sc.link[5] = lb.y; sc.link[6] = lb.z; sc.link[7] = lb.w;
         }
-        // Trilinear weights (wy * wz) * wx of the 8 corners.
-        const float wx[2] = {1.f - pb.x, pb.x};
-        const float wy[2] = {1.f - pb.y, pb.y};
-        const float wz[2] = {1.f - pb.z, pb.z};
+        const float w[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            sc.acc[k] = fmaf(wy[(k >> 1) & 1] * wz[k & 1] * wx[k >> 2], v, sc.acc[k]);
+        for (int k = 0; k < 8; ++k) sc.acc[k] = fmaf(w[k], v, sc.acc[k]);
     }
     wave_lds_fence();  // records are rewritten by the next chunk
 }
