@@ -174,7 +174,7 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  *   stream in the workspace and the backward replays it instead of
  *   re-marching the ray (a ray whose records do not fit is re-marched);
  *   for batches at or above it the forward also logs where it placed the
- *   samples of each ray's first 8 chunks, and the backward evaluates those
+ *   samples of each ray's first 16 chunks, and the backward evaluates those
  *   chunks from the log instead of placing them again.
  * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
@@ -196,7 +196,7 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 /* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
  * fastest for an n_rays batch (0 for an empty batch): the record stream
  * (about 4.6 KiB per ray) below the ordering threshold, the ray-ordering
- * workspace plus the chunk log (about 1.2 KiB per ray) at or above it.  No
+ * workspace plus the chunk log (about 2.3 KiB per ray) at or above it.  No
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 

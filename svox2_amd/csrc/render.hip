@@ -781,7 +781,7 @@ struct ChunkLog {
     float* __restrict__ rec;
     int4* __restrict__ head;
 };
-constexpr int kLogChunks = 8;
+constexpr int kLogChunks = 16;
 constexpr int kLogWords = 36;
 
 // Record stream of the fused call: the forward appends, per ray, a header and

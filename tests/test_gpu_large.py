@@ -78,11 +78,13 @@ def test_512_colours_and_grads(gpu, big, march, dtype):
 def test_512_chunk_log(gpu, big):
     """The fused call on a full 65,536-ray batch with the workspace it asks for
     (ray order plus the chunk log: the backward evaluates the forward's logged
-    chunk placements, the first 8 chunks of each ray, and places the rest
+    chunk placements, the first 16 chunks of each ray, and places the rest
     itself) against the call without a workspace (input order, every chunk
     placed by the backward): colours bit-exact, gradients within 1e-5 of
-    each other (atomic order differs), masks equal.  Rays with more than 8
-    chunks occur on this grid, so both kinds of chunk run in one backward."""
+    each other (atomic order differs), masks equal.  Each ray's log head
+    holds min(chunks, 16) and whether the ray went on, its chunk count taken
+    from the plain forward.  Rays with more than 16 chunks occur on this
+    batch, so both kinds of chunk run in one backward."""
     import ctypes
     sc, grid, links, o, d, gt = big
     lib = _C.load_library()
@@ -91,7 +93,7 @@ def test_512_chunk_log(gpu, big):
     gt_t = gt.to(gpu)
     nbytes = lib.svox2_volume_render_cuvol_fused_workspace_bytes(n)
     order_bytes = lib.svox2_ray_order_workspace_bytes(n)
-    rec_bytes = (n * 8 * 36 * 4 + 255) // 256 * 256
+    rec_bytes = (n * 16 * 36 * 4 + 255) // 256 * 256
     assert nbytes == order_bytes + rec_bytes + (16 * n + 255) // 256 * 256
     ws = torch.zeros(nbytes // 4, dtype=torch.int32, device=gpu)
 
@@ -120,11 +122,18 @@ def test_512_chunk_log(gpu, big):
     rgb_a, gd_a, gs_a, m_a = run(ws)
     rgb_b, gd_b, gs_b, m_b = run(None)
     heads = ws.cpu().numpy()[(order_bytes + rec_bytes) // 4:].reshape(-1, 4)[:n]
-    assert heads[:, 0].min() >= 0 and heads[:, 0].max() == 8
-    assert np.any(heads[:, 1] == 1)  # rays continued past their logged chunks
+    chunks = torch.empty(n, dtype=torch.int32, device=gpu)
+    _C.volume_render_cuvol(grid._to_cpp(), rays._to_cpp(), grid.opt._to_cpp(),
+                           ray_cost_out=chunks)
+    chunks = chunks.cpu().numpy()
+    assert chunks.max() < 1023  # below the cost clamp: the counts are exact
+    assert np.array_equal(heads[:, 0], np.minimum(chunks, 16))
+    assert np.array_equal(heads[:, 1], (chunks > 16).astype(np.int32))
+    assert np.any(chunks > 16)  # rays continued past their logged chunks
     assert np.array_equal(rgb_a, rgb_b)
     rel = lambda a, b: np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
-    print(f"chunk log: {int((heads[:, 1] == 1).sum())} rays past 8 chunks; grad rel "
+    print(f"chunk log: {int((heads[:, 1] == 1).sum())} rays past 16 chunks (max "
+          f"{chunks.max()}); grad rel "
           f"{rel(gd_a, gd_b):.2e} / {rel(gs_a, gs_b):.2e}")
     assert rel(gd_a, gd_b) < 1e-5 and rel(gs_a, gs_b) < 1e-5
     assert np.array_equal(m_a, m_b)
