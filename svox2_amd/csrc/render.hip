@@ -1176,44 +1176,61 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     sc.c2 = group_bcast(l[2], last);
     sc.have = true;
     wave_lds_fence();
-    // Each iteration reads the next sample's weights, value and move ahead,
-    // so their LDS latency overlaps the current sample's work.
-    uint32_t m = live_mask;
-    int s_next = __ffs(m) - 1;
-    float4 wa_next = *reinterpret_cast<const float4*>(st.rec[s_next].w);
-    float4 wb_next = *reinterpret_cast<const float4*>(st.rec[s_next].w + 4);
-    float v_next = st.rec[s_next].val[st.my_q];
-    int mv_next = st.rec[s_next].move;
-    while (m) {
-        const int s_cur = s_next;
-        const float4 wa = wa_next, wb = wb_next;
-        const float v = v_next;
-        const int move = mv_next;
-        m &= m - 1;
-        if (m) {
-            s_next = __ffs(m) - 1;
-            wa_next = *reinterpret_cast<const float4*>(st.rec[s_next].w);
-            wb_next = *reinterpret_cast<const float4*>(st.rec[s_next].w + 4);
-            v_next = st.rec[s_next].val[st.my_q];
-            mv_next = st.rec[s_next].move;
-        }
-        if (move != 0) {
-            const SampleRec& R = st.rec[s_cur];
-            const int4 la = *reinterpret_cast<const int4*>(R.lk);
-            const int4 lb = *reinterpret_cast<const int4*>(R.lk + 4);
-            if (move & kMoveJump) {
+    // Two staged records alternate: while one sample is added, the next
+    // one's record is on its way from LDS.  Every read is unconditional
+    // (a lane past the last live sample reads the first one again), so each
+    // wait covers exactly the record it needs.
+    struct Staged {
+        float4 wa, wb;
+        int4 la, lb;
+        float v;
+        int move;
+    };
+    auto stage = [&](int s) {
+        const SampleRec& R = st.rec[s];
+        Staged q;
+        q.wa = *reinterpret_cast<const float4*>(R.w);
+        q.wb = *reinterpret_cast<const float4*>(R.w + 4);
+        q.la = *reinterpret_cast<const int4*>(R.lk);
+        q.lb = *reinterpret_cast<const int4*>(R.lk + 4);
+        q.v = R.val[st.my_q];
+        q.move = R.move;
+        return q;
+    };
+    auto add = [&](const Staged& q) {
+        if (q.move != 0) {
+            if (q.move & kMoveJump) {
                 sc.flush_all(g, go, lane);
             } else {
-                sc.template shift<4>(((move >> 1) & 3) - 1, g, go, lane);
-                sc.template shift<2>(((move >> 3) & 3) - 1, g, go, lane);
-                sc.template shift<1>(((move >> 5) & 3) - 1, g, go, lane);
+                sc.template shift<4>(((q.move >> 1) & 3) - 1, g, go, lane);
+                sc.template shift<2>(((q.move >> 3) & 3) - 1, g, go, lane);
+                sc.template shift<1>(((q.move >> 5) & 3) - 1, g, go, lane);
             }
-            sc.link[0] = la.x; sc.link[1] = la.y; sc.link[2] = la.z; sc.link[3] = la.w;
-            sc.link[4] = lb.x; sc.link[5] = lb.y; sc.link[6] = lb.z; sc.link[7] = lb.w;
+            sc.link[0] = q.la.x; sc.link[1] = q.la.y; sc.link[2] = q.la.z; sc.link[3] = q.la.w;
+            sc.link[4] = q.lb.x; sc.link[5] = q.lb.y; sc.link[6] = q.lb.z; sc.link[7] = q.lb.w;
         }
-        const float w[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+        const float w[8] = {q.wa.x, q.wa.y, q.wa.z, q.wa.w, q.wb.x, q.wb.y, q.wb.z, q.wb.w};
 #pragma unroll
-        for (int k = 0; k < 8; ++k) sc.acc[k] = fmaf(w[k], v, sc.acc[k]);
+        for (int k = 0; k < 8; ++k) sc.acc[k] = fmaf(w[k], q.v, sc.acc[k]);
+    };
+    uint32_t m = live_mask;
+    const int s0 = __ffs(m) - 1;
+    m &= m - 1u;
+    Staged x = stage(s0);
+    bool have_y = m != 0u;
+    Staged y = stage(have_y ? __ffs(m) - 1 : s0);
+    m &= m - 1u;
+    while (true) {
+        add(x);
+        if (!have_y) break;
+        const bool have_x = m != 0u;
+        x = stage(have_x ? __ffs(m) - 1 : s0);
+        m &= m - 1u;
+        add(y);
+        if (!have_x) break;
+        have_y = m != 0u;
+        y = stage(have_y ? __ffs(m) - 1 : s0);
+        m &= m - 1u;
     }
     wave_lds_fence();  // records are rewritten by the next chunk
 }
@@ -1385,8 +1402,10 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
 #ifndef SVOX2_REC_WAVES
 #define SVOX2_REC_WAVES 4
 #endif
+// The replay kernel at 5 waves: 96 registers hold the scatter loop's two
+// staged records without spilling.
 #ifndef SVOX2_REPLAY_WAVES
-#define SVOX2_REPLAY_WAVES 6
+#define SVOX2_REPLAY_WAVES 5
 #endif
 #define SVOX2_REPLAY_KERNEL __global__ __launch_bounds__(kBlock) \
     __attribute__((amdgpu_waves_per_eu(SVOX2_REPLAY_WAVES)))
