@@ -473,7 +473,7 @@ def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
     outs = []
     for b0, b1, bsub, used in _blocks(grid, origins, dirs, opt, clip, samples, max_samples,
                                       density.shape[0], compact):
-        dens, shb = (density[used], sh[used]) if compact else (density, sh)
+        dens, shb = (density, sh) if used is None else (density[used], sh[used])
         for c0, c1, sub in _ray_chunks(bsub, b1 - b0, max_samples):
             r0, r1 = b0 + c0, b0 + c1
             outs.append(_composite(grid, dirs[r0:r1], opt, dens, shb, sub, r1 - r0, thresholds))
@@ -484,9 +484,9 @@ def _blocks(grid: Grid, origins, dirs, opt: Opts, clip, samples, max_samples: in
             compact: bool):
     """(r0, r1, samples of rays [r0, r1) relative to r0, used rows) in ray
     order: marched block by block when ``samples`` is None, else cut from it;
-    compacted by ``_compact_rows`` when ``compact``, else with every row."""
-    every = torch.arange(rows)
-    post = (lambda smp: _compact_rows(grid, smp, rows)) if compact else (lambda smp: (smp, every))
+    compacted by ``_compact_rows`` when ``compact``, else with used rows None
+    (every row, in the caller's own numbering)."""
+    post = (lambda smp: _compact_rows(grid, smp, rows)) if compact else (lambda smp: (smp, None))
     if samples is None:
         yield from march().sample_blocks(grid, origins, dirs, opt, clip=clip, post=post)
     else:
@@ -526,8 +526,8 @@ def mse_grads_fp64(grid: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, samp
     rgbs = []
     for b0, b1, bsub, used in _blocks(grid, origins, dirs, opt, clip, samples, max_samples,
                                       dens_all.shape[0], opt.sigma_thresh >= 0):
-        dens = dens_all[used].requires_grad_(True)
-        sh = sh_all[used].requires_grad_(True)
+        dens = (dens_all.clone() if used is None else dens_all[used]).requires_grad_(True)
+        sh = (sh_all.clone() if used is None else sh_all[used]).requires_grad_(True)
         for c0, c1, sub in _ray_chunks(bsub, b1 - b0, max_samples):
             r0, r1 = b0 + c0, b0 + c1
             extra = {}
@@ -543,9 +543,9 @@ def mse_grads_fp64(grid: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, samp
             loss.backward()
             rgbs.append(rgb.detach())
         if dens.grad is not None:
-            gd.index_add_(0, used, dens.grad)
+            gd.index_add_(0, used, dens.grad) if used is not None else gd.add_(dens.grad)
         if sh.grad is not None:
-            gs.index_add_(0, used, sh.grad)
+            gs.index_add_(0, used, sh.grad) if used is not None else gs.add_(sh.grad)
     rgb = torch.cat(rgbs, 0) if rgbs else torch.zeros((0, 3), dtype=torch.float64)
     return rgb.numpy(), gd.numpy(), gs.numpy()
 
