@@ -176,6 +176,153 @@ def measure(grid, batches, steps, warmup, s_sigma, s_sh):
     )
 
 
+def _event_ms(fn, reps: int) -> float:
+    """Average ms of fn() over reps launches, HIP events on the current stream
+    (the stream the kernels are launched on)."""
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _roof(ms: float, nbytes: float) -> dict:
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return dict(ms=ms, alg_bytes=int(nbytes), alg_gbps=gbps, frac=gbps / HBM_PEAK_GBS)
+
+
+def _tv_pairs(links: torch.Tensor, cells: torch.Tensor, rows: int, ignore_edge: bool):
+    """(cells inside the grid, (cell, neighbour) pairs whose link addresses a
+    row) of one tv_grad_sparse launch: the cell and its +z, +y, +x
+    neighbours, an outside neighbour reading link 0, and with ignore_edge the
+    cells whose own link is 0 skipped (loss_kernel.cu:178-246).  Each pair is
+    one data read and, for a nonzero difference, one gradient atomic and one
+    mask store per channel."""
+    s0, s1, s2 = links.shape
+    n = s0 * s1 * s2
+    flat = links.reshape(-1).long()
+    c = cells.long()
+    z, y, x = c % s2, (c // s2) % s1, c // (s1 * s2)
+    inside = (c >= 0) & (x < s0)
+    at = lambda i: flat[i.clamp(0, n - 1)]
+    zero = torch.zeros_like(c)
+    l000 = at(c)
+    l001 = torch.where(z + 1 < s2, at(c + 1), zero)
+    l010 = torch.where(y + 1 < s1, at(c + s2), zero)
+    l100 = torch.where(x + 1 < s0, at(c + s1 * s2), zero)
+    ok = lambda l: ((l >= 0) & (l < rows)).long()
+    live = inside & ~((l000 == 0) & ignore_edge)
+    pairs = (ok(l000) + ok(l001) + ok(l010) + ok(l100)) * live.long()
+    return int(inside.sum()), int(pairs.sum())
+
+
+def side_kernels(wl, steps: int, warmup: int, reps: int = 20) -> dict:
+    """The kernels around the march, on the workloads they serve.
+
+    train_256: the full training step on the 256^3 grid (about 8% occupied,
+    SH-2, float32): a 5000-ray fused forward+backward, TV of the density
+    (lambda 1e-5) and of the SH (lambda 1e-3) on a contiguous 1% of the cells,
+    then RMSprop on both (beta 0.95, lr 30 and 1e-2) over the touched rows,
+    through SparseGrid as a training script calls it; ms per step is host
+    wall time with the device synchronised, as a training loop sees it.
+
+    Kernel-only times (HIP events, launches repeated on fixed inputs) with
+    algorithmic bytes per launch against the 8 TB/s HBM peak:
+    - tv_grad_sparse: 16 B of links per cell, and per (cell, neighbour) pair
+      whose link addresses a row, per channel, a data read, an 8 B gradient
+      atomic and a 1 B mask store;
+    - rmsprop_step: 24 B per updated element (data, state and gradient, each
+      read and written) plus the indexer (8 B per row index);
+    - accel_dist_prop on the 512^3 links: two reads of the links, a write of
+      every empty link, and the pyramid cleared, then read and rewritten by
+      the node pass;
+    - sample (trilinear density + SH at 1M random points of the 512^3 grid):
+      per point 12 B in, 8 links, 8 rows of 112 B, 112 B out."""
+    import ctypes
+
+    import svox2_amd
+    from svox2_amd import _C, scenes
+    out = {}
+    sc, cams, _ = scenes.workload_blobs(256, scenes.SPARSE256, seed=1, device=wl.dev)
+    grid = scenes.to_sparse_grid(sc, device=wl.dev)
+    grid.opt.sigma_thresh, grid.opt.stop_thresh = 1e-8, 1e-7
+    gen = torch.Generator().manual_seed(5)
+    batches = []
+    for _ in range(POOL):
+        o, d = cams.random_batch(5000, gen)
+        batches.append((svox2_amd.Rays(o.to(wl.dev), d.to(wl.dev)),
+                        torch.rand(5000, 3, generator=gen).to(wl.dev)))
+    np.random.seed(0)  # the TV run's random start, drawn with np.random as the reference
+
+    def step(i):
+        rays, gt = batches[i % POOL]
+        grid.volume_render_fused(rays, gt)
+        grid.inplace_tv_grad(grid.density_data.grad, scaling=1e-5, sparse_frac=0.01)
+        grid.inplace_tv_color_grad(grid.sh_data.grad, scaling=1e-3, sparse_frac=0.01)
+        grid.optim_density_step(30.0, beta=0.95)
+        grid.optim_sh_step(1e-2, beta=0.95)
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(steps):
+        step(i)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    out["train_256"] = dict(ms_per_step=ms, rays_per_s=5000 / (ms * 1e-3), rows=int(grid.density_data.size(0)))
+
+    # Kernel-only: the inputs of one step, reused.
+    rays, gt = batches[0]
+    grid.volume_render_fused(rays, gt)
+    rows = grid.density_data.size(0)
+    cells = grid._get_rand_cells(0.01, contiguous=True)
+    mask = grid._get_sparse_grad_indexer()
+    gd, gs = grid.density_data.grad, grid.sh_data.grad
+    for name, data, grad, lo, hi, edge, lam in (
+            ("tv_density", grid.density_data, gd, 0, 1, False, 1e-5),
+            ("tv_sh", grid.sh_data, gs, 0, grid.sh_data.size(1), True, 1e-3)):
+        inside, pairs = _tv_pairs(grid.links, cells, rows, edge)
+        s_data = data.element_size()
+        nbytes = 16 * inside + (hi - lo) * pairs * (s_data + 8 + 1)
+        ms = _event_ms(lambda: _C.tv_grad_sparse(grid.links, data, cells, mask, lo, hi, lam,
+                                                 False, 2.0, edge, False, -1.0, -1.0, grad),
+                       reps)
+        out[name] = dict(_roof(ms, nbytes), cells=inside, pairs=pairs, channels=hi - lo)
+    grid._ensure_rms()
+    idx = torch.nonzero(mask.flatten(), as_tuple=False).flatten()
+    for name, data, rms, grad, lr in (
+            ("rmsprop_density", grid.density_data, grid.density_rms, gd, 30.0),
+            ("rmsprop_sh", grid.sh_data, grid.sh_rms, gs, 1e-2)):
+        dim = data.size(1)
+        nbytes = 24 * idx.numel() * dim + 8 * idx.numel()
+        ms = _event_ms(lambda: _C.rmsprop_step(data.data, rms, grad, idx, 0.95, lr, 1e-8,
+                                               -1e9, lr), reps)
+        out[name] = dict(_roof(ms, nbytes), rows_updated=int(idx.numel()), channels=dim)
+
+    g512 = wl.grid(torch.float32)
+    links = g512.links.clone()
+    n = links.numel()
+    empty = int((links < 0).sum())
+    size3 = (ctypes.c_int32 * 3)(*links.shape)
+    ws_bytes = _C.load_library().svox2_accel_dist_prop_workspace_bytes(size3)
+    nbytes = 8 * n + 4 * empty + 3 * ws_bytes
+    ms = _event_ms(lambda: _C.accel_dist_prop(links), 5)
+    out["accel_dist_prop_512"] = dict(_roof(ms, nbytes), cells=n, empty=empty)
+    assert torch.equal(links, g512.links)  # idempotent on accelerated links
+
+    pts = (torch.rand(1 << 20, 3, generator=torch.Generator().manual_seed(3)) * 2 - 1).to(wl.dev)
+    npts = pts.size(0)
+    with torch.no_grad():
+        ms = _event_ms(lambda: g512.sample(pts), reps)
+    out["sample_512_1m"] = dict(_roof(ms, npts * (12 + 32 + 8 * 112 + 112)), points=npts)
+    return out
+
+
 def image_forward(grid, cams, reps=5):
     cam = cams.camera(0)
     with torch.no_grad():
@@ -394,6 +541,14 @@ def main():
             log(f"serve {rt['qps']:.0f} QPS: batched p50 {b['p50_ms']:.2f} ms p99 "
                 f"{b['p99_ms']:.2f} ms {b['rays_per_s']:.3g} rays/s; one-per-launch p50 "
                 f"{p1['p50_ms']:.2f} ms p99 {p1['p99_ms']:.2f} ms {p1['rays_per_s']:.3g} rays/s")
+        log("256^3 training step and the kernels around the march")
+        extra["side"] = side_kernels(wl, args.steps, args.warmup)
+        sd = extra["side"]
+        log(f"256^3 training step (5000 rays, TV, RMSprop): {sd['train_256']['ms_per_step']:.3f} ms")
+        for k in ("tv_density", "tv_sh", "rmsprop_density", "rmsprop_sh", "accel_dist_prop_512",
+                  "sample_512_1m"):
+            log(f"{k}: {sd[k]['ms']:.3f} ms, {sd[k]['alg_gbps']:.0f} GB/s algorithmic "
+                f"({100 * sd[k]['frac']:.1f}% of HBM peak)")
         log("800x800 image forward")
         extra["image_800_fp32"] = image_forward(g32, wl.cams)
         extra["image_800_bf16"] = image_forward(gb, wl.cams)

@@ -1,11 +1,13 @@
 // Empty-space distance codes for the ray march (accel_dist_prop) on gfx950.
 //
 // Pass 1 marks, for every occupied cell, each ancestor of an occupancy
-// pyramid (sizes halved with ceiling while all three exceed 1); pass 2
-// rewrites every empty link to -1 - k, k = number of consecutive empty
-// ancestors.  The pyramid lives in the caller's workspace and is cleared by a
-// fill kernel on the same stream (no memset node, so the sequence can be
-// captured in a graph).
+// pyramid (sizes halved with ceiling while all three exceed 1); pass 2 turns
+// each level-0 node's flag into the number of empty levels from it up to its
+// first occupied ancestor; pass 3 rewrites every empty link to -1 - k, k =
+// number of consecutive empty ancestors, from its parent node's count (one
+// byte read per cell instead of a walk up the pyramid).  The pyramid lives in
+// the caller's workspace and is cleared by a fill kernel on the same stream
+// (no memset node, so the sequence can be captured in a graph).
 //
 // Semantics: accel_dist_set_kernel / accel_dist_prop_kernel
 // (svox2/csrc/misc_kernel.cu:109-181) and the host function (:356-416).
@@ -34,55 +36,122 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* __restrict__ p, 
     if (i < n_words) p[i] = 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void k_accel_set(const int32_t* __restrict__ links, int s0,
-                                                      int s1, int s2,
-                                                      uint8_t* __restrict__ tmp) {
-    const int64_t tid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (tid >= static_cast<int64_t>(s0) * s1 * s2) return;
-    int z = static_cast<int>(tid % s2);
-    const int64_t xy = tid / s2;
-    int y = static_cast<int>(xy % s1);
-    int x = static_cast<int>(xy / s1);
-    if (links[tid] < 0) return;
-    int sx = s0, sy = s1, sz = s2;
-    uint8_t* base = tmp;
-    while (sx > 1 && sy > 1 && sz > 1) {
-        x >>= 1;
-        y >>= 1;
-        z >>= 1;
+// The occupancy pyramid's levels: level k (k = 0 .. n - 1) has size
+// (.., sy[k], sz[k]) and starts at byte off[k] of the workspace.  Cells
+// number below 2^31, so the smallest axis is below 1291 and there are at
+// most 11 levels.
+constexpr int kMaxLevels = 16;
+struct Pyramid {
+    int n;
+    int sy[kMaxLevels];
+    int sz[kMaxLevels];
+    int64_t off[kMaxLevels];
+};
+
+Pyramid make_pyramid(int sx, int sy, int sz) {
+    Pyramid p{};
+    int64_t off = 0;
+    while (sx > 1 && sy > 1 && sz > 1 && p.n < kMaxLevels) {
         sx = half_ceil(sx);
         sy = half_ceil(sy);
         sz = half_ceil(sz);
-        base[(static_cast<int64_t>(x) * sy + y) * sz + z] = 1;
-        base += static_cast<int64_t>(sx) * sy * sz;
+        p.sy[p.n] = sy;
+        p.sz[p.n] = sz;
+        p.off[p.n] = off;
+        off += static_cast<int64_t>(sx) * sy * sz;
+        ++p.n;
+    }
+    return p;
+}
+
+// The set and prop passes: workgroups run over the grid's (x, y) rows
+// (blockIdx.x = x * s1 + y, one uniform division per workgroup) and
+// kBlock * kCells z cells each (blockIdx.y); a thread takes kCells cells
+// kBlock apart, so its loads of all of them are in flight together.
+constexpr int kCells = 8;
+
+__device__ __forceinline__ void row_cells(int s1, int& x, int& y, int& z0) {
+    const int row = static_cast<int>(blockIdx.x);
+    x = row / s1;
+    y = row - x * s1;
+    z0 = static_cast<int>(blockIdx.y) * (kBlock * kCells) + static_cast<int>(threadIdx.x);
+}
+
+// Index of the level-k ancestor (k >= 0) of cell (x, y, z) in the pyramid.
+__device__ __forceinline__ int64_t ancestor(const Pyramid& p, int k, int x, int y, int z) {
+    return p.off[k] + (static_cast<int64_t>(x >> (k + 1)) * p.sy[k] + (y >> (k + 1))) * p.sz[k] +
+           (z >> (k + 1));
+}
+
+// Pass 1: every occupied cell marks all its ancestors.
+__global__ __launch_bounds__(kBlock) void k_accel_set(const int32_t* __restrict__ links, int s1,
+                                                      int s2, Pyramid pyr,
+                                                      uint8_t* __restrict__ tmp) {
+    int x, y, z0;
+    row_cells(s1, x, y, z0);
+    const int64_t row = (static_cast<int64_t>(x) * s1 + y) * s2;
+    int32_t l[kCells];
+#pragma unroll
+    for (int j = 0; j < kCells; ++j) {
+        const int z = z0 + j * kBlock;
+        l[j] = z < s2 ? links[row + z] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kCells; ++j) {
+        const int z = z0 + j * kBlock;
+        if (l[j] < 0) continue;
+#pragma unroll
+        for (int k = 0; k < kMaxLevels; ++k)
+            if (k < pyr.n) tmp[ancestor(pyr, k, x, y, z)] = 1;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_accel_prop(int32_t* __restrict__ links, int s0,
-                                                       int s1, int s2,
-                                                       const uint8_t* __restrict__ tmp) {
-    const int64_t tid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (tid >= static_cast<int64_t>(s0) * s1 * s2) return;
-    if (links[tid] >= 0) return;
-    int z = static_cast<int>(tid % s2);
-    const int64_t xy = tid / s2;
-    int y = static_cast<int>(xy % s1);
-    int x = static_cast<int>(xy / s1);
-    int sx = s0, sy = s1, sz = s2;
-    const uint8_t* base = tmp;
-    int result = -1;
-    while (sx > 1 && sy > 1 && sz > 1) {
-        x >>= 1;
-        y >>= 1;
-        z >>= 1;
-        sx = half_ceil(sx);
-        sy = half_ceil(sy);
-        sz = half_ceil(sz);
-        if (base[(static_cast<int64_t>(x) * sy + y) * sz + z]) break;
-        result -= 1;
-        base += static_cast<int64_t>(sx) * sy * sz;
+// Pass 2, over the level-0 nodes: each node's byte becomes the number of
+// empty pyramid levels from it up to its first occupied ancestor (0 if it is
+// occupied, n if none is).  A node's byte is read and written by its own
+// thread only; the flags of levels 1 and up are read together.
+__global__ __launch_bounds__(kBlock) void k_accel_node(Pyramid pyr, uint8_t* __restrict__ tmp) {
+    const int row = static_cast<int>(blockIdx.x);
+    const int a = row / pyr.sy[0];
+    const int b = row - a * pyr.sy[0];
+    const int c = static_cast<int>(blockIdx.y) * kBlock + static_cast<int>(threadIdx.x);
+    if (c >= pyr.sz[0]) return;
+    const int64_t self = pyr.off[0] + (static_cast<int64_t>(a) * pyr.sy[0] + b) * pyr.sz[0] + c;
+    const bool own = tmp[self] != 0;
+    uint32_t occ = 0u;
+#pragma unroll
+    for (int k = 1; k < kMaxLevels; ++k) {
+        if (k < pyr.n) {
+            const int64_t i = pyr.off[k] +
+                              (static_cast<int64_t>(a >> k) * pyr.sy[k] + (b >> k)) * pyr.sz[k] +
+                              (c >> k);
+            occ |= (tmp[i] != 0 ? 1u : 0u) << k;
+        }
     }
-    links[tid] = result;
+    tmp[self] = static_cast<uint8_t>(own ? 0 : (occ ? __ffs(static_cast<int>(occ)) - 1 : pyr.n));
+}
+
+// Pass 3: every empty link becomes -1 - (its level-0 parent's byte): -1 - k
+// for k consecutive empty ancestors (-1 with no pyramid).
+__global__ __launch_bounds__(kBlock) void k_accel_prop(int32_t* __restrict__ links, int s1,
+                                                       int s2, Pyramid pyr,
+                                                       const uint8_t* __restrict__ tmp) {
+    int x, y, z0;
+    row_cells(s1, x, y, z0);
+    const int64_t row = (static_cast<int64_t>(x) * s1 + y) * s2;
+    int32_t l[kCells];
+    int e[kCells];
+#pragma unroll
+    for (int j = 0; j < kCells; ++j) {
+        const int z = z0 + j * kBlock;
+        l[j] = z < s2 ? links[row + z] : 0;
+        e[j] = (z < s2 && pyr.n > 0) ? tmp[ancestor(pyr, 0, x, y, z)] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kCells; ++j) {
+        const int z = z0 + j * kBlock;
+        if (z < s2 && l[j] < 0) links[row + z] = -1 - e[j];
+    }
 }
 
 }  // namespace
@@ -114,9 +183,20 @@ extern "C" int svox2_accel_dist_prop(int32_t* links, const int32_t size[3], void
         k_fill_zero<<<blocks_for(words, kBlock), kBlock, 0, s>>>(
             static_cast<uint32_t*>(workspace), words);
     }
-    const unsigned blocks = blocks_for(cells, kBlock);
-    k_accel_set<<<blocks, kBlock, 0, s>>>(links, size[0], size[1], size[2], tmp);
-    k_accel_prop<<<blocks, kBlock, 0, s>>>(links, size[0], size[1], size[2], tmp);
+    const Pyramid pyr = make_pyramid(size[0], size[1], size[2]);
+    // Fewer than 2^31 cells give at most 11 levels.
+    SVOX2_REQUIRE(pyr.n < kMaxLevels, "grid too large for the occupancy pyramid");
+    SVOX2_REQUIRE(size[2] <= (1 << 24), "grid size[2] = %d above 2^24", size[2]);
+    const dim3 grid(static_cast<unsigned>(static_cast<int64_t>(size[0]) * size[1]),
+                    blocks_for(size[2], kBlock * kCells));
+    k_accel_set<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
+    if (pyr.n > 0) {
+        const dim3 nodes(static_cast<unsigned>(static_cast<int64_t>(half_ceil(size[0])) *
+                                               pyr.sy[0]),
+                         blocks_for(pyr.sz[0], kBlock));
+        k_accel_node<<<nodes, kBlock, 0, s>>>(pyr, tmp);
+    }
+    k_accel_prop<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
     return check_launch("accel_dist_prop");
 }
 

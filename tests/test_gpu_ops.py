@@ -21,10 +21,17 @@ def small_sparse(gpu):
 
 
 def test_accel_dist_prop_bit_exact(gpu, small_sparse):
+    """Bit-exact against the oracle (itself checked against brute-force
+    pyramids) on the blob grid and on random links of odd and degenerate
+    shapes: a unit axis (no pyramid level), a pyramid of one 2x2x2 level,
+    long thin grids and sizes that are not powers of two."""
     sc, _ = small_sparse
-    for links in (sc.links, torch.from_numpy(
-            np.where(np.random.default_rng(0).random((13, 9, 20)) < 0.03,
-                     np.arange(13 * 9 * 20).reshape(13, 9, 20), -1).astype(np.int32))):
+    rng = np.random.default_rng(0)
+    shapes = [(13, 9, 20), (1, 5, 7), (2, 2, 2), (3, 300, 2), (70, 65, 129), (33, 17, 2)]
+    randoms = [torch.from_numpy(np.where(rng.random(sh) < p, np.arange(np.prod(sh)).reshape(sh),
+                                         -1).astype(np.int32))
+               for sh, p in zip(shapes, (0.03, 0.2, 0.1, 0.01, 0.002, 0.05))]
+    for links in [sc.links] + randoms:
         ref = O.accel_dist_prop(links.numpy())
         t = links.clone().to(gpu)
         _C.accel_dist_prop(t)
