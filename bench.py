@@ -7,8 +7,11 @@ density_data.grad / sh_data.grad + touched-row mask) over one batch of random
 camera rays already resident in HBM.  Prints one JSON line with the
 throughput, a roofline entry for the fused kernel (algorithmic bytes per
 launch / average launch time from HIP events on the launch stream, against the
-8 TB/s HBM peak), side measurements (BF16 storage, 5,000-ray batches, an
-800x800 forward image) and a CPU baseline: the scalar C port of the
+8 TB/s HBM peak), side measurements (BF16 storage, 5,000-ray batches, the
+million-ray autograd step, the serving loop, the 256^3 training step with
+the TV, RMSprop, accel_dist_prop and sample kernels against their own
+algorithmic bytes, an 800x800 forward image) and a CPU baseline: the scalar
+C port of the
 reference's fused forward+backward (oracle/march_ref.c) timed on one host core
 on a bounded sample of the same batch.
 
