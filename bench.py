@@ -11,9 +11,8 @@ launch / average launch time from HIP events on the launch stream, against the
 million-ray autograd step, the serving loop, the 256^3 training step with
 the TV, RMSprop, accel_dist_prop and sample kernels against their own
 algorithmic bytes, an 800x800 forward image) and a CPU baseline: the scalar
-C port of the
-reference's fused forward+backward (oracle/march_ref.c) timed on one host core
-on a bounded sample of the same batch.
+C port of the reference's fused forward+backward (oracle/march_ref.c) timed on
+one host core on a bounded sample of the same batch.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
