@@ -159,6 +159,65 @@ def test_training_step_facade(gpu):
     assert moved_s.any() and not (moved_s & ~grid.sparse_sh_grad_indexer).any()
 
 
+def test_training_step_values(gpu, march):
+    """The training step's values against the oracle's composition of the
+    reference loop (opt/opt.py:492-603) on a 64^3 sparse grid.
+
+    - The fused gradients plus both TV terms (the same contiguous cell runs,
+      drawn from np.random as the reference does) are compared with the FP64
+      fused gradients plus the oracle's TV, to 1e-3 of their max.
+    - The touched-row masks are compared with the fused mask or-ed with the
+      TV masks.
+    - Then one RMSprop step per quantity (lr 30 and 1e-2, beta 0.95, fresh
+      state) is compared with the oracle's closed form applied to the GPU's
+      own gradients on those rows."""
+    gen = torch.Generator().manual_seed(4)
+    sc = scenes.make_blob_grid(64, scenes.SPARSE256, gen, device=gpu)
+    cams = scenes.CameraSet(scenes.hemisphere_cameras(10, gen))
+    o, d = cams.random_batch(2000, gen)
+    gt = torch.rand(2000, 3, generator=gen)
+    grid = _grid(sc, gpu)
+    links = grid.links.cpu().numpy()
+    g = host_grid(sc, links=links)
+    dens0 = grid.density_data.data.cpu().double().numpy().copy()
+    sh0 = grid.sh_data.data.cpu().double().numpy().copy()
+    grid.volume_render_fused(svox2_amd.Rays(o.to(gpu), d.to(gpu)), gt.to(gpu))
+    np.random.seed(3)
+    grid.inplace_tv_grad(grid.density_data.grad, scaling=1e-5, sparse_frac=0.01)
+    grid.inplace_tv_color_grad(grid.sh_data.grad, scaling=1e-3, sparse_frac=0.01)
+    gd = grid.density_data.grad.cpu().double().numpy()
+    gs = grid.sh_data.grad.cpu().double().numpy()
+    mask_d = grid.sparse_grad_indexer.cpu().numpy().copy()
+    mask_s = grid.sparse_sh_grad_indexer.cpu().numpy().copy()
+
+    cells_total = links.size
+    num = max(int(0.01 * cells_total), 1)
+    np.random.seed(3)
+    start_d = np.random.randint(0, cells_total)
+    start_s = np.random.randint(0, cells_total)
+    run = lambda st: (np.arange(st, st + num) % cells_total).astype(np.int64)
+    smp = march.samples(g, o.numpy(), d.numpy(), OPT)
+    _, gd64, gs64 = O.mse_grads_fp64(g, o.numpy(), d.numpy(), OPT, gt.numpy(), samples=smp)
+    tv_d, m_d = O.tv_grad_sparse(links, np.asarray(g.density).reshape(-1, 1), run(start_d), 0, 1,
+                                 1e-5, False, False)
+    tv_s, m_s = O.tv_grad_sparse(links, g.sh, run(start_s), 0, g.sh.shape[1], 1e-3, True, False)
+    assert _rel(gd, gd64.reshape(-1, 1) + tv_d) < GRAD_RTOL
+    assert _rel(gs, gs64 + tv_s) < GRAD_RTOL
+    fused_mask = O.touched_rows(g, smp)
+    assert np.array_equal(mask_d, fused_mask | m_d)
+    assert np.array_equal(mask_s, fused_mask | m_s)
+
+    grid.optim_density_step(30.0, beta=0.95)
+    grid.optim_sh_step(1e-2, beta=0.95)
+    for data0, grad, mask, lr, new in (
+            (dens0, gd, mask_d, 30.0, grid.density_data.data),
+            (sh0, gs, mask_s, 1e-2, grid.sh_data.data)):
+        want, _, _ = O.rmsprop_step(data0, np.zeros_like(data0), grad, np.nonzero(mask)[0],
+                                    0.95, lr, 1e-8, -1e9, lr)
+        got = new.cpu().double().numpy()
+        assert np.abs(got - want).max() <= 1e-5 * max(1.0, np.abs(want).max())
+
+
 def test_fused_ray_order(gpu, dense64):
     """The fused call's ray orders on a 65536-ray batch (above the ordering
     threshold of an MI355X, 40960 rays): with a caller-owned workspace the
