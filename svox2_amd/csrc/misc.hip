@@ -154,6 +154,100 @@ __global__ __launch_bounds__(kBlock) void k_accel_prop(int32_t* __restrict__ lin
     }
 }
 
+// Vector forms of passes 1 and 3 for grids whose rows (z) hold a multiple of
+// 4 cells: a thread takes 4 consecutive cells of one row with one 16-byte
+// load (and, in pass 3, one 16-byte store; occupied links are stored back
+// unchanged), kVec such groups kBlock * 4 cells apart over the flat cell
+// index.  The 4 cells' level-0 parents are 2 adjacent bytes, read as one
+// 16-bit word.  The streaming passes then move 16 bytes per lane.
+constexpr int kVec = 4;
+
+struct CellDiv {
+    uint32_t s2, s1;
+    double inv2, inv1;  // 1.0 / s2, 1.0 / s1
+};
+
+// n / d for n < 2^32 and 1 <= d < 2^31, inv = 1 / d in double: the product
+// lies within 2^-20 of n / d, so its truncation is off by at most one, which
+// the remainder test corrects.
+__device__ __forceinline__ uint32_t div_u32(uint32_t n, uint32_t d, double inv) {
+    int64_t q = static_cast<int64_t>(static_cast<uint32_t>(static_cast<double>(n) * inv));
+    const int64_t r = static_cast<int64_t>(n) - q * static_cast<int64_t>(d);
+    q += (r >= static_cast<int64_t>(d)) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return static_cast<uint32_t>(q);
+}
+
+__device__ __forceinline__ void cell_xyz(uint32_t cell, const CellDiv& cd, int& x, int& y,
+                                         int& z) {
+    const uint32_t row = div_u32(cell, cd.s2, cd.inv2);
+    z = static_cast<int>(cell - row * cd.s2);
+    const uint32_t xq = div_u32(row, cd.s1, cd.inv1);
+    x = static_cast<int>(xq);
+    y = static_cast<int>(row - xq * cd.s1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_accel_set4(const int4* __restrict__ links, int64_t n4,
+                                                       CellDiv cd, Pyramid pyr,
+                                                       uint8_t* __restrict__ tmp) {
+    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * (kBlock * kVec) + threadIdx.x;
+    int4 l[kVec];
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+        const int64_t v = v0 + j * kBlock;
+        l[j] = v < n4 ? links[v] : make_int4(-1, -1, -1, -1);
+    }
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+        const int c[4] = {l[j].x, l[j].y, l[j].z, l[j].w};
+        if (max(max(c[0], c[1]), max(c[2], c[3])) < 0) continue;
+        int x, y, z;
+        cell_xyz(static_cast<uint32_t>(4 * (v0 + j * kBlock)), cd, x, y, z);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c[q] < 0) continue;
+#pragma unroll
+            for (int k = 0; k < kMaxLevels; ++k)
+                if (k < pyr.n) tmp[ancestor(pyr, k, x, y, z + q)] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_accel_prop4(int4* __restrict__ links, int64_t n4,
+                                                        CellDiv cd, Pyramid pyr,
+                                                        const uint8_t* __restrict__ tmp) {
+    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * (kBlock * kVec) + threadIdx.x;
+    int4 l[kVec];
+    uint32_t e[kVec];
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+        const int64_t v = v0 + j * kBlock;
+        l[j] = make_int4(0, 0, 0, 0);
+        e[j] = 0u;
+        if (v < n4) {
+            l[j] = links[v];
+            if (pyr.n > 0) {
+                int x, y, z;
+                cell_xyz(static_cast<uint32_t>(4 * v), cd, x, y, z);
+                e[j] = *reinterpret_cast<const uint16_t*>(tmp + ancestor(pyr, 0, x, y, z));
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+        const int64_t v = v0 + j * kBlock;
+        if (v >= n4) continue;
+        const int e0 = -1 - static_cast<int>(e[j] & 0xFFu);  // parent of cells 0 and 1
+        const int e1 = -1 - static_cast<int>(e[j] >> 8);     // parent of cells 2 and 3
+        int4 out;
+        out.x = l[j].x < 0 ? e0 : l[j].x;
+        out.y = l[j].y < 0 ? e0 : l[j].y;
+        out.z = l[j].z < 0 ? e1 : l[j].z;
+        out.w = l[j].w < 0 ? e1 : l[j].w;
+        links[v] = out;
+    }
+}
+
 }  // namespace
 }  // namespace svox2amd
 
@@ -187,16 +281,32 @@ extern "C" int svox2_accel_dist_prop(int32_t* links, const int32_t size[3], void
     // Fewer than 2^31 cells give at most 11 levels.
     SVOX2_REQUIRE(pyr.n < kMaxLevels, "grid too large for the occupancy pyramid");
     SVOX2_REQUIRE(size[2] <= (1 << 24), "grid size[2] = %d above 2^24", size[2]);
+    // Rows of a multiple of 4 cells (and 16-byte aligned links) take the
+    // vector passes; the level-0 parents of 4 such cells then sit at an even
+    // byte offset of the pyramid.
+    const bool vec = size[2] % 4 == 0 && reinterpret_cast<uintptr_t>(links) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(tmp) % 2 == 0;
     const dim3 grid(static_cast<unsigned>(static_cast<int64_t>(size[0]) * size[1]),
                     blocks_for(size[2], kBlock * kCells));
-    k_accel_set<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
+    const int64_t n4 = cells / 4;
+    const CellDiv cd{static_cast<uint32_t>(size[2]), static_cast<uint32_t>(size[1]),
+                     1.0 / size[2], 1.0 / size[1]};
+    const unsigned vblocks = blocks_for(n4, kBlock * kVec);
+    if (vec)
+        k_accel_set4<<<vblocks, kBlock, 0, s>>>(reinterpret_cast<const int4*>(links), n4, cd, pyr,
+                                                tmp);
+    else
+        k_accel_set<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
     if (pyr.n > 0) {
         const dim3 nodes(static_cast<unsigned>(static_cast<int64_t>(half_ceil(size[0])) *
                                                pyr.sy[0]),
                          blocks_for(pyr.sz[0], kBlock));
         k_accel_node<<<nodes, kBlock, 0, s>>>(pyr, tmp);
     }
-    k_accel_prop<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
+    if (vec)
+        k_accel_prop4<<<vblocks, kBlock, 0, s>>>(reinterpret_cast<int4*>(links), n4, cd, pyr, tmp);
+    else
+        k_accel_prop<<<grid, kBlock, 0, s>>>(links, size[1], size[2], pyr, tmp);
     return check_launch("accel_dist_prop");
 }
 
