@@ -222,6 +222,26 @@ def _tv_pairs(links: torch.Tensor, cells: torch.Tensor, rows: int, ignore_edge: 
     return int(inside.sum()), int(pairs.sum())
 
 
+def _valid_corners(grid, pts: torch.Tensor) -> int:
+    """Corners, over all points, whose link addresses a row: the trilinear
+    cell of each point (grid coordinates clamped to the grid, lower corner at
+    most size - 2 per axis) and its 8 corners."""
+    n = torch.tensor(grid.links.shape, device=pts.device)
+    g = grid.world2grid(pts).clamp(min=0)
+    g = torch.minimum(g, (n - 1).to(g.dtype))
+    lo = torch.minimum(g.floor().long(), n - 2)
+    flat = grid.links.reshape(-1).long()
+    rows = grid.density_data.size(0)
+    total = 0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                i = ((lo[:, 0] + dx) * n[1] + lo[:, 1] + dy) * n[2] + lo[:, 2] + dz
+                lk = flat[i]
+                total += int(((lk >= 0) & (lk < rows)).sum())
+    return total
+
+
 def side_kernels(wl, steps: int, warmup: int, reps: int = 20) -> dict:
     """The kernels around the march, on the workloads they serve.
 
@@ -243,7 +263,10 @@ def side_kernels(wl, steps: int, warmup: int, reps: int = 20) -> dict:
       every empty link, and the pyramid cleared, then read and rewritten by
       the node pass;
     - sample (trilinear density + SH at 1M random points of the 512^3 grid):
-      per point 12 B in, 8 links, 8 rows of 112 B, 112 B out."""
+      per point 12 B in, 8 links and 112 B out, plus a 112 B row per corner
+      that has one; its backward reads the point, 8 links and 112 B of
+      output gradient and adds into the gradient row of each corner that has
+      one (28 float atomics of 8 B)."""
     import ctypes
 
     import svox2_amd
@@ -319,9 +342,19 @@ def side_kernels(wl, steps: int, warmup: int, reps: int = 20) -> dict:
 
     pts = (torch.rand(1 << 20, 3, generator=torch.Generator().manual_seed(3)) * 2 - 1).to(wl.dev)
     npts = pts.size(0)
+    corners = _valid_corners(g512, pts)
     with torch.no_grad():
         ms = _event_ms(lambda: g512.sample(pts), reps)
-    out["sample_512_1m"] = dict(_roof(ms, npts * (12 + 32 + 8 * 112 + 112)), points=npts)
+    out["sample_512_1m"] = dict(_roof(ms, npts * (12 + 32 + 112) + corners * 112), points=npts,
+                                corners_with_rows=corners)
+    gd, gs, _, _ = g512._get_data_grads()
+    gen_g = torch.Generator().manual_seed(4)
+    god = torch.randn(npts, 1, generator=gen_g).to(wl.dev)
+    gos = torch.randn(npts, g512.sh_data.size(1), generator=gen_g).to(wl.dev)
+    spec = g512._to_cpp()
+    ms = _event_ms(lambda: _C.sample_grid_backward(spec, pts, god, gos, gd, gs, True), reps)
+    out["sample_backward_512_1m"] = dict(_roof(ms, npts * (12 + 32 + 112) + corners * 28 * 8),
+                                         points=npts, corners_with_rows=corners)
     return out
 
 
@@ -548,7 +581,7 @@ def main():
         sd = extra["side"]
         log(f"256^3 training step (5000 rays, TV, RMSprop): {sd['train_256']['ms_per_step']:.3f} ms")
         for k in ("tv_density", "tv_sh", "rmsprop_density", "rmsprop_sh", "accel_dist_prop_512",
-                  "sample_512_1m"):
+                  "sample_512_1m", "sample_backward_512_1m"):
             log(f"{k}: {sd[k]['ms']:.3f} ms, {sd[k]['alg_gbps']:.0f} GB/s algorithmic "
                 f"({100 * sd[k]['frac']:.1f}% of HBM peak)")
         log("800x800 image forward")
