@@ -109,7 +109,7 @@ typedef struct svox2_grads {
  * final log transmittance per ray.  ray_cost_out, if not NULL, receives each
  * ray's march chunk count, which svox2_volume_render_cuvol_backward can use
  * to order its rays.
- * Replaces volume_render_cuvol (svox2.cpp:28; render_lerp_kernel_cuvol.cu:918). */
+ * Replaces volume_render_cuvol (svox2.cpp:17, bound at :78; render_lerp_kernel_cuvol.cu:918). */
 int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
                               const float* dirs, const float* ray_clip,
                               int64_t n_rays, const svox2_render_options_t* opt,
@@ -121,7 +121,7 @@ int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
  * times the rays the device holds resident (40960 on a 256-CU MI355X) are
  * processed in descending order of each ray's segment length in the grid
  * (same results); smaller batches, or a NULL workspace, in input order.
- * Replaces volume_render_cuvol (svox2.cpp:28; render_lerp_kernel_cuvol.cu:918)
+ * Replaces volume_render_cuvol (svox2.cpp:17, bound at :78; render_lerp_kernel_cuvol.cu:918)
  * for callers that can lend the workspace (the autograd forward). */
 int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const float* origins,
                                       const float* dirs, const float* ray_clip,
@@ -131,7 +131,7 @@ int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const float* ori
                                       size_t workspace_bytes, void* stream);
 
 /* Forward render of a pinhole image; rgb_out [height][width][3].
- * Replaces volume_render_cuvol_image (svox2.cpp:31; render_lerp_kernel_cuvol.cu:960). */
+ * Replaces volume_render_cuvol_image (svox2.cpp:18, bound at :79; render_lerp_kernel_cuvol.cu:960). */
 int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
                                     const svox2_camera_t* cam,
                                     const svox2_render_options_t* opt,
@@ -144,7 +144,7 @@ int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
  * svox2_ray_order_workspace_bytes(n_rays) bytes, batches of at least four
  * times the rays the device holds resident are processed in descending order
  * of cost (same results); otherwise, or with NULLs, in input order.
- * Replaces volume_render_cuvol_backward (svox2.cpp:33; render_lerp_kernel_cuvol.cu:1010). */
+ * Replaces volume_render_cuvol_backward (svox2.cpp:20, bound at :80; render_lerp_kernel_cuvol.cu:1010). */
 int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                        const float* origins, const float* dirs,
                                        const float* ray_clip, int64_t n_rays,
@@ -176,7 +176,7 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  *   for batches at or above it the forward also logs where it placed the
  *   samples of each ray's first 16 chunks, and the backward evaluates those
  *   chunks from the log instead of placing them again.
- * Replaces volume_render_cuvol_fused (svox2.cpp:39; render_lerp_kernel_cuvol.cu:1070). */
+ * Replaces volume_render_cuvol_fused (svox2.cpp:22, bound at :81; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
                                     const float* ray_clip, int64_t n_rays,
@@ -241,14 +241,14 @@ int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid,
 
 /* Trilinear samples of density [n][1] and, if want_colors, sh [n][sh_data_dim]
  * at points (world coordinates through grid->offset/scaling).
- * Replaces sample_grid (svox2.cpp:57; svox2_kernel.cu:181). */
+ * Replaces sample_grid (svox2.cpp:11, bound at :76; svox2_kernel.cu:181). */
 int svox2_sample_grid(const svox2_grid_t* grid, const float* points,
                       int64_t n_points, int32_t want_colors,
                       float* density_out, float* sh_out, void* stream);
 
 /* Adjoint of svox2_sample_grid; accumulates into grad_density_out / grad_sh_out
  * (row strides in elements, 0 = contiguous).
- * Replaces sample_grid_backward (svox2.cpp:59; svox2_kernel.cu:219). */
+ * Replaces sample_grid_backward (svox2.cpp:13, bound at :77; svox2_kernel.cu:219). */
 int svox2_sample_grid_backward(const svox2_grid_t* grid, const float* points,
                                int64_t n_points,
                                const float* grad_out_density,
@@ -264,7 +264,7 @@ int svox2_sample_grid_backward(const svox2_grid_t* grid, const float* points,
  * contiguous), scaled by
  * scale / n_cells and accumulated into grad_data; mask_out (NULL allowed) is
  * set for every row that receives a gradient.
- * Replaces tv_grad_sparse (svox2.cpp:45; loss_kernel.cu:558). */
+ * Replaces tv_grad_sparse (svox2.cpp:63, bound at :99; loss_kernel.cu:558). */
 int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
                          const void* data, int32_t storage, int64_t rows,
                          int32_t data_dim, int64_t data_stride,
@@ -278,8 +278,9 @@ int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
 /* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
  * rms and grad; grad is zeroed where updated.  Row strides (elements, 0 =
  * dim) apply to data, rms and grad separately.  indexer_kind selects every row,
- * a byte mask (n_index = rows) or int64 row indices (n_index entries).
- * Replaces rmsprop_step (svox2.cpp:66; optim_kernel.cu:154). */
+ * a byte mask (n_index = rows) or int64 row indices (n_index entries; entries
+ * outside [0, rows) are skipped).
+ * Replaces rmsprop_step (svox2.cpp:70, bound at :110; optim_kernel.cu:154). */
 int svox2_rmsprop_step(void* data, int32_t storage, float* rms, float* grad,
                        int64_t rows, int32_t dim, int64_t data_stride,
                        int64_t rms_stride, int64_t grad_stride,
@@ -288,12 +289,34 @@ int svox2_rmsprop_step(void* data, int32_t storage, float* rms, float* grad,
                        float lr, float epsilon, float minval, float lr_last,
                        void* stream);
 
+/* SGD update data -= lr * grad (lr_last for the last column), grad zeroed
+ * where applied, over the rows the indexer selects (as svox2_rmsprop_step).
+ * Replaces sgd_step (svox2.cpp:72, bound at :111; optim_kernel.cu:218). */
+int svox2_sgd_step(void* data, int32_t storage, float* grad, int64_t rows,
+                   int32_t dim, int64_t data_stride, int64_t grad_stride,
+                   int32_t indexer_kind, const void* indexer, int64_t n_index,
+                   float lr, float lr_last, void* stream);
+
+/* RMSprop over rows of ncols columns split into two groups, [0, split) and
+ * [split, ncols), each with its own byte mask (one per row), beta and
+ * learning rate; both masks NULL updates every row.  With the interleaved
+ * voxel row [SH | density] (ncols = sh_data_dim + 1, split = sh_data_dim)
+ * this is optim_sh_step followed by optim_density_step in one pass over the
+ * rows.  No counterpart in the reference: it fuses the two rmsprop_step
+ * calls of SparseGrid.optim_sh_step / optim_density_step (svox2.py:1972-2041). */
+int svox2_rmsprop_step_grouped(void* data, int32_t storage, float* rms, float* grad,
+                               int64_t rows, int32_t ncols, int64_t data_stride,
+                               int64_t rms_stride, int64_t grad_stride, int32_t split,
+                               const uint8_t* mask_a, const uint8_t* mask_b,
+                               float beta_a, float lr_a, float beta_b, float lr_b,
+                               float epsilon, float minval, void* stream);
+
 /* Bytes of scratch svox2_accel_dist_prop needs for a links table of this size. */
 size_t svox2_accel_dist_prop_workspace_bytes(const int32_t size[3]);
 
 /* Rewrites every empty link in place to -1 - k, k = number of consecutive
  * empty ancestor levels of the occupancy pyramid.
- * Replaces accel_dist_prop (svox2.cpp:71; misc_kernel.cu:356). */
+ * Replaces accel_dist_prop (svox2.cpp:54, bound at :105; misc_kernel.cu:356). */
 int svox2_accel_dist_prop(int32_t* links, const int32_t size[3],
                           void* workspace, size_t workspace_bytes,
                           void* stream);

@@ -20,7 +20,8 @@ reference lines it follows:
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
   (loss_kernel.cu:178-246,558-604), ``rmsprop_step`` (optim_kernel.cu:15-93),
-  ``accel_dist_prop`` (misc_kernel.cu:109-181,356-416).
+  ``sgd_step`` (optim_kernel.cu:97-147), ``accel_dist_prop``
+  (misc_kernel.cu:109-181,356-416).
 
 Parity pins (tests/test_oracle_*.py): closed-form compositing of uniform
 fields, background-only cases, F.grid_sample, exactness on linear fields, the
@@ -716,6 +717,19 @@ def rmsprop_step(data, rms, grad, rows_sel, beta, lr, eps, minval, lr_last):
         data[r] = np.maximum(data[r] - lrs * g / (np.sqrt(rms[r]) + eps), minval)
         grad[r] = 0.0
     return data, rms, grad
+
+
+def sgd_step(data, grad, rows_sel, lr, lr_last):
+    """Closed form per element (optim_kernel.cu:97-103, lr_last on the last
+    column as :111-115): data - lr * g on the selected rows, g zeroed there;
+    returns new (data, grad) as float64 arrays."""
+    data, grad = (np.array(a, np.float64) for a in (data, grad))
+    lrs = np.full(data.shape[1], lr, np.float64)
+    lrs[-1] = lr_last
+    rows_sel = np.asarray(rows_sel, np.int64)
+    data[rows_sel] -= lrs * grad[rows_sel]
+    grad[rows_sel] = 0.0
+    return data, grad
 
 
 def accel_dist_prop(links: np.ndarray) -> np.ndarray:

@@ -25,7 +25,7 @@ __all__ = [
     "SparseGridSpec", "RaysSpec", "CameraSpec", "RenderOptions", "GridOutputGrads",
     "volume_render_cuvol", "volume_render_cuvol_image", "volume_render_cuvol_backward",
     "volume_render_cuvol_fused", "volume_render_cuvol_stats", "tv_grad_sparse",
-    "rmsprop_step", "sample_grid", "sample_grid_backward", "accel_dist_prop",
+    "rmsprop_step", "sgd_step", "rmsprop_step_grouped", "sample_grid", "sample_grid_backward", "accel_dist_prop",
     "library_path", "load_library", "EXPORTED_SYMBOLS",
 ]
 
@@ -47,7 +47,8 @@ EXPORTED_SYMBOLS = (
     "svox2_cell_map_bytes",
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
-    "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_accel_dist_prop_workspace_bytes",
+    "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
+    "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
 )
 
@@ -149,6 +150,9 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
                                  vp, i32, i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
         "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
                                f32, f32, f32, vp],
+        "svox2_sgd_step": [vp, i32, vp, i64, i32, i64, i64, i32, vp, i64, f32, f32, vp],
+        "svox2_rmsprop_step_grouped": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, vp,
+                                       f32, f32, f32, f32, f32, f32, vp],
         "svox2_accel_dist_prop": [vp, ctypes.POINTER(i32), vp, ctypes.c_size_t, vp],
         "svox2_build_cell_map": [vp, ctypes.POINTER(i32), i64, vp, ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_tiles": [vp, i32, i32, i32, vp, i32, O, vp, vp, i32, i32,
@@ -683,6 +687,23 @@ def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Te
               float(ndc_coeffx), float(ndc_coeffy), _ptr(grad_data), _stream(data))
 
 
+def _indexer_args(indexer: torch.Tensor, rows: int):
+    """(kind, pointer, length) of the reference's indexer: a 0-dim tensor
+    selects every row, a bool/uint8 (rows,) tensor is a row mask, an int64
+    tensor lists rows; None for an empty 1-d indexer (nothing to update)."""
+    if indexer.dim() == 0:
+        return INDEXER_ALL, None, 0
+    if indexer.numel() == 0:
+        return None
+    if indexer.dtype in (torch.bool, torch.uint8):
+        _check(indexer, "indexer", (torch.bool, torch.uint8), 1)
+        if indexer.numel() != rows:
+            raise RuntimeError("a mask indexer must have one entry per row")
+        return INDEXER_MASK, _ptr(indexer), indexer.numel()
+    _check(indexer, "indexer", (torch.int64,), 1)
+    return INDEXER_INDEX, _ptr(indexer), indexer.numel()
+
+
 def rmsprop_step(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor,
                  indexer: torch.Tensor, beta: float, lr: float, epsilon: float, minval: float,
                  lr_last: float) -> None:
@@ -695,21 +716,61 @@ def rmsprop_step(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor,
     if rms.shape != data.shape or grad.shape != data.shape:
         raise RuntimeError("rms / grad must match data's shape")
     rows, dim = data.shape
-    if indexer.dim() == 0:
-        kind, iptr, n = INDEXER_ALL, None, 0
-    elif indexer.numel() == 0:
+    sel = _indexer_args(indexer, rows)
+    if sel is None:
         return
-    elif indexer.dtype in (torch.bool, torch.uint8):
-        _check(indexer, "indexer", (torch.bool, torch.uint8), 1)
-        kind, iptr, n = INDEXER_MASK, _ptr(indexer), indexer.numel()
-    else:
-        _check(indexer, "indexer", (torch.int64,), 1)
-        kind, iptr, n = INDEXER_INDEX, _ptr(indexer), indexer.numel()
+    kind, iptr, n = sel
     with _on_device(data):
         _call("svox2_rmsprop_step", _ptr(data), _storage_of(data), _ptr(rms), _ptr(grad), rows,
               dim, _row_stride(data), _row_stride(rms), _row_stride(grad), kind, iptr, n,
               float(beta), float(lr), float(epsilon), float(minval), float(lr_last),
               _stream(data))
+
+
+def sgd_step(data: torch.Tensor, grad: torch.Tensor, indexer: torch.Tensor, lr: float,
+             lr_last: float) -> None:
+    """sgd_step (optim_kernel.cu:218-266): data -= lr * grad over the indexed
+    rows (lr_last on the last column), grad zeroed there."""
+    _check_rows(data, "data", _DATA_DTYPES)
+    _check_rows(grad, "grad", (torch.float32,))
+    if grad.shape != data.shape:
+        raise RuntimeError("grad must match data's shape")
+    rows, dim = data.shape
+    sel = _indexer_args(indexer, rows)
+    if sel is None:
+        return
+    kind, iptr, n = sel
+    with _on_device(data):
+        _call("svox2_sgd_step", _ptr(data), _storage_of(data), _ptr(grad), rows, dim,
+              _row_stride(data), _row_stride(grad), kind, iptr, n, float(lr), float(lr_last),
+              _stream(data))
+
+
+def rmsprop_step_grouped(data: torch.Tensor, rms: torch.Tensor, grad: torch.Tensor, split: int,
+                         mask_a: Optional[torch.Tensor], mask_b: Optional[torch.Tensor],
+                         beta_a: float, lr_a: float, beta_b: float, lr_b: float,
+                         epsilon: float, minval: float = -1e9) -> None:
+    """svox2_rmsprop_step_grouped: RMSprop of columns [0, split) with
+    (mask_a, beta_a, lr_a) and [split, ncols) with (mask_b, beta_b, lr_b) in
+    one pass; masks None = every row.  No reference counterpart."""
+    _check_rows(data, "data", _DATA_DTYPES)
+    _check_rows(rms, "rms", (torch.float32,))
+    _check_rows(grad, "grad", (torch.float32,))
+    if rms.shape != data.shape or grad.shape != data.shape:
+        raise RuntimeError("rms / grad must match data's shape")
+    rows, ncols = data.shape
+    if (mask_a is None) != (mask_b is None):
+        raise RuntimeError("give both group masks or neither")
+    for m, name in ((mask_a, "mask_a"), (mask_b, "mask_b")):
+        if m is not None:
+            _check(m, name, (torch.bool, torch.uint8), 1)
+            if m.numel() != rows:
+                raise RuntimeError(f"{name} must have one entry per row")
+    with _on_device(data):
+        _call("svox2_rmsprop_step_grouped", _ptr(data), _storage_of(data), _ptr(rms), _ptr(grad),
+              rows, ncols, _row_stride(data), _row_stride(rms), _row_stride(grad), int(split),
+              _ptr(mask_a), _ptr(mask_b), float(beta_a), float(lr_a), float(beta_b),
+              float(lr_b), float(epsilon), float(minval), _stream(data))
 
 
 def accel_dist_prop(links: torch.Tensor) -> None:

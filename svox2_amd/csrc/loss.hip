@@ -1,15 +1,33 @@
 // Sparse total-variation gradient for gfx950.
 //
-// One lane per (cell, channel), channels fastest: a 64-lane wavefront covers
-// ~2.4 cells of SH-2 data, so link reads of consecutive cells and the data and
-// gradient rows of each cell are touched by adjacent lanes.
+// What is computed (tv_grad_sparse_kernel, svox2/csrc/loss_kernel.cu:178-246,
+// and its launcher :558-604), for each listed cell (x, y, z) and channel c:
+//   v000 = the cell's value, v001 / v010 / v100 = its +z / +y / +x neighbours;
+//   dx, dy, dz = neighbour - v000;
+//   idelta = (scale / n_cells) / sqrt(1e-9 + dx^2 + dy^2 + dz^2)   (unscaled diffs);
+//   dx, dy, dz *= size_axis / 256;
+//   grad[v000] += -(dx + dy + dz) idelta, grad[v001] += dz idelta, ... for every
+//   link that addresses a row and value that is nonzero, and the row's mask.
+// The reference's edge rules are kept: a neighbour past the grid's end reads
+// link 0 (row 0) and receives gradient; a missing neighbour reads 0, or v000
+// with ignore_edge, which also skips cells whose link is 0; ignore_last_z
+// skips cells with z = size2 - 2.
 //
-// Semantics of tv_grad_sparse_kernel (svox2/csrc/loss_kernel.cu:178-246) and
-// its launcher (:558-604), including the reference's edge behaviour: an
-// out-of-range neighbour reads link 0 (data row 0) and receives gradient,
-// ignore_edge skips cells whose link is 0, and ignore_last_z drops the z+1
-// neighbour of the second-to-last z slice.
+// Layout of the work.  A wavefront takes 64 listed cells at a time, one lane
+// per cell: it splits the flat index with invariant-divisor multiplies (no
+// integer division), reads the cell's four links and decides whether any of
+// them addresses a row.  At about 8 % occupancy most cells address none and
+// stop there.  A ballot compacts the live cells, and the wave then processes
+// them two at a time, one 32-lane half per cell and lane = channel: the
+// half's four data reads are each one coalesced run of a row, and each of its
+// four gradient updates is one atomic instruction into one row, so the
+// interleaved 128-byte voxel rows are touched once per cell and neighbour.
+// One-channel data (density) skips the compaction: each lane finishes its
+// own cell.
 #include "common.hpp"
+#include "fastdiv.hpp"
+
+#include <algorithm>
 
 namespace svox2amd {
 namespace {
@@ -17,60 +35,200 @@ namespace {
 constexpr int kBlock = 256;
 
 template <typename DT>
-__global__ __launch_bounds__(kBlock) void k_tv_grad_sparse(
-        const int32_t* __restrict__ links, int s0, int s1, int s2,
-        const DT* __restrict__ data, int rows, int64_t dst, int64_t gst,
-        const int32_t* __restrict__ cells,
-        int64_t total, int start_dim, int nch, float scale, int ignore_edge,
-        int ignore_last_z, uint8_t* __restrict__ mask, float* __restrict__ grad) {
-    const int64_t tid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (tid >= total) return;
-    const int idx = static_cast<int>(tid % nch) + start_dim;
-    const int xyz = cells[tid / nch];
-    const int z = xyz % s2;
-    const int xy = xyz / s2;
-    const int y = xy % s1;
-    const int x = xy / s1;
-    if (x >= s0 || xyz < 0) return;  // cell index outside the grid
-    const int offx = s1 * s2, offy = s2;
-    const int32_t* lp = links + (static_cast<int64_t>(x) * offx + y * offy + z);
-    if (ignore_edge && lp[0] == 0) return;
+struct TvJob {
+    const int32_t* links;
+    int s0, s1, s2;
+    FastDiv div_z;  // by s2
+    FastDiv div_y;  // by s1
+    const DT* data;
+    int rows;
+    int64_t dst, gst;  // row strides of data and grad
+    const int32_t* cells;
+    int64_t n_cells;
+    int start_dim, nch;
+    float scale;  // already divided by n_cells
+    int ignore_edge, ignore_last_z;
+    uint8_t* mask;
+    float* grad;
+    float sc0, sc1, sc2;  // size_axis / 256
+};
 
-    const float sc0 = static_cast<float>(s0) * (1.f / 256.f);
-    const float sc1 = static_cast<float>(s1) * (1.f / 256.f);
-    const float sc2 = static_cast<float>(s2) * (1.f / 256.f);
+struct CellLinks {
+    int l000, l001, l010, l100;
+    bool live;
+};
 
-    const int l000 = lp[0];
-    const int l001 = ((z + 1 < s2) && (!ignore_last_z || z != s2 - 2)) ? lp[1] : 0;
-    const int l010 = (y + 1 < s1) ? lp[offy] : 0;
-    const int l100 = (x + 1 < s0) ? lp[offx] : 0;
-    if (ignore_last_z && z == s2 - 2) return;
+template <typename DT>
+__device__ __forceinline__ bool valid_row(const TvJob<DT>& J, int l) {
+    return static_cast<uint32_t>(l) < static_cast<uint32_t>(J.rows);
+}
 
-    auto ok = [&](int l) { return static_cast<uint32_t>(l) < static_cast<uint32_t>(rows); };
-    const float v000 = ok(l000) ? ld(data, static_cast<int64_t>(l000) * dst + idx) : 0.f;
-    const float nul = ignore_edge ? v000 : 0.f;
-    const float v001 = ok(l001) ? ld(data, static_cast<int64_t>(l001) * dst + idx) : nul;
-    const float v010 = ok(l010) ? ld(data, static_cast<int64_t>(l010) * dst + idx) : nul;
-    const float v100 = ok(l100) ? ld(data, static_cast<int64_t>(l100) * dst + idx) : nul;
+// The four links of listed cell k (live = false if it contributes nothing).
+template <typename DT>
+__device__ __forceinline__ CellLinks cell_links(const TvJob<DT>& J, int64_t k) {
+    CellLinks c{-1, -1, -1, -1, false};
+    if (k >= J.n_cells) return c;
+    const int xyz = J.cells[k];
+    if (xyz < 0) return c;
+    uint32_t z, y;
+    const uint32_t xy = J.div_z.divmod(static_cast<uint32_t>(xyz), z);
+    const uint32_t x = J.div_y.divmod(xy, y);
+    if (x >= static_cast<uint32_t>(J.s0)) return c;  // past the grid
+    const int64_t base = (static_cast<int64_t>(x) * J.s1 + y) * J.s2 + z;
+    const int32_t* lp = J.links + base;
+    c.l000 = lp[0];
+    if (J.ignore_edge && c.l000 == 0) return c;
+    if (J.ignore_last_z && static_cast<int>(z) == J.s2 - 2) return c;
+    c.l001 = static_cast<int>(z) + 1 < J.s2 ? lp[1] : 0;
+    c.l010 = static_cast<int>(y) + 1 < J.s1 ? lp[J.s2] : 0;
+    c.l100 = static_cast<int>(x) + 1 < J.s0 ? lp[static_cast<int64_t>(J.s1) * J.s2] : 0;
+    c.live = valid_row(J, c.l000) || valid_row(J, c.l001) || valid_row(J, c.l010) ||
+             valid_row(J, c.l100);
+    return c;
+}
 
+// The four gradient terms of channel ch of a cell, in the reference's float
+// operation order, and which of them the reference applies: a term is added
+// (and its row marked) when its scaled difference is nonzero, whatever the
+// product with idelta rounds to.
+template <typename DT>
+__device__ __forceinline__ void cell_terms(const TvJob<DT>& J, const CellLinks& c, int ch,
+                                           float t[4], bool act[4]) {
+    const float v000 = valid_row(J, c.l000) ? ld(J.data, c.l000 * J.dst + ch) : 0.f;
+    const float nul = J.ignore_edge ? v000 : 0.f;
+    const float v001 = valid_row(J, c.l001) ? ld(J.data, c.l001 * J.dst + ch) : nul;
+    const float v010 = valid_row(J, c.l010) ? ld(J.data, c.l010 * J.dst + ch) : nul;
+    const float v100 = valid_row(J, c.l100) ? ld(J.data, c.l100 * J.dst + ch) : nul;
     float dx = v100 - v000;
     float dy = v010 - v000;
     float dz = v001 - v000;
-    const float idelta = scale * rsqrtf(1e-9f + dx * dx + dy * dy + dz * dz);
-    dx *= sc0;
-    dy *= sc1;
-    dz *= sc2;
+    const float idelta = J.scale * rsqrtf(1e-9f + dx * dx + dy * dy + dz * dz);
+    dx *= J.sc0;
+    dy *= J.sc1;
+    dz *= J.sc2;
     const float sm = -(dx + dy + dz);
-    auto add = [&](int l, float val) {
-        if (ok(l) && val != 0.f) {
-            atomic_add_f32(grad + static_cast<int64_t>(l) * gst + idx, val * idelta);
-            if (mask != nullptr) mask[l] = 1;
+    const float v[4] = {sm, dz, dy, dx};
+    const int ls[4] = {c.l000, c.l001, c.l010, c.l100};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        t[i] = v[i] * idelta;
+        act[i] = valid_row(J, ls[i]) && v[i] != 0.f;
+    }
+}
+
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_tv_cells(TvJob<DT> J) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
+    for (int64_t base = wave * 64; base < J.n_cells; base += nwaves * 64) {
+        const CellLinks c = cell_links(J, base + lane);
+        if (J.nch == 1) {  // one channel: the lane finishes its cell
+            if (c.live) {
+                float t[4];
+                bool act[4];
+                const int ch = J.start_dim;
+                cell_terms(J, c, ch, t, act);
+                const int ls[4] = {c.l000, c.l001, c.l010, c.l100};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (act[i]) {
+                        atomic_add_f32(J.grad + ls[i] * J.gst + ch, t[i]);
+                        if (J.mask) J.mask[ls[i]] = 1;
+                    }
+                }
+            }
+            continue;
         }
-    };
-    add(l000, sm);
-    add(l001, dz);
-    add(l010, dy);
-    add(l100, dx);
+        uint64_t m = __ballot(c.live);
+        const int half = lane >> 5, hl = lane & 31;
+        while (m != 0) {  // uniform: two live cells per pass, one per half
+            const int j0 = __builtin_ctzll(m);
+            m &= m - 1;
+            int j1 = -1;
+            if (m != 0) {
+                j1 = __builtin_ctzll(m);
+                m &= m - 1;
+            }
+            const int j = half ? j1 : j0;
+            CellLinks h;
+            const int src0 = j0, src1 = j1 >= 0 ? j1 : j0;
+            h.l000 = half ? __builtin_amdgcn_readlane(c.l000, src1)
+                          : __builtin_amdgcn_readlane(c.l000, src0);
+            h.l001 = half ? __builtin_amdgcn_readlane(c.l001, src1)
+                          : __builtin_amdgcn_readlane(c.l001, src0);
+            h.l010 = half ? __builtin_amdgcn_readlane(c.l010, src1)
+                          : __builtin_amdgcn_readlane(c.l010, src0);
+            h.l100 = half ? __builtin_amdgcn_readlane(c.l100, src1)
+                          : __builtin_amdgcn_readlane(c.l100, src0);
+            if (j < 0) continue;
+            const int ls[4] = {h.l000, h.l001, h.l010, h.l100};
+            bool any[4] = {false, false, false, false};
+            for (int k = hl; k < J.nch; k += 32) {
+                const int ch = J.start_dim + k;
+                float t[4];
+                bool act[4];
+                cell_terms(J, h, ch, t, act);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (act[i]) {
+                        atomic_add_f32(J.grad + ls[i] * J.gst + ch, t[i]);
+                        any[i] = true;
+                    }
+                }
+            }
+            if (J.mask) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    // one store per half and row that received a gradient
+                    const uint64_t b = __ballot(any[i]);
+                    const uint32_t mine = static_cast<uint32_t>(b >> (32 * half));
+                    if (mine != 0 && hl == __builtin_ctz(mine)) J.mask[ls[i]] = 1;
+                }
+            }
+        }
+    }
+}
+
+template <typename DT>
+int launch(const TvJob<DT>& J, hipStream_t s) {
+    const int64_t windows = (J.n_cells + 63) / 64;
+    const int64_t blocks = std::min<int64_t>((windows + 3) / 4, 256 * 8);
+    k_tv_cells<DT><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
+    return check_launch("tv_grad_sparse");
+}
+
+template <typename DT>
+int run_tv(const DT* data, const int32_t* links, const int32_t size[3], int64_t rows,
+           int32_t data_dim, int64_t data_stride, int64_t grad_stride, const int32_t* cells,
+           int64_t n_cells, uint8_t* mask, int32_t start_dim, int32_t end_dim, float scale,
+           int32_t ignore_edge, int32_t ignore_last_z, float* grad, hipStream_t s) {
+    TvJob<DT> J;
+    J.links = links;
+    J.s0 = size[0];
+    J.s1 = size[1];
+    J.s2 = size[2];
+    J.div_z = FastDiv(static_cast<uint32_t>(size[2]));
+    J.div_y = FastDiv(static_cast<uint32_t>(size[1]));
+    J.data = data;
+    J.rows = static_cast<int>(rows);
+    J.dst = data_stride > 0 ? data_stride : data_dim;
+    J.gst = grad_stride > 0 ? grad_stride : data_dim;
+    SVOX2_REQUIRE(J.dst >= data_dim && J.gst >= data_dim, "row stride below data_dim");
+    J.cells = cells;
+    J.n_cells = n_cells;
+    J.start_dim = start_dim;
+    J.nch = end_dim - start_dim;
+    // The reference divides the scale by the number of cells (loss_kernel.cu:597).
+    J.scale = scale / static_cast<float>(n_cells);
+    J.ignore_edge = ignore_edge;
+    J.ignore_last_z = ignore_last_z;
+    J.mask = mask;
+    J.grad = grad;
+    J.sc0 = static_cast<float>(size[0]) * (1.f / 256.f);
+    J.sc1 = static_cast<float>(size[1]) * (1.f / 256.f);
+    J.sc2 = static_cast<float>(size[2]) * (1.f / 256.f);
+    return launch(J, s);
 }
 
 }  // namespace
@@ -106,25 +264,12 @@ extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
     SVOX2_REQUIRE(data != nullptr && rand_cells != nullptr && grad_data != nullptr,
                   "data / rand_cells / grad_data are NULL");
     SVOX2_REQUIRE(data_stride >= 0 && grad_stride >= 0, "negative row stride");
-    const int64_t dst = data_stride > 0 ? data_stride : data_dim;
-    const int64_t gst = grad_stride > 0 ? grad_stride : data_dim;
-    SVOX2_REQUIRE(dst >= data_dim && gst >= data_dim, "row stride below data_dim");
-    const int nch = end_dim - start_dim;
-    const int64_t total = n_cells * nch;
-    // The reference divides the scale by the number of cells (loss_kernel.cu:597).
-    const float sc = scale / static_cast<float>(n_cells);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const unsigned blocks = blocks_for(total, kBlock);
-    if (storage == SVOX2_STORAGE_BF16) {
-        k_tv_grad_sparse<uint16_t><<<blocks, kBlock, 0, s>>>(
-            links, size[0], size[1], size[2], static_cast<const uint16_t*>(data),
-            static_cast<int>(rows), dst, gst, rand_cells, total, start_dim, nch, sc, ignore_edge,
-            ignore_last_z, mask_out, grad_data);
-    } else {
-        k_tv_grad_sparse<float><<<blocks, kBlock, 0, s>>>(
-            links, size[0], size[1], size[2], static_cast<const float*>(data),
-            static_cast<int>(rows), dst, gst, rand_cells, total, start_dim, nch, sc, ignore_edge,
-            ignore_last_z, mask_out, grad_data);
-    }
-    return check_launch("tv_grad_sparse");
+    if (storage == SVOX2_STORAGE_BF16)
+        return run_tv(static_cast<const uint16_t*>(data), links, size, rows, data_dim, data_stride,
+                      grad_stride, rand_cells, n_cells, mask_out, start_dim, end_dim, scale,
+                      ignore_edge, ignore_last_z, grad_data, s);
+    return run_tv(static_cast<const float*>(data), links, size, rows, data_dim, data_stride,
+                  grad_stride, rand_cells, n_cells, mask_out, start_dim, end_dim, scale,
+                  ignore_edge, ignore_last_z, grad_data, s);
 }

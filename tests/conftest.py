@@ -45,6 +45,39 @@ def host_grid(scene, dtype=torch.float32, links=None):
                   offset=0.5 * N - 0.5, scaling=0.5 * N)
 
 
+# Gradient tolerances against the FP64 oracle.  Float32 atomics accumulate in
+# arbitrary order, so results are not bit-reproducible:
+#   GRAD_MAX_RTOL   max|g - g64| <= GRAD_MAX_RTOL * max|g64| over the tensor;
+#   GRAD_ELEM_RTOL  |g - g64| <= GRAD_ELEM_RTOL * |g64| for every element with
+#                   |g64| > GRAD_ELEM_FLOOR * max|g64|, so that a mistake
+#                   confined to rows with small gradients (an edge corner's
+#                   weight, a dropped term) is not hidden by the largest ones.
+# Measured GPU errors are about 1e-6 (max-normalised); the per-element figure
+# is bound by float32 cancellation in dsigma = ws (total T - accum).
+GRAD_MAX_RTOL = 2e-5
+GRAD_ELEM_RTOL = 1e-2
+GRAD_ELEM_FLOOR = 1e-3
+
+
+def grad_errors(g, g64):
+    """(max-normalised error, worst per-element relative error over elements
+    above GRAD_ELEM_FLOOR of the max) of a gradient against the FP64 oracle."""
+    g = np.asarray(g, np.float64).reshape(-1)
+    g64 = np.asarray(g64, np.float64).reshape(-1)
+    mx = max(float(np.abs(g64).max()), 1e-30)
+    err = np.abs(g - g64)
+    big = np.abs(g64) > GRAD_ELEM_FLOOR * mx
+    elem = float((err[big] / np.abs(g64[big])).max()) if big.any() else 0.0
+    return float(err.max() / mx), elem
+
+
+def assert_grad_close(g, g64, name="grad", max_rtol=GRAD_MAX_RTOL, elem_rtol=GRAD_ELEM_RTOL):
+    e_max, e_elem = grad_errors(g, g64)
+    print(f"{name}: max-normalised {e_max:.3e}, per-element {e_elem:.3e}")
+    assert e_max <= max_rtol, f"{name}: max-normalised error {e_max:.3e} > {max_rtol}"
+    assert e_elem <= elem_rtol, f"{name}: per-element error {e_elem:.3e} > {elem_rtol}"
+
+
 @pytest.fixture(scope="session")
 def gpu():
     if not torch.cuda.is_available():

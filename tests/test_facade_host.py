@@ -52,6 +52,51 @@ def test_grid_construction_layouts():
         svox2_amd.SparseGrid(reso=4, background_nlayers=4)
 
 
+@pytest.mark.parametrize("size,z_order", [((8, 8, 8), False), ((8, 8, 8), True),
+                                          ((16, 16, 16), True), ((7, 10, 5), False)])
+def test_sphere_bound_numbering_closed_form(size, z_order):
+    """use_sphere_bound keeps the cells whose lower corner (2 i / n - 1 per
+    axis) is within 1 + sqrt(3) / max(n) of the centre, an exact integer test
+    (sum (2 i - n)^2 / n^2 against an irrational bound), and numbers them
+    0, 1, ... in ascending order of their initial link (Morton code or raster
+    index); every other link is -1.  Brute force over the cells."""
+    g = svox2_amd.SparseGrid(reso=list(size), use_z_order=z_order, use_sphere_bound=True)
+    links = g.links.numpy()
+    bound = (1 + 3 ** 0.5 / max(size)) ** 2
+    kept = []
+    for i in range(size[0]):
+        for j in range(size[1]):
+            for k in range(size[2]):
+                r2 = sum(((2 * c - n) / n) ** 2 for c, n in zip((i, j, k), size))
+                if r2 <= bound:
+                    code = (oracle_morton(i, j, k) if z_order and size[0] == size[1] == size[2]
+                            else (i * size[1] + j) * size[2] + k)
+                    kept.append((code, (i, j, k)))
+    want = np.full(size, -1, np.int32)
+    for row, (_, cell) in enumerate(sorted(kept)):
+        want[cell] = row
+    assert np.array_equal(links, want)
+    assert g.capacity == len(kept) and g.sh_data.shape[0] == len(kept)
+
+
+def oracle_morton(x, y, z):
+    import oracle.oracle as O
+    return O.morton_code_3(x, y, z)
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_sphere_bound_matches_float32_corner_norm(n):
+    """The float64 test agrees with evaluating the corner's norm in float32
+    (the corner at (i - 1/2) * 2/n + 1/n - 1, its norm and the bound all
+    rounded to float32)."""
+    from svox2_amd.svox2 import _sphere_bound_cells
+    ax = torch.arange(n, dtype=torch.float32) - 0.5
+    X, Y, Z = torch.meshgrid(ax, ax, ax, indexing="ij")
+    pts = torch.stack((X, Y, Z), -1).view(-1, 3) * np.float32(2.0 / n) + np.float32(1.0 / n - 1.0)
+    f32 = pts.norm(dim=-1) <= np.float32(1.0 + 3 ** 0.5 / n)
+    assert torch.equal(_sphere_bound_cells((n, n, n), "cpu"), f32)
+
+
 def test_world_grid_maps_inverse():
     g = svox2_amd.SparseGrid(reso=[10, 12, 14], radius=[1.5, 2.0, 0.5], center=[0.1, -0.2, 0.3])
     p = torch.rand(100, 3) * 2 - 1
@@ -61,23 +106,26 @@ def test_world_grid_maps_inverse():
     assert torch.allclose(g.world2grid(corner), torch.full((1, 3), -0.5), atol=1e-5)
 
 
-def test_rand_cells_and_indexer_conversion():
+def test_rand_cells_and_indexer():
     g = svox2_amd.SparseGrid(reso=10)
     np.random.seed(0)
     cells = g._get_rand_cells(0.05)
     assert cells.numel() == 50 and cells.dtype == torch.int32
     assert int(cells.min()) >= 0 and int(cells.max()) < 1000
-    # wrap-around keeps a contiguous run modulo the cell count
-    d = torch.diff(cells.long()) % 1000
-    assert torch.all(d == 1)
+    # a run of consecutive cells from np.random.randint(0, cells), wrapping
+    for seed in range(40):
+        np.random.seed(seed)
+        start = np.random.RandomState(seed).randint(0, 1000)
+        got = g._get_rand_cells(0.05).numpy()
+        assert np.array_equal(got, (start + np.arange(50)) % 1000)
     assert g._get_rand_cells(1.0) is None
+    assert g._get_rand_cells(1.0, force=True).numel() == 1000
+    # the optimisers take the touched-row mask as it is (no host sync); no
+    # mask means every row
     assert g._maybe_convert_sparse_grad_indexer().dim() == 0
     g.sparse_grad_indexer = torch.zeros(1000, dtype=torch.bool)
     g.sparse_grad_indexer[[3, 7]] = True
-    idx = g._maybe_convert_sparse_grad_indexer()
-    assert idx.dtype == torch.int64 and idx.tolist() == [3, 7]
-    g.sparse_grad_indexer[:500] = True
-    assert g._maybe_convert_sparse_grad_indexer().dtype == torch.bool
+    assert g._maybe_convert_sparse_grad_indexer() is g.sparse_grad_indexer
 
 
 def test_compute_methods_refuse_cpu():

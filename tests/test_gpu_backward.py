@@ -1,10 +1,12 @@
 """GPU gradients (fused MSE kernel and the autograd backward) against the FP64
 oracle's autograd gradient and FP64 finite differences.
 
-Tolerances: colours 2e-5 absolute; gradients max|g - g64| <= 1e-3 * max|g64|
-(per tensor; float32 accumulation through atomics in arbitrary order, so the
-results are not bit-reproducible); the touched-row mask is bit-exact against
-the rows that are corners of a contributing sample of the FP32 march.
+Tolerances: colours 2e-5 absolute; gradients against the FP64 oracle
+max|g - g64| <= 2e-5 * max|g64| per tensor and |g - g64| <= 1e-2 * |g64| per
+element above 1e-3 of the max (tests/conftest.py: float32 accumulation through
+atomics in arbitrary order, so the results are not bit-reproducible); the
+touched-row mask is bit-exact against the rows that are corners of a
+contributing sample of the FP32 march.
 """
 import numpy as np
 import pytest
@@ -13,12 +15,11 @@ import torch
 import oracle.oracle as O
 import svox2_amd
 from svox2_amd import scenes
-from tests.conftest import host_grid
+from tests.conftest import assert_grad_close, grad_errors, host_grid
 
 pytestmark = pytest.mark.gpu
 
 OPT = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
-GRAD_RTOL = 1e-3
 
 
 def _grid(sc, dev, dtype=torch.float32):
@@ -48,10 +49,10 @@ def test_fused_dense64_vs_fp64(gpu, dense64, march, dtype):
                                torch.from_numpy(gt), gpu)
     smp = march.samples(g, o, d, OPT)
     rgb64, gd64, gs64 = O.mse_grads_fp64(g, o, d, OPT, gt, samples=smp)
-    e = (np.abs(rgb - rgb64).max(), _rel(gd, gd64), _rel(gs, gs64))
-    print(f"dense64 fused {dtype}: colour {e[0]:.3e}, grad_density rel {e[1]:.3e}, "
-          f"grad_sh rel {e[2]:.3e}")
-    assert e[0] < 2e-5 and e[1] < GRAD_RTOL and e[2] < GRAD_RTOL
+    print(f"dense64 fused {dtype}: colour {np.abs(rgb - rgb64).max():.3e}")
+    assert np.abs(rgb - rgb64).max() < 2e-5
+    assert_grad_close(gd, gd64, "grad_density")
+    assert_grad_close(gs, gs64, "grad_sh")
     assert np.array_equal(mask, O.touched_rows(g, smp))
 
 
@@ -64,7 +65,8 @@ def test_fused_beta_and_sparsity(gpu, dense64, march):
                             torch.from_numpy(gt[:1024]), gpu, **kw)
     rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:1024], d[:1024], OPT, gt[:1024], **kw)
     assert np.abs(rgb - rgb64).max() < 2e-5
-    assert _rel(gd, gd64) < GRAD_RTOL and _rel(gs, gs64) < GRAD_RTOL
+    assert_grad_close(gd, gd64, "grad_density")
+    assert_grad_close(gs, gs64, "grad_sh")
 
 
 def test_autograd_backward_matches_oracle(gpu, dense64, march):
@@ -80,7 +82,8 @@ def test_autograd_backward_matches_oracle(gpu, dense64, march):
     gs = grid.sh_data.grad.cpu().numpy()
     rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:n], d[:n], OPT, gt[:n])
     assert np.abs(rgb.detach().cpu().numpy() - rgb64).max() < 2e-5
-    assert _rel(gd, gd64) < GRAD_RTOL and _rel(gs, gs64) < GRAD_RTOL
+    assert_grad_close(gd, gd64, "grad_density")
+    assert_grad_close(gs, gs64, "grad_sh")
 
 
 def test_finite_differences_with_ray_clip(gpu, march):
@@ -165,7 +168,7 @@ def test_training_step_values(gpu, march):
 
     - The fused gradients plus both TV terms (the same contiguous cell runs,
       drawn from np.random as the reference does) are compared with the FP64
-      fused gradients plus the oracle's TV, to 1e-3 of their max.
+      fused gradients plus the oracle's TV, to the gradient tolerances.
     - The touched-row masks are compared with the fused mask or-ed with the
       TV masks.
     - Then one RMSprop step per quantity (lr 30 and 1e-2, beta 0.95, fresh
@@ -201,8 +204,8 @@ def test_training_step_values(gpu, march):
     tv_d, m_d = O.tv_grad_sparse(links, np.asarray(g.density).reshape(-1, 1), run(start_d), 0, 1,
                                  1e-5, False, False)
     tv_s, m_s = O.tv_grad_sparse(links, g.sh, run(start_s), 0, g.sh.shape[1], 1e-3, True, False)
-    assert _rel(gd, gd64.reshape(-1, 1) + tv_d) < GRAD_RTOL
-    assert _rel(gs, gs64 + tv_s) < GRAD_RTOL
+    assert_grad_close(gd, gd64.reshape(-1, 1) + tv_d, "grad_density + TV")
+    assert_grad_close(gs, gs64 + tv_s, "grad_sh + TV")
     fused_mask = O.touched_rows(g, smp)
     assert np.array_equal(mask_d, fused_mask | m_d)
     assert np.array_equal(mask_s, fused_mask | m_s)

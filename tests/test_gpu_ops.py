@@ -114,3 +114,142 @@ def test_rmsprop_all_mask_index(gpu, dtype):
     _C.rmsprop_step(data, rms0.clone().to(gpu), grad0.clone().to(gpu),
                     torch.zeros(0, dtype=torch.int64, device=gpu), 0.95, 0.1, 1e-8, -1.5, 0.05)
     assert torch.equal(data.cpu(), data0)
+
+
+def _interleaved(R, gen, dtype, dev):
+    """(buffer (R, 32), density view (R, 1), sh view (R, 27)) of the package's
+    voxel row layout, random contents."""
+    buf = torch.randn(R, 32, generator=gen).to(dtype).to(dev)
+    return buf, buf[:, 27:28], buf[:, :27]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_rmsprop_and_sgd_on_interleaved_rows(gpu, dtype):
+    """rmsprop_step and sgd_step on the strided views of the interleaved row
+    layout (density: one column at offset 27; SH: 27 columns), every indexer
+    kind, against the closed forms; a sparse mask spanning many 64-row
+    windows; index entries out of range are skipped; pad columns untouched."""
+    gen = torch.Generator().manual_seed(12)
+    R = 70001
+    mask = torch.rand(R, generator=gen) < 0.03
+    rows_m = np.nonzero(mask.numpy())[0]
+    idx = torch.cat([torch.nonzero(mask).flatten(), torch.tensor([-3, R, R + 7])])
+    for view in ("density", "sh"):
+        for kind, indexer, rows in (("all", torch.empty(()), np.arange(R)),
+                                    ("mask", mask, rows_m), ("index", idx, rows_m)):
+            for optim in ("rmsprop", "sgd"):
+                data_b, dd, ds = _interleaved(R, gen, dtype, gpu)
+                rms_b, rd_, rs_ = _interleaved(R, gen, torch.float32, gpu)
+                rms_b.abs_()
+                grad_b, gd_, gs_ = _interleaved(R, gen, torch.float32, gpu)
+                data, rms, grad = (dd, rd_, gd_) if view == "density" else (ds, rs_, gs_)
+                before = [t.cpu().clone() for t in (data_b, rms_b, grad_b)]
+                d0, r0, g0 = (t.double().cpu().numpy() for t in (data, rms, grad))
+                if optim == "rmsprop":
+                    _C.rmsprop_step(data, rms, grad, indexer.to(gpu), 0.95, 0.1, 1e-8, -1.5, 0.05)
+                    wd, wr, wg = O.rmsprop_step(d0, r0, g0, rows, 0.95, 0.1, 1e-8, -1.5, 0.05)
+                else:
+                    _C.sgd_step(data, grad, indexer.to(gpu), 0.1, 0.05)
+                    wd, wg = O.sgd_step(d0, g0, rows, 0.1, 0.05)
+                    wr = r0
+                tol = 1e-6 if dtype == torch.float32 else 8e-3
+                tag = (view, kind, optim)
+                assert np.allclose(data.double().cpu().numpy(), wd, rtol=tol, atol=tol), tag
+                assert np.allclose(rms.cpu().numpy(), wr, rtol=1e-5, atol=1e-12), tag
+                assert np.array_equal(grad.cpu().numpy(), wg.astype(np.float32)), tag
+                # columns outside the view are untouched
+                cols = slice(27, 28) if view == "density" else slice(0, 27)
+                other = np.ones(32, bool)
+                other[cols] = False
+                for b, b0 in zip((data_b, rms_b, grad_b), before):
+                    assert torch.equal(b.cpu()[:, other], b0[:, other]), tag
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_rmsprop_grouped_equals_two_calls(gpu, dtype):
+    """rmsprop_step_grouped over [SH | density] rows with separate masks and
+    learning rates equals rmsprop_step on the SH view then on the density
+    view, bit for bit; the facade's optim_density_sh_step equals
+    optim_sh_step + optim_density_step."""
+    gen = torch.Generator().manual_seed(13)
+    R = 50000
+    m_sh = (torch.rand(R, generator=gen) < 0.05).to(gpu)
+    m_d = (torch.rand(R, generator=gen) < 0.05).to(gpu)
+    data_b, dd, ds = _interleaved(R, gen, dtype, gpu)
+    rms_b, rd_, rs_ = _interleaved(R, gen, torch.float32, gpu)
+    rms_b.abs_().mul_(torch.rand(R, 32, generator=gen).to(gpu) > 0.5)
+    grad_b, gd_, gs_ = _interleaved(R, gen, torch.float32, gpu)
+    copies = [t.clone() for t in (data_b, rms_b, grad_b)]
+    _C.rmsprop_step(ds, rs_, gs_, m_sh, 0.95, 1e-2, 1e-8, -1e9, 1e-2)
+    _C.rmsprop_step(dd, rd_, gd_, m_d, 0.9, 30.0, 1e-8, -1e9, 30.0)
+    cd, cr, cg = copies
+    _C.rmsprop_step_grouped(cd[:, :28], cr[:, :28], cg[:, :28], 27, m_sh, m_d,
+                            0.95, 1e-2, 0.9, 30.0, 1e-8, -1e9)
+    for a, b in zip((data_b, rms_b, grad_b), copies):
+        assert torch.equal(a, b)
+    # the facade
+    gen = torch.Generator().manual_seed(2)
+    sc = scenes.make_blob_grid(32, scenes.BlobSpec(5, (0.1, 0.2), (5.0, 20.0)), gen)
+    grids = [scenes.to_sparse_grid(sc, device=gpu, dtype=dtype) for _ in range(2)]
+    Rg = grids[0].density_data.size(0)
+    gg = torch.randn(Rg, 32, generator=gen).to(gpu)
+    md = (torch.rand(Rg, generator=gen) < 0.3).to(gpu)
+    ms = (torch.rand(Rg, generator=gen) < 0.3).to(gpu)
+    for g in grids:
+        gd, gs, _, _ = g._get_data_grads()
+        gd.copy_(gg[:, 27:28])
+        gs.copy_(gg[:, :27])
+        g.sparse_grad_indexer, g.sparse_sh_grad_indexer = md.clone(), ms.clone()
+    grids[0].optim_sh_step(1e-2, beta=0.95)
+    grids[0].optim_density_step(30.0, beta=0.95)
+    grids[1].optim_density_sh_step(30.0, 1e-2, beta=0.95)
+    for a, b in ((grids[0].density_data, grids[1].density_data),
+                 (grids[0].sh_data, grids[1].sh_data)):
+        assert torch.equal(a.data, b.data) and torch.equal(a.grad, b.grad)
+    assert torch.equal(grids[0].sh_rms, grids[1].sh_rms)
+    assert torch.equal(grids[0].density_rms, grids[1].density_rms)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_tv_grad_sparse_shapes_and_layouts(gpu, dtype):
+    """tv_grad_sparse against the oracle beyond the blob grid: an odd grid
+    (17 x 12 x 9, so the flat cell index splits by non-powers of two) with
+    links mixing rows, -1 and skip levels; random cells with repeats; the
+    SH columns and the density column of interleaved 32-wide rows; 40
+    channels (more than one pass of a 32-lane half); and scale 0, where the
+    gradient stays zero but rows are still marked wherever a difference is
+    nonzero, as the reference's rule marks them."""
+    rng = np.random.default_rng(21)
+    shape = (17, 12, 9)
+    cells_n = int(np.prod(shape))
+    R = 600
+    links = rng.integers(-4, R, size=shape).astype(np.int32)
+    links[rng.random(shape) < 0.5] = -1
+    cells = rng.integers(0, cells_n, size=900).astype(np.int32)
+    buf = torch.from_numpy(rng.standard_normal((R, 32))).float().to(dtype)
+    wide = torch.from_numpy(rng.standard_normal((R, 40))).float().to(dtype)
+    cases = [("sh", 0, 27, 1e-3, True, False), ("density", 0, 1, 1e-2, False, True),
+             ("sh", 3, 20, 0.0, False, False), ("wide", 0, 40, 0.5, True, False),
+             ("wide", 5, 39, 2.0, False, True)]
+    lk = torch.from_numpy(links).to(gpu)
+    cells_d = torch.from_numpy(cells).to(gpu)
+    for kind, lo, hi, scale, ie, lz in cases:
+        cols = {"sh": slice(0, 27), "density": slice(27, 28)}.get(kind)
+        if kind == "wide":
+            host = wide
+            dev_data = wide.to(gpu)
+            grad = torch.zeros(R, 40, device=gpu)
+        else:
+            host = buf[:, cols]
+            dev_data = buf.to(gpu)[:, cols]
+            grad = torch.zeros(R, 32, device=gpu)[:, cols]
+        mask = torch.zeros(R, dtype=torch.bool, device=gpu)
+        _C.tv_grad_sparse(lk, dev_data, cells_d, mask, lo, hi, scale, False, 2.0, ie, lz,
+                          -1.0, -1.0, grad)
+        rg, rm = O.tv_grad_sparse(links, host.double().numpy(), cells, lo, hi, scale, ie, lz)
+        g = grad.cpu().numpy()
+        tag = (kind, lo, hi, scale, ie, lz)
+        assert np.abs(g - rg).max() <= 1e-5 * max(np.abs(rg).max(), 1e-30), tag
+        assert np.array_equal(mask.cpu().numpy(), rm), tag
+        if scale == 0.0:
+            assert not g.any() and rm.any(), tag

@@ -188,6 +188,11 @@ def test_all_empty_grid(gpu):
 
 
 def test_image_kernel_matches_ray_kernel(gpu, dense64, march):
+    """The image kernel against (1) the ray kernel on the oracle's float32
+    camera rays (the same arithmetic as cam2world_ray, so bit-exact), (2)
+    the FP64 colours of those rays, and (3) the ray kernel on the rays of
+    Camera.gen_rays (the reference's FP64 Python generator, svox2.py:157-183,
+    within an ulp of (1)), to the colour tolerance."""
     g, o, d, gt, sc = dense64
     grid = _grid(sc, gpu)
     cams = scenes.CameraSet(scenes.hemisphere_cameras(2, torch.Generator().manual_seed(1)),
@@ -202,6 +207,10 @@ def test_image_kernel_matches_ray_kernel(gpu, dense64, march):
     rgb64 = O.render_fp64(g, ro, rd, OPT, torch.from_numpy(g.density),
                           torch.from_numpy(g.sh)).numpy()
     assert np.abs(rgb - rgb64).max() < 2e-5
+    rays = cam.gen_rays()
+    rgb_gen = _render(grid, rays.origins, rays.dirs, gpu)
+    print(f"image kernel vs Camera.gen_rays rays: {np.abs(img.reshape(-1, 3) - rgb_gen).max():.3e}")
+    assert np.abs(img.reshape(-1, 3) - rgb_gen).max() < 2e-5
 
 
 def test_contiguous_and_interleaved_layouts_agree(gpu, dense64):

@@ -262,3 +262,13 @@ def test_tv_grad_axis_factor():
     g128, m128 = O.tv_grad_sparse(l128, data, cells128, 0, 2, 1.0, False, False)
     assert np.array_equal(m256, m128) and m256.sum() > 100
     assert np.array_equal(g128, 0.5 * g256)
+
+
+def test_sgd_closed_form_known_answer():
+    """data - lr * g per selected row, lr_last on the last column, the
+    gradient zeroed only where applied: a hand-computed case."""
+    data = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    grad = np.array([[0.5, 1.0, -2.0], [1.0, 1.0, 1.0]])
+    d, g = O.sgd_step(data, grad, [0], 0.1, 0.25)
+    assert np.allclose(d, [[0.95, 1.9, 3.5], [4.0, 5.0, 6.0]], rtol=0, atol=1e-15)
+    assert np.array_equal(g, [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]])

@@ -202,6 +202,42 @@ def test_touched_rows_match_contributions(dense64, march):
     assert m.sum() > 0
 
 
+@pytest.mark.parametrize("cut", [None, 5])
+def test_touched_rows_axis_ray_closed_form(march, cut):
+    """The touched set in closed form, both ways (nothing missing, nothing
+    extra).  A ray along +x at grid (y, z) = (3.25, 4.5) through an N = 8
+    grid, world = grid coordinates, uniform density 0.05 (no stop):
+    samples sit at x = -0.5 + k/2 clamped to [0, 7], in cells
+    l = (min(trunc(x), 6), 3, 4), and each marks its 8 corners
+    (render_util.cuh:137-140), so the touched rows are the cells
+    x in [0, 7], y in {3, 4}, z in {4, 5}.  With the density zeroed at
+    x >= cut, samples in cells l_x >= cut have density 0 (all corners zero)
+    and do not contribute, so x stops at cut."""
+    N = 8
+    links = np.arange(N ** 3, dtype=np.int32).reshape(N, N, N)
+    dens = np.full(N ** 3, 0.05)
+    if cut is not None:
+        dens.reshape(N, N, N)[cut:] = 0.0
+    sh = np.random.default_rng(0).standard_normal((N ** 3, 27))
+    g = O.Grid(links, dens, sh, 9, offset=np.zeros(3), scaling=np.ones(3))
+    opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+    o = np.array([[-3.0, 3.25, 4.5]], np.float32)
+    d = np.array([[1.0, 0.0, 0.0]], np.float32)
+    smp = march.samples(g, o, d, opt)
+    got = O.touched_rows(g, smp).reshape(N, N, N)
+    want = np.zeros((N, N, N), bool)
+    want[: N if cut is None else cut + 1, 3:5, 4:6] = True
+    assert np.array_equal(got, want)
+    # a hole in the links: its row is never touched, the rest stays
+    links2 = links.copy()
+    links2[2, 3, 4] = -1
+    g2 = O.Grid(links2, dens, sh, 9, offset=np.zeros(3), scaling=np.ones(3))
+    got2 = O.touched_rows(g2, march.samples(g2, o, d, opt)).reshape(N, N, N)
+    want2 = want.copy()
+    want2[2, 3, 4] = False
+    assert np.array_equal(got2, want2)
+
+
 def test_c_fused_port_matches_fp64(march):
     """The scalar C port of the fused forward+backward (the CPU baseline)
     against the FP64 autograd oracle, with per-ray clips, beta and sparsity."""
