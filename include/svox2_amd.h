@@ -138,6 +138,28 @@ int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
                                     float* rgb_out, float* log_transmit_out,
                                     void* stream);
 
+/* Expected termination distance of each ray, depth_out [n]: the sum over the
+ * samples of svox2_volume_render_cuvol (same placement, skips, densities,
+ * transmittance and stop) of weight * (t / step_size) * world_step, a length
+ * along the unit world direction (not z-depth).  A ray that misses the grid
+ * gets 0; last_sample_opaque and the background do not apply.
+ * Replaces volume_render_expected_term (svox2.cpp:25, bound at :82;
+ * render_lerp_kernel_cuvol.cu:1154). */
+int svox2_volume_render_expected_term(const svox2_grid_t* grid, const float* origins,
+                                      const float* dirs, const float* ray_clip,
+                                      int64_t n_rays, const svox2_render_options_t* opt,
+                                      float* depth_out, void* stream);
+
+/* Distance (t / step_size) * world_step of the first sample along each ray
+ * whose trilinear density strictly exceeds sigma_thresh (Dex-NeRF depth), or
+ * 0 if none; the samples are those of svox2_volume_render_cuvol without the
+ * transmittance stop.  Replaces volume_render_sigma_thresh (svox2.cpp:28,
+ * bound at :83; render_lerp_kernel_cuvol.cu:1176). */
+int svox2_volume_render_sigma_thresh(const svox2_grid_t* grid, const float* origins,
+                                     const float* dirs, const float* ray_clip,
+                                     int64_t n_rays, const svox2_render_options_t* opt,
+                                     float sigma_thresh, float* depth_out, void* stream);
+
 /* Backward of svox2_volume_render_cuvol given dL/drgb (grad_out [n][3]) and
  * the forward colours (color_cache [n][3]); accumulates into grads.  With the
  * forward's ray_cost_out (ray_cost) and a device workspace of

@@ -17,6 +17,8 @@ reference lines it follows:
 * ``volume_render_gradcheck_lerp`` - a line-by-line, dtype-generic restatement
   of SparseGrid._volume_render_gradcheck_lerp (svox2/svox2.py:658-884) with the
   grid passed explicitly; the CPU baseline and a cross-check of render_fp64.
+* ``depth_fp64`` (render_lerp_kernel_cuvol.cu:122-225), the depth
+  renders over the same sample list.
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
   (loss_kernel.cu:178-246,558-604), ``rmsprop_step`` (optim_kernel.cu:15-93),
@@ -493,6 +495,61 @@ def _blocks(grid: Grid, origins, dirs, opt: Opts, clip, samples, max_samples: in
     else:
         for r0, r1, sub in _ray_chunks(samples, np.asarray(origins).shape[0], max_samples):
             yield (r0, r1, *post(sub))
+
+
+def depth_fp64(grid: Grid, origins, dirs, opt: Opts, sigma_thresh: Optional[float] = None,
+               clip=None) -> np.ndarray:
+    """FP64 depth of each ray (render_lerp_kernel_cuvol.cu:122-225) over the
+    FP32 march's sample list, whose FP32 decisions (which samples are
+    evaluated, which contribute, where the ray stops) are the kernel's:
+      sigma_thresh None: sum over counted samples of
+        w (t / step) ws, w = exp(-sum of earlier ws sigma) (1 - exp(-ws sigma)),
+        sigma trilinear in FP64, with last_sample_opaque off (the reference's
+        depth march has none);
+      sigma_thresh s: (t / step) ws of the first sample whose FP32 density
+        exceeds s, marched without the transmittance stop; 0 if none.
+    Rays that miss the grid get 0."""
+    from dataclasses import replace
+    origins = np.asarray(origins, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    n = origins.shape[0]
+    out = np.zeros(n, np.float64)
+    step = float(np.float32(opt.step_size))
+    if sigma_thresh is not None:
+        o2 = replace(opt, sigma_thresh=float(sigma_thresh), stop_thresh=0.0,
+                     last_sample_opaque=False)
+        smp = march().samples(grid, origins, dirs, o2, clip=clip)
+        hit = np.nonzero(smp["flag"] & 1)[0]
+        ray = smp["ray"][hit]
+        first = np.ones(hit.size, bool)
+        first[1:] = ray[1:] != ray[:-1]
+        k = hit[first]
+        out[smp["ray"][k]] = (smp["t"][k].astype(np.float64) / step) * \
+            smp["ws"][k].astype(np.float64)
+        return out
+    o2 = replace(opt, last_sample_opaque=False)
+    smp = march().samples(grid, origins, dirs, o2, clip=clip)
+    counted = (smp["flag"] & 3) == 3
+    ray = smp["ray"][counted].astype(np.int64)
+    if ray.size == 0:
+        return out
+    lk = corner_gather(grid.links, smp["l"][counted]).astype(np.int64)
+    dens = np.asarray(grid.density, np.float64).reshape(-1)
+    valid = (lk >= 0) & (lk < dens.shape[0])
+    w8 = corner_weights(torch.from_numpy(smp["p"][counted]).double()).numpy() * valid
+    sigma = (w8 * dens[np.where(valid, lk, 0)]).sum(1)
+    ws = smp["ws"][counted].astype(np.float64)
+    od = ws * sigma
+    # exclusive per-ray cumulative optical depth (samples are grouped by ray)
+    csum = np.cumsum(od)
+    first = np.ones(ray.size, bool)
+    first[1:] = ray[1:] != ray[:-1]
+    start = np.maximum.accumulate(np.where(first, np.arange(ray.size), 0))
+    before = csum - od - (csum[start] - od[start])
+    w = np.exp(-before) * (1.0 - np.exp(-od))
+    t = smp["t"][counted].astype(np.float64)
+    np.add.at(out, ray, w * (t / step) * ws)
+    return out
 
 
 def touched_rows(grid: Grid, samples: dict) -> np.ndarray:

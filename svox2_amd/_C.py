@@ -24,7 +24,8 @@ import torch
 __all__ = [
     "SparseGridSpec", "RaysSpec", "CameraSpec", "RenderOptions", "GridOutputGrads",
     "volume_render_cuvol", "volume_render_cuvol_image", "volume_render_cuvol_backward",
-    "volume_render_cuvol_fused", "volume_render_cuvol_stats", "tv_grad_sparse",
+    "volume_render_cuvol_fused", "volume_render_cuvol_stats", "volume_render_expected_term",
+    "volume_render_sigma_thresh", "tv_grad_sparse",
     "rmsprop_step", "sgd_step", "rmsprop_step_grouped", "sample_grid", "sample_grid_backward", "accel_dist_prop",
     "library_path", "load_library", "EXPORTED_SYMBOLS",
 ]
@@ -46,7 +47,8 @@ EXPORTED_SYMBOLS = (
     "svox2_ray_order_workspace_bytes", "svox2_volume_render_cuvol_fused_workspace_bytes",
     "svox2_cell_map_bytes",
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
-    "svox2_volume_render_cuvol_stats", "svox2_sample_grid", "svox2_sample_grid_backward",
+    "svox2_volume_render_cuvol_stats", "svox2_volume_render_expected_term",
+    "svox2_volume_render_sigma_thresh", "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
     "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
@@ -144,6 +146,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp, Gr,
                                             vp, ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
+        "svox2_volume_render_expected_term": [G, vp, vp, vp, i64, O, vp, vp],
+        "svox2_volume_render_sigma_thresh": [G, vp, vp, vp, i64, O, f32, vp, vp],
         "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
         "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
         "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, vp, i64,
@@ -516,6 +520,33 @@ def volume_render_cuvol_stats(grid: SparseGridSpec, rays: RaysSpec,
     with _on_device(o):
         _call("svox2_volume_render_cuvol_stats", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
               o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _stream(o))
+    return out
+
+
+def volume_render_expected_term(grid: SparseGridSpec, rays: RaysSpec,
+                                opt: RenderOptions) -> torch.Tensor:
+    """volume_render_expected_term (render_lerp_kernel_cuvol.cu:1154-1174):
+    (n,) expected termination distance along each ray."""
+    g = _grid_struct(grid, need_sh=False)
+    o, d, clip = _rays(rays)
+    out = torch.empty(o.size(0), dtype=torch.float32, device=o.device)
+    with _on_device(o):
+        _call("svox2_volume_render_expected_term", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(out), _stream(o))
+    return out
+
+
+def volume_render_sigma_thresh(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                               sigma_thresh: float) -> torch.Tensor:
+    """volume_render_sigma_thresh (render_lerp_kernel_cuvol.cu:1176-1197): (n,)
+    distance of the first sample with density above sigma_thresh (0 if none)."""
+    g = _grid_struct(grid, need_sh=False)
+    o, d, clip = _rays(rays)
+    out = torch.empty(o.size(0), dtype=torch.float32, device=o.device)
+    with _on_device(o):
+        _call("svox2_volume_render_sigma_thresh", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
+              o.size(0), ctypes.byref(_opt_struct(opt)), float(sigma_thresh), _ptr(out),
+              _stream(o))
     return out
 
 

@@ -12,6 +12,7 @@
 // Semantics: accel_dist_set_kernel / accel_dist_prop_kernel
 // (svox2/csrc/misc_kernel.cu:109-181) and the host function (:356-416).
 #include "common.hpp"
+#include "fastdiv.hpp"
 
 namespace svox2amd {
 namespace {
@@ -162,29 +163,19 @@ __global__ __launch_bounds__(kBlock) void k_accel_prop(int32_t* __restrict__ lin
 // 16-bit word.  The streaming passes then move 16 bytes per lane.
 constexpr int kVec = 4;
 
+// Splits of the flat cell index (< 2^31) by the row length s2 and by s1, as
+// invariant-divisor multiplies.
 struct CellDiv {
-    uint32_t s2, s1;
-    double inv2, inv1;  // 1.0 / s2, 1.0 / s1
+    FastDiv s2, s1;
 };
-
-// n / d for n < 2^32 and 1 <= d < 2^31, inv = 1 / d in double: the product
-// lies within 2^-20 of n / d, so its truncation is off by at most one, which
-// the remainder test corrects.
-__device__ __forceinline__ uint32_t div_u32(uint32_t n, uint32_t d, double inv) {
-    int64_t q = static_cast<int64_t>(static_cast<uint32_t>(static_cast<double>(n) * inv));
-    const int64_t r = static_cast<int64_t>(n) - q * static_cast<int64_t>(d);
-    q += (r >= static_cast<int64_t>(d)) ? 1 : 0;
-    q -= (r < 0) ? 1 : 0;
-    return static_cast<uint32_t>(q);
-}
 
 __device__ __forceinline__ void cell_xyz(uint32_t cell, const CellDiv& cd, int& x, int& y,
                                          int& z) {
-    const uint32_t row = div_u32(cell, cd.s2, cd.inv2);
-    z = static_cast<int>(cell - row * cd.s2);
-    const uint32_t xq = div_u32(row, cd.s1, cd.inv1);
-    x = static_cast<int>(xq);
-    y = static_cast<int>(row - xq * cd.s1);
+    uint32_t zr, yr;
+    const uint32_t row = cd.s2.divmod(cell, zr);
+    x = static_cast<int>(cd.s1.divmod(row, yr));
+    y = static_cast<int>(yr);
+    z = static_cast<int>(zr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_accel_set4(const int4* __restrict__ links, int64_t n4,
@@ -289,8 +280,8 @@ extern "C" int svox2_accel_dist_prop(int32_t* links, const int32_t size[3], void
     const dim3 grid(static_cast<unsigned>(static_cast<int64_t>(size[0]) * size[1]),
                     blocks_for(size[2], kBlock * kCells));
     const int64_t n4 = cells / 4;
-    const CellDiv cd{static_cast<uint32_t>(size[2]), static_cast<uint32_t>(size[1]),
-                     1.0 / size[2], 1.0 / size[1]};
+    const CellDiv cd{FastDiv(static_cast<uint32_t>(size[2])),
+                     FastDiv(static_cast<uint32_t>(size[1]))};
     const unsigned vblocks = blocks_for(n4, kBlock * kVec);
     if (vec)
         k_accel_set4<<<vblocks, kBlock, 0, s>>>(reinterpret_cast<const int4*>(links), n4, cd, pyr,

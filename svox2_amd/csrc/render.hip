@@ -1470,6 +1470,64 @@ SVOX2_FWD_KERNEL void k_render_rays(
     }
 }
 
+// Depth along each ray (trace_ray_expected_term / trace_ray_sigma_thresh,
+// render_lerp_kernel_cuvol.cu:122-225).  Both walk the colour march's
+// samples: the same placement, skips, densities and (for the expected term)
+// transmittance and stop, so only the accumulator differs.
+//   kDepthExpected: sum over contributing samples of w (t / step) ws, the
+//     expected termination distance along the unit world direction;
+//   kDepthSigma: (t / step) ws of the first sample whose density exceeds the
+//     given threshold (no transmittance or stop; the host passes that
+//     threshold as o.sigma_thresh and a log_stop of -inf).
+// A ray that misses the grid gets 0.  Within a chunk the terms are summed
+// across lanes, then chunk sums in march order.
+constexpr int kDepthExpected = 0;
+constexpr int kDepthSigma = 1;
+
+template <typename DT, int MODE>
+SVOX2_FWD_KERNEL void k_render_depth(GridView<DT> g, OptView o,
+                                     const float* __restrict__ origins,
+                                     const float* __restrict__ dirs,
+                                     const float* __restrict__ clip, int64_t n,
+                                     float* __restrict__ depth_out) {
+    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    Ray r;
+    setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
+    float depth = 0.f;
+    if (r.tmin <= r.tmax) {
+        float t = r.tmin, log_t = 0.f;
+        bool more = true;
+        int guard = 0;
+        while (more && ++guard < kMaxIters) {
+            Chunk c;
+            march_chunk(g, o, r, lane, t, log_t, c);
+            if (MODE == kDepthSigma) {
+                if (c.live_mask != 0) {
+                    const float tj = group_bcast(c.t, __ffs(c.live_mask) - 1);
+                    depth = (tj / o.step) * r.ws;
+                    break;
+                }
+            } else if (c.live_mask != 0) {
+                float term = c.live ? c.weight * (c.t / o.step) * r.ws : 0.f;
+                term += __shfl_xor(term, 1, kGroup);
+                term += __shfl_xor(term, 2, kGroup);
+                term += __shfl_xor(term, 4, kGroup);
+                term += __shfl_xor(term, 8, kGroup);
+                term += __shfl_xor(term, 16, kGroup);
+                depth += term;
+            }
+            log_t = c.log_t_end;
+            t = c.next_t;
+            more = c.more;
+        }
+    }
+    if (lane == 0) depth_out[ray] = depth;
+}
+
 // World ray of pixel (ix, iy) (cam2world_ray, render_util.cuh:526-544; NDC
 // warp world2ndc :504-524 when ndc_coeffx > 0).
 __device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, int iy,
@@ -2020,6 +2078,62 @@ extern "C" int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const
     return render_rays(grid, origins, dirs, ray_clip, n_rays, opt, rgb_out, log_transmit_out,
                        ray_cost_out, workspace, workspace_bytes, stream,
                        "volume_render_cuvol_ordered");
+}
+
+// Depth of a ray batch, mode kDepthExpected or kDepthSigma (threshold
+// sigma_thresh); the render options' last_sample_opaque and background do
+// not apply, as in the reference's depth kernels.
+int render_depth(const svox2_grid_t* grid, const float* origins, const float* dirs,
+                 const float* ray_clip, int64_t n_rays, const svox2_render_options_t* opt,
+                 int mode, float sigma_thresh, float* depth_out, void* stream) {
+    int st = check_grid(grid, false);
+    if (st) return st;
+    OptView o;
+    if ((st = make_opt(opt, o))) return st;
+    if ((st = check_rays(origins, dirs, n_rays))) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(depth_out != nullptr, "depth_out is NULL");
+    o.last_opaque = 0;
+    if (mode == kDepthSigma) {
+        o.sigma_thresh = sigma_thresh;
+        o.log_stop = -INFINITY;
+    }
+    const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    const hipStream_t s = as_stream(stream);
+    auto launch = [&](auto g, auto dtp) {
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        if (mode == kDepthSigma)
+            hipLaunchKernelGGL((k_render_depth<DT, kDepthSigma>), dim3(blocks), dim3(kBlock), 0,
+                               s, g, o, origins, dirs, ray_clip, n_rays, depth_out);
+        else
+            hipLaunchKernelGGL((k_render_depth<DT, kDepthExpected>), dim3(blocks), dim3(kBlock),
+                               0, s, g, o, origins, dirs, ray_clip, n_rays, depth_out);
+    };
+    if (grid->storage == SVOX2_STORAGE_BF16)
+        launch(make_view<uint16_t>(*grid), (uint16_t*)nullptr);
+    else
+        launch(make_view<float>(*grid), (float*)nullptr);
+    return check_launch(mode == kDepthSigma ? "volume_render_sigma_thresh"
+                                            : "volume_render_expected_term");
+}
+
+extern "C" int svox2_volume_render_expected_term(const svox2_grid_t* grid, const float* origins,
+                                                 const float* dirs, const float* ray_clip,
+                                                 int64_t n_rays,
+                                                 const svox2_render_options_t* opt,
+                                                 float* depth_out, void* stream) {
+    return render_depth(grid, origins, dirs, ray_clip, n_rays, opt, kDepthExpected, 0.f,
+                        depth_out, stream);
+}
+
+extern "C" int svox2_volume_render_sigma_thresh(const svox2_grid_t* grid, const float* origins,
+                                                const float* dirs, const float* ray_clip,
+                                                int64_t n_rays,
+                                                const svox2_render_options_t* opt,
+                                                float sigma_thresh, float* depth_out,
+                                                void* stream) {
+    return render_depth(grid, origins, dirs, ray_clip, n_rays, opt, kDepthSigma, sigma_thresh,
+                        depth_out, stream);
 }
 
 extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const float* origins,

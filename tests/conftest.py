@@ -85,3 +85,18 @@ def gpu():
     from svox2_amd import _C
     _C.load_library()  # raises if the kernel library is missing
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def sparse256(gpu):
+    """(SceneData, accelerated GPU grid, its links, origins, dirs) of the
+    256^3 blob grid (about 8 % occupied) with 4096 camera rays; the grid's
+    thresholds are sigma 1e-8, stop 1e-7."""
+    from svox2_amd import scenes
+    sc, cams, gen = scenes.workload_blobs(256, scenes.SPARSE256, seed=0, n_cameras=20,
+                                          device=gpu)
+    grid = scenes.to_sparse_grid(sc, device=gpu)
+    grid.opt.sigma_thresh, grid.opt.stop_thresh = 1e-8, 1e-7
+    links_acc = grid.links.cpu().numpy()
+    o, d = cams.random_batch(4096, gen)
+    return sc, grid, links_acc, o, d

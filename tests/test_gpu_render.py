@@ -86,16 +86,6 @@ def test_bf16_storage_forward(gpu, dense64, march):
     assert np.array_equal(st, st_ref)
 
 
-@pytest.fixture(scope="module")
-def sparse256(gpu):
-    sc, cams, gen = scenes.workload_blobs(256, scenes.SPARSE256, seed=0, n_cameras=20,
-                                          device=gpu)
-    grid = _grid(sc, gpu)
-    links_acc = grid.links.cpu().numpy()
-    o, d = cams.random_batch(4096, gen)
-    return sc, grid, links_acc, o, d
-
-
 def test_sparse256_accel_and_stats(gpu, sparse256, march):
     sc, grid, links_acc, o, d = sparse256
     assert np.array_equal(links_acc, O.accel_dist_prop(sc.links.cpu().numpy()))

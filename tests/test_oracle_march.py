@@ -300,3 +300,53 @@ def test_compacted_blocks_match_whole_grid(march):
         assert np.array_equal(rgb, rgb_w.detach().numpy())
         assert np.abs(gd - dens.grad.numpy()).max() <= 1e-12 * np.abs(dens.grad.numpy()).max()
         assert np.abs(gs - sh.grad.numpy()).max() <= 1e-12 * np.abs(sh.grad.numpy()).max()
+
+
+def _axis_grid(N, density):
+    links = np.arange(N ** 3, dtype=np.int32).reshape(N, N, N)
+    sh = np.zeros((N ** 3, 27))
+    return O.Grid(links, np.asarray(density, np.float64).reshape(-1), sh, 9,
+                  offset=np.zeros(3), scaling=np.ones(3))
+
+
+@pytest.mark.parametrize("sigma,stop", [(0.2, 1e-7), (5.0, 1e-7), (5.0, 0.0)])
+def test_depth_expected_closed_form(march, sigma, stop):
+    """Expected termination on a uniform field, world = grid coordinates, a
+    ray along +x entering at x = -0.5: samples at t_k = 2.5 + k/2 up to
+    t = 18.5 (x = 15.5), world step 1/2, weights a^k (1 - a) with
+    a = exp(-sigma/2), the ray stopping after the first sample whose
+    transmittance a^(k+1) falls below stop_thresh; depth = sum w_k t_k."""
+    N = 16
+    g = _axis_grid(N, np.full(N ** 3, sigma))
+    o = np.array([[-3.0, 7.25, 8.5], [-3.0, 7.25, 8.5]], np.float32)
+    d = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]], np.float32)  # the second points away
+    opt = O.Opts(sigma_thresh=1e-8, stop_thresh=stop)
+    k = np.arange(33)
+    t = 2.5 + 0.5 * k
+    a = np.exp(-sigma * 0.5)
+    w = a ** k * (1 - a)
+    if stop > 0:
+        stopped = a ** (k + 1) < stop
+        if stopped.any():
+            last = int(np.argmax(stopped))
+            w = w[:last + 1]
+            t = t[:last + 1]
+    got = O.depth_fp64(g, o, d, opt)
+    assert got[0] == pytest.approx(float((w * t).sum()), rel=1e-12)
+    assert got[1] == 0.0
+
+
+def test_depth_sigma_thresh_closed_form(march):
+    """First crossing on a linear ramp: density = x index, so the trilinear
+    density at x is x itself (exact in float32); with threshold 6.3 the
+    first sample above it is x = 6.5, at t = 9.5 along the ray from x = -3,
+    depth (t / step) ws = 9.5.  A threshold above every density gives 0."""
+    N = 16
+    x = np.arange(N, dtype=np.float64)[:, None, None] * np.ones((1, N, N))
+    g = _axis_grid(N, x)
+    o = np.array([[-3.0, 7.25, 8.5]], np.float32)
+    d = np.array([[1.0, 0.0, 0.0]], np.float32)
+    opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
+    assert O.depth_fp64(g, o, d, opt, sigma_thresh=6.3)[0] == 9.5
+    assert O.depth_fp64(g, o, d, opt, sigma_thresh=0.25)[0] == 3.5  # x = 0.5
+    assert O.depth_fp64(g, o, d, opt, sigma_thresh=20.0)[0] == 0.0
