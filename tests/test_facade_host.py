@@ -178,3 +178,26 @@ def test_scene_generators_deterministic():
     assert torch.all(o[:, 2] >= 0)
     # rays point towards the scene
     assert torch.all((-o * d).sum(1) > 0)
+
+
+def test_load_legacy_data_layout(tmp_path):
+    """The reference's legacy checkpoint key "data" ((rows, 1 + C): density
+    first, then SH; svox2.py:1560-1564), with no radius / center (unit radius
+    at the origin) loads on the CPU with the values in place."""
+    rng = np.random.default_rng(0)
+    links = np.full((6, 5, 4), -1, np.int32)
+    links[1:4, 2, 1:3] = np.arange(6).reshape(3, 2)
+    data = rng.standard_normal((6, 28)).astype(np.float32)
+    path = str(tmp_path / "legacy.npz")
+    np.savez(path, links=links, data=data)
+    g = svox2_amd.SparseGrid.load(path)
+    assert g.links.shape == (6, 5, 4) and torch.equal(g.links, torch.from_numpy(links))
+    assert np.array_equal(g.density_data.data.numpy(), data[:, :1])
+    assert np.array_equal(g.sh_data.data.numpy(), data[:, 1:])
+    assert g.basis_dim == 9 and g.capacity == 6
+    assert torch.equal(g.radius, torch.ones(3)) and torch.equal(g.center, torch.zeros(3))
+    # background models and other bases are refused, not half-loaded
+    np.savez(path, links=links, data=data, background_data=np.zeros((1, 2, 4), np.float32),
+             background_links=np.zeros((2, 1), np.int32))
+    with pytest.raises(NotImplementedError):
+        svox2_amd.SparseGrid.load(path)
