@@ -343,6 +343,63 @@ int svox2_accel_dist_prop(int32_t* links, const int32_t size[3],
                           void* workspace, size_t workspace_bytes,
                           void* stream);
 
+/* ---- Grid resampling (SparseGrid.resample, svox2.py:1223-1424) ----------- */
+
+/* Density of every cell of a new reso[0] x reso[1] x reso[2] grid, sampled
+ * trilinearly from `grid` (as sample_grid with grid coordinates) at the old
+ * grid's coordinates (axes[ix], axes[reso0 + iy], axes[reso0 + reso1 + iz]);
+ * density_out is the dense raster-order volume.  No counterpart of its own in
+ * the reference: it is resample's first sample() pass (svox2.py:1298-1307). */
+int svox2_resample_density(const svox2_grid_t* grid, const float* axes,
+                           const int32_t reso[3], float* density_out, void* stream);
+
+/* Rows of the new grid: for row r, SH channels sampled from `grid` at kept
+ * cell kept_cells[r]'s point and the density density_volume[kept_cells[r]],
+ * written in out_storage (SVOX2_STORAGE_*) with row strides in elements.
+ * Resample's second sample() pass (svox2.py:1383-1392). */
+int svox2_resample_rows(const svox2_grid_t* grid, const float* axes, const int32_t reso[3],
+                        const int32_t* kept_cells, int64_t n_rows,
+                        const float* density_volume, void* sh_out, int64_t sh_stride,
+                        void* density_out, int64_t density_stride, int32_t out_storage,
+                        void* stream);
+
+/* Raises grid_weight_out (a dense float volume of data's size, holding
+ * non-negative values) to the maximum compositing weight of any sample of the
+ * camera's pixel rays through the dense density volume `data`, at the 8
+ * corners of the sample's cell; offset / scaling map world to volume
+ * coordinates.  Replaces grid_weight_render (svox2.cpp:55, bound at :106;
+ * misc_kernel.cu:418). */
+int svox2_grid_weight_render(const float* data, const int32_t size[3],
+                             const svox2_camera_t* cam, float step_size,
+                             float stop_thresh, int32_t last_sample_opaque,
+                             const float offset[3], const float scaling[3],
+                             float* grid_weight_out, void* stream);
+
+/* mask_out[i] = values[i] >= thresh, or 1 on the last z slice when
+ * keep_last_z.  Resample's thresholding (svox2.py:1349-1371). */
+int svox2_threshold_mask(const float* values, const int32_t size[3], float thresh,
+                         int32_t keep_last_z, uint8_t* mask_out, void* stream);
+
+/* One 3x3x3 dilation of a byte mask (nonzero = set), out of place.
+ * Replaces dilate (svox2.cpp:53, bound at :104; misc_kernel.cu:339). */
+int svox2_dilate(const uint8_t* mask, const int32_t size[3], uint8_t* out, void* stream);
+
+/* Workspace bytes of the mask compaction for a volume of this size. */
+size_t svox2_mask_compact_workspace_bytes(const int32_t size[3]);
+
+/* Compaction of a byte mask into links, pass 1: counts the set cells in order
+ * (raster, or Morton when z_order: cubic power-of-two side <= 1024) into the
+ * workspace; the number of set cells is then the uint32 at word
+ * ceil(cells / 4096) of the workspace.  Pass 2 (svox2_mask_compact_assign)
+ * gives the k-th set cell in that order link k, every other cell -1, and
+ * (if kept_cells_out is not NULL) kept_cells_out[k] = its flat cell index.
+ * Resample's link numbering (svox2.py:1397-1412). */
+int svox2_mask_compact_count(const uint8_t* mask, const int32_t size[3], int32_t z_order,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int svox2_mask_compact_assign(const uint8_t* mask, const int32_t size[3], int32_t z_order,
+                              const void* workspace, int32_t* links_out,
+                              int32_t* kept_cells_out, void* stream);
+
 /* Bytes of the cell map of a links table of this size: one 16-bit entry per
  * 2x2x2 block of cells, ((size[i] - 2) / 2 + 1) blocks per axis.  No
  * counterpart in the reference. */

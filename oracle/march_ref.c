@@ -23,6 +23,8 @@
  *                the GPU stats kernel produces;
  *   ref_samples  every evaluated (not skipped) sample of every ray, ignoring
  *                the stop test, for the FP64 oracle to integrate;
+ *   ref_grid_weight_render  per-voxel maximum weight of a camera's rays
+ *                through a dense density volume (resample's weight render);
  *   ref_fused    forward colour plus the MSE gradient (render_ray_backward_kernel
  *                with grad_out_is_rgb, render_lerp_kernel_cuvol.cu:692-773, and
  *                trace_ray_cuvol_backward :227-383), scattering every
@@ -359,23 +361,99 @@ int ref_ray_setup(const ref_grid_t* g, const float* origins, const float* dirs,
 
 /* World ray of pixel (ix, iy) of a pinhole camera (c2w 3x4 row-major), as
  * cam2world_ray (render_util.cuh:526-544) without the NDC warp. */
+static void pixel_ray(const float* c2w, float fx, float fy, float cx, float cy, int ix, int iy,
+                      float* origin, float* dir) {
+    float x = ((float)ix + 0.5f - cx) / fx;
+    float y = ((float)iy + 0.5f - cy) / fy;
+    float z = sqrtf((float)((double)(x * x + y * y) + 1.0));
+    x /= z;
+    y /= z;
+    z = 1.0f / z;
+    dir[0] = c2w[0] * x + c2w[1] * y + c2w[2] * z;
+    dir[1] = c2w[4] * x + c2w[5] * y + c2w[6] * z;
+    dir[2] = c2w[8] * x + c2w[9] * y + c2w[10] * z;
+    origin[0] = c2w[3];
+    origin[1] = c2w[7];
+    origin[2] = c2w[11];
+}
+
+/* All pixel rays of a camera, row-major. */
 int ref_camera_rays(const float* c2w, float fx, float fy, float cx, float cy, int32_t width,
                     int32_t height, float* origins, float* dirs) {
     for (int iy = 0; iy < height; ++iy) {
         for (int ix = 0; ix < width; ++ix) {
-            float x = ((float)ix + 0.5f - cx) / fx;
-            float y = ((float)iy + 0.5f - cy) / fy;
-            float z = sqrtf((float)((double)(x * x + y * y) + 1.0));
-            x /= z;
-            y /= z;
-            z = 1.0f / z;
             const int64_t i = (int64_t)iy * width + ix;
-            dirs[3 * i + 0] = c2w[0] * x + c2w[1] * y + c2w[2] * z;
-            dirs[3 * i + 1] = c2w[4] * x + c2w[5] * y + c2w[6] * z;
-            dirs[3 * i + 2] = c2w[8] * x + c2w[9] * y + c2w[10] * z;
-            origins[3 * i + 0] = c2w[3];
-            origins[3 * i + 1] = c2w[7];
-            origins[3 * i + 2] = c2w[11];
+            pixel_ray(c2w, fx, fy, cx, cy, ix, iy, origins + 3 * i, dirs + 3 * i);
+        }
+    }
+    return 0;
+}
+
+/* Per-voxel maximum compositing weight of one camera's pixel rays through
+ * the dense float volume data (size[0] x size[1] x size[2]), raised into out
+ * (same shape, non-negative): grid_trace_ray / grid_weight_render_kernel
+ * (misc_kernel.cu:184-334,418-445).  Sequential per ray: t from the volume
+ * entry in steps of step_size (no skip), position fmaf(t, d, o) clamped,
+ * trilinear density; a sample with density > 1e-8 has weight
+ * exp(logT) (1 - exp(-ws sigma)), raises the 8 corners of its cell, and
+ * stops the ray when logT falls below log(stop_thresh).  With
+ * last_sample_opaque the last sample gets density + 1e9 and logT reset to 0. */
+int ref_grid_weight_render(const float* data, const int32_t* size, const float* c2w, float fx,
+                           float fy, float cx, float cy, int32_t width, int32_t height,
+                           float step_size, float stop_thresh, int32_t last_opaque,
+                           const float* offset, const float* scaling, float* out) {
+    ref_grid_t g;
+    memset(&g, 0, sizeof(g));
+    for (int i = 0; i < 3; ++i) {
+        g.size[i] = size[i];
+        g.offset[i] = offset[i];
+        g.scaling[i] = scaling[i];
+    }
+    ref_opt_t o;
+    memset(&o, 0, sizeof(o));
+    o.step = step_size;
+    const float log_stop = log_stop_threshold(stop_thresh);
+    const int64_t sx = (int64_t)size[1] * size[2], sy = size[2];
+    for (int iy = 0; iy < height; ++iy) {
+        for (int ix = 0; ix < width; ++ix) {
+            float wo[3], wd[3];
+            pixel_ray(c2w, fx, fy, cx, cy, ix, iy, wo, wd);
+            ray_t r;
+            setup_ray(&g, &o, wo, wd, NULL, &r);
+            if (r.tmin > r.tmax) continue;
+            float t = r.tmin, log_t = 0.f;
+            int guard = 0;
+            while (t <= r.tmax && ++guard < (1 << 22)) {
+                float p[3];
+                int l[3];
+                for (int a = 0; a < 3; ++a) {
+                    float x = fmaf(t, r.d[a], r.o[a]);
+                    x = fminf(fmaxf(x, 0.f), (float)size[a] - 1.f);
+                    l[a] = (int)x;
+                    if (l[a] > size[a] - 2) l[a] = size[a] - 2;
+                    p[a] = x - (float)l[a];
+                }
+                const int64_t base = l[0] * sx + l[1] * sy + l[2];
+                const float v[8] = {data[base], data[base + 1], data[base + sy],
+                                    data[base + sy + 1], data[base + sx], data[base + sx + 1],
+                                    data[base + sx + sy], data[base + sx + sy + 1]};
+                float sigma = trilerp8(v, p);
+                if (last_opaque && t + step_size > r.tmax) {
+                    sigma += 1e9f;
+                    log_t = 0.f;
+                }
+                if (sigma > 1e-8f) {
+                    const float pcnt = r.ws * sigma;
+                    const float w = expf(log_t) * (1.f - expf(-pcnt));
+                    log_t -= pcnt;
+                    const int64_t c[8] = {base, base + 1, base + sy, base + sy + 1, base + sx,
+                                          base + sx + 1, base + sx + sy, base + sx + sy + 1};
+                    for (int k = 0; k < 8; ++k)
+                        if (w > out[c[k]]) out[c[k]] = w;
+                    if (log_t < log_stop) break;
+                }
+                t += step_size;
+            }
         }
     }
     return 0;

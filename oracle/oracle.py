@@ -140,6 +140,11 @@ class MarchRef:
         lib.ref_fused.argtypes = [G, vp, vp, vp, i64, O, vp, ctypes.c_float, ctypes.c_float,
                                   vp, vp, vp, vp]
         lib.ref_fused.restype = ctypes.c_int
+        f32 = ctypes.c_float
+        lib.ref_grid_weight_render.argtypes = [vp, vp, vp, f32, f32, f32, f32, ctypes.c_int32,
+                                               ctypes.c_int32, f32, f32, ctypes.c_int32, vp, vp,
+                                               vp]
+        lib.ref_grid_weight_render.restype = ctypes.c_int
         self.lib = lib
 
     @staticmethod
@@ -273,6 +278,22 @@ class MarchRef:
         d = np.zeros((width * height, 3), np.float32)
         self.lib.ref_camera_rays(_c_ptr(m), fx, fy, cx, cy, width, height, _c_ptr(o), _c_ptr(d))
         return o, d
+
+    def grid_weight_render(self, data, c2w, fx, fy, cx, cy, width, height, step_size,
+                           stop_thresh, last_opaque, offset, scaling, out=None):
+        """Raises out (float32, data's shape; zeros if None) to the per-voxel
+        maximum ray weight of one camera through the dense volume data."""
+        data = np.ascontiguousarray(data, np.float32)
+        out = np.zeros_like(data) if out is None else out
+        assert out.dtype == np.float32 and out.shape == data.shape and out.flags.c_contiguous
+        size = np.array(data.shape, np.int32)
+        m = np.ascontiguousarray(np.asarray(c2w, dtype=np.float32)[:3, :4])
+        off = np.asarray(offset, np.float32).reshape(3).copy()
+        sc = np.asarray(scaling, np.float32).reshape(3).copy()
+        self.lib.ref_grid_weight_render(_c_ptr(data), _c_ptr(size), _c_ptr(m), fx, fy, cx, cy,
+                                        width, height, step_size, stop_thresh, int(last_opaque),
+                                        _c_ptr(off), _c_ptr(sc), _c_ptr(out))
+        return out
 
 
 _MARCH: Optional[MarchRef] = None
@@ -787,6 +808,58 @@ def sgd_step(data, grad, rows_sel, lr, lr_last):
     data[rows_sel] -= lrs * grad[rows_sel]
     grad[rows_sel] = 0.0
     return data, grad
+
+
+def dilate(mask: np.ndarray) -> np.ndarray:
+    """One 3x3x3 dilation of a boolean volume, neighbours clipped to the grid
+    (dilate_kernel, misc_kernel.cu:20-48): the OR of the 27 shifted copies of
+    the volume padded with False."""
+    m = np.asarray(mask, bool)
+    pad = np.pad(m, 1, constant_values=False)
+    out = np.zeros_like(m)
+    X, Y, Z = m.shape
+    for dx in range(3):
+        for dy in range(3):
+            for dz in range(3):
+                out |= pad[dx:dx + X, dy:dy + Y, dz:dz + Z]
+    return out
+
+
+def mask_to_links(mask: np.ndarray, z_order: bool = False):
+    """(links, kept cells) of a boolean volume (svox2.py:1397-1412): the k-th
+    kept cell in raster order, or in Morton order with z_order, gets link k;
+    the others -1; kept[k] is its flat raster index."""
+    m = np.asarray(mask, bool)
+    flat = m.reshape(-1)
+    links = np.full(flat.size, -1, np.int32)
+    if z_order:
+        code = gen_morton(m.shape[0]).reshape(-1)  # Morton code of each raster cell
+        order = np.argsort(code)                   # raster cells in Morton order
+        kept = order[flat[order]]
+    else:
+        kept = np.nonzero(flat)[0]
+    links[kept] = np.arange(kept.size, dtype=np.int32)
+    return links.reshape(m.shape), kept.astype(np.int64)
+
+
+def resample_axes(old_size, new_size) -> np.ndarray:
+    """The new cells' coordinates along each axis in old grid coordinates,
+    as svox2.py:1275-1294 computes them (float32 linspace per axis),
+    concatenated x | y | z."""
+    axes = []
+    for a in range(3):
+        f = 0.5 * old_size[a] / new_size[a]
+        axes.append(torch.linspace(f - 0.5, old_size[a] - f - 0.5, new_size[a],
+                                   dtype=torch.float32).numpy())
+    return np.concatenate(axes)
+
+
+def resample_points(axes: np.ndarray, new_size, cells: np.ndarray) -> np.ndarray:
+    """Grid-coordinate points (n, 3) of flat new-grid cells."""
+    X, Y, Z = new_size
+    ax, ay, az = axes[:X], axes[X:X + Y], axes[X + Y:]
+    cells = np.asarray(cells, np.int64)
+    return np.stack([ax[cells // (Y * Z)], ay[(cells // Z) % Y], az[cells % Z]], 1)
 
 
 def accel_dist_prop(links: np.ndarray) -> np.ndarray:

@@ -26,7 +26,8 @@ __all__ = [
     "volume_render_cuvol", "volume_render_cuvol_image", "volume_render_cuvol_backward",
     "volume_render_cuvol_fused", "volume_render_cuvol_stats", "volume_render_expected_term",
     "volume_render_sigma_thresh", "tv_grad_sparse",
-    "rmsprop_step", "sgd_step", "rmsprop_step_grouped", "sample_grid", "sample_grid_backward", "accel_dist_prop",
+    "rmsprop_step", "sgd_step", "rmsprop_step_grouped", "dilate", "grid_weight_render",
+    "resample_density", "resample_rows", "threshold_mask", "mask_to_links", "sample_grid", "sample_grid_backward", "accel_dist_prop",
     "library_path", "load_library", "EXPORTED_SYMBOLS",
 ]
 
@@ -51,7 +52,10 @@ EXPORTED_SYMBOLS = (
     "svox2_volume_render_sigma_thresh", "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
     "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
-    "svox2_accel_dist_prop", "svox2_last_error", "svox2_version",
+    "svox2_accel_dist_prop", "svox2_resample_density", "svox2_resample_rows",
+    "svox2_grid_weight_render", "svox2_threshold_mask", "svox2_dilate",
+    "svox2_mask_compact_workspace_bytes", "svox2_mask_compact_count",
+    "svox2_mask_compact_assign", "svox2_last_error", "svox2_version",
 )
 
 
@@ -159,6 +163,15 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
                                        f32, f32, f32, f32, f32, f32, vp],
         "svox2_accel_dist_prop": [vp, ctypes.POINTER(i32), vp, ctypes.c_size_t, vp],
         "svox2_build_cell_map": [vp, ctypes.POINTER(i32), i64, vp, ctypes.c_size_t, vp],
+        "svox2_resample_density": [G, vp, ctypes.POINTER(i32), vp, vp],
+        "svox2_resample_rows": [G, vp, ctypes.POINTER(i32), vp, i64, vp, vp, i64, vp, i64, i32,
+                                vp],
+        "svox2_grid_weight_render": [vp, ctypes.POINTER(i32), Cm, f32, f32, i32,
+                                     ctypes.POINTER(f32), ctypes.POINTER(f32), vp, vp],
+        "svox2_threshold_mask": [vp, ctypes.POINTER(i32), f32, i32, vp, vp],
+        "svox2_dilate": [vp, ctypes.POINTER(i32), vp, vp],
+        "svox2_mask_compact_count": [vp, ctypes.POINTER(i32), i32, vp, ctypes.c_size_t, vp],
+        "svox2_mask_compact_assign": [vp, ctypes.POINTER(i32), i32, vp, vp, vp, vp],
         "svox2_volume_render_cuvol_tiles": [vp, i32, i32, i32, vp, i32, O, vp, vp, i32, i32,
                                             vp],
     }
@@ -172,6 +185,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.svox2_volume_render_cuvol_fused_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_ray_order_workspace_bytes.argtypes = [i64]
     lib.svox2_ray_order_workspace_bytes.restype = ctypes.c_size_t
+    lib.svox2_mask_compact_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
+    lib.svox2_mask_compact_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
     lib.svox2_accel_dist_prop_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_last_error.restype = ctypes.c_char_p
@@ -816,6 +831,117 @@ def accel_dist_prop(links: torch.Tensor) -> None:
               _stream(links))
     key = (links.device.index, links.data_ptr())
     _ACCEL_EPOCH[key] = _ACCEL_EPOCH.get(key, 0) + 1
+
+
+# ---------------------------------------------------------------------------
+# Resampling pieces
+# ---------------------------------------------------------------------------
+def _size3(t: torch.Tensor):
+    return (ctypes.c_int32 * 3)(*t.shape)
+
+
+def dilate(grid: torch.Tensor) -> torch.Tensor:
+    """dilate (misc_kernel.cu:339-354): a new bool/uint8 (X, Y, Z) mask, set
+    where any cell of the 3x3x3 neighbourhood (clipped to the grid) is set."""
+    _check(grid, "grid", (torch.bool, torch.uint8), 3)
+    out = torch.empty_like(grid)
+    with _on_device(grid):
+        _call("svox2_dilate", _ptr(grid), _size3(grid), _ptr(out), _stream(grid))
+    return out
+
+
+def grid_weight_render(data: torch.Tensor, cam: CameraSpec, step_size: float,
+                       stop_thresh: float, last_sample_opaque: bool, offset, scaling,
+                       grid_weight_out: torch.Tensor) -> None:
+    """grid_weight_render (misc_kernel.cu:418-445): raises grid_weight_out
+    (float32, data's shape, non-negative) to the per-voxel maximum weight of
+    the camera's rays through the dense density volume ``data``.  offset and
+    scaling (3 values each, tensors on any device or sequences) map world to
+    volume coordinates."""
+    _check(data, "data", (torch.float32,), 3)
+    _check(grid_weight_out, "grid_weight_out", (torch.float32,), 3)
+    if grid_weight_out.shape != data.shape:
+        raise RuntimeError("grid_weight_out must match data's shape")
+
+    def vec(v):
+        v = v.detach().double().cpu().flatten().tolist() if isinstance(v, torch.Tensor) else v
+        if len(v) != 3:
+            raise RuntimeError("offset / scaling must have 3 entries")
+        return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+    with _on_device(data):
+        _call("svox2_grid_weight_render", _ptr(data), _size3(data),
+              ctypes.byref(_cam_struct(cam)), float(step_size), float(stop_thresh),
+              int(bool(last_sample_opaque)), vec(offset), vec(scaling), _ptr(grid_weight_out),
+              _stream(data))
+
+
+def resample_density(grid: SparseGridSpec, axes: torch.Tensor, reso) -> torch.Tensor:
+    """svox2_resample_density: (reso) float32 volume of the grid's density at
+    the new cells' points (axes = [x coords | y coords | z coords], grid
+    coordinates of the old grid, float32 on the grid's device)."""
+    g = _grid_struct(grid, need_sh=False, need_map=False)
+    _check(axes, "axes", (torch.float32,), 1)
+    reso = [int(r) for r in reso]
+    if axes.numel() != sum(reso):
+        raise RuntimeError("axes must hold reso[0] + reso[1] + reso[2] values")
+    out = torch.empty(reso, dtype=torch.float32, device=axes.device)
+    with _on_device(axes):
+        _call("svox2_resample_density", ctypes.byref(g), _ptr(axes), (ctypes.c_int32 * 3)(*reso),
+              _ptr(out), _stream(axes))
+    return out
+
+
+def resample_rows(grid: SparseGridSpec, axes: torch.Tensor, reso, kept_cells: torch.Tensor,
+                  density_volume: torch.Tensor, sh_out: torch.Tensor,
+                  density_out: torch.Tensor) -> None:
+    """svox2_resample_rows: fills the new grid's rows (sh_out (n, C),
+    density_out (n, 1), float32 or bf16, row-strided) from the kept cells."""
+    g = _grid_struct(grid, need_sh=True, need_map=False)
+    _check(axes, "axes", (torch.float32,), 1)
+    _check(kept_cells, "kept_cells", (torch.int32,), 1)
+    _check(density_volume, "density_volume", (torch.float32,))
+    _check_rows(sh_out, "sh_out", _DATA_DTYPES, g.sh_data_dim)
+    _check_rows(density_out, "density_out", (sh_out.dtype,), 1)
+    n = kept_cells.numel()
+    if sh_out.size(0) != n or density_out.size(0) != n:
+        raise RuntimeError("sh_out / density_out must have one row per kept cell")
+    with _on_device(axes):
+        _call("svox2_resample_rows", ctypes.byref(g), _ptr(axes),
+              (ctypes.c_int32 * 3)(*[int(r) for r in reso]), _ptr(kept_cells), n,
+              _ptr(density_volume), _ptr(sh_out), _row_stride(sh_out), _ptr(density_out),
+              _row_stride(density_out), _storage_of(sh_out), _stream(axes))
+
+
+def threshold_mask(values: torch.Tensor, thresh: float, keep_last_z: bool = False) -> torch.Tensor:
+    """svox2_threshold_mask: bool volume values >= thresh (last z slice set
+    when keep_last_z)."""
+    _check(values, "values", (torch.float32,), 3)
+    out = torch.empty(values.shape, dtype=torch.bool, device=values.device)
+    with _on_device(values):
+        _call("svox2_threshold_mask", _ptr(values), _size3(values), float(thresh),
+              int(bool(keep_last_z)), _ptr(out), _stream(values))
+    return out
+
+
+def mask_to_links(mask: torch.Tensor, z_order: bool = False):
+    """(links (X, Y, Z) int32, kept cells (n,) int32): the k-th set cell in
+    raster (or Morton, z_order) order gets link k, the others -1; kept[k] is
+    its flat cell index.  One host synchronisation reads the count."""
+    _check(mask, "mask", (torch.bool, torch.uint8), 3)
+    lib = load_library()
+    size = _size3(mask)
+    nbytes = int(lib.svox2_mask_compact_workspace_bytes(size))
+    ws = torch.empty(nbytes // 4, dtype=torch.int32, device=mask.device)
+    with _on_device(mask):
+        _call("svox2_mask_compact_count", _ptr(mask), size, int(bool(z_order)), _ptr(ws),
+              ctypes.c_size_t(nbytes), _stream(mask))
+        n = int(ws[-1].item())
+        links = torch.empty(mask.shape, dtype=torch.int32, device=mask.device)
+        kept = torch.empty(n, dtype=torch.int32, device=mask.device)
+        _call("svox2_mask_compact_assign", _ptr(mask), size, int(bool(z_order)), _ptr(ws),
+              _ptr(links), _ptr(kept) if n > 0 else None, _stream(mask))
+    return links, kept
 
 
 # ---------------------------------------------------------------------------

@@ -1563,6 +1563,100 @@ __device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, in
     }
 }
 
+// t of lane k of a chunk whose lane 0 has t0: the float chain t0 + step + ...
+// + step (k additions), by one fma when every addition is exact (see
+// place_chunk), else replayed.
+__device__ __forceinline__ float chain_t(float t0, int k, const OptView& o) {
+    if (sum_chain_exact(t0, o.step, o.step_ulp_max))
+        return fmaf(static_cast<float>(k), o.step, t0);
+    float t = t0;
+#pragma unroll
+    for (int kk = 1; kk < 32; ++kk) {
+        const float nx = t + o.step;
+        t = (kk <= k) ? nx : t;
+    }
+    return t;
+}
+
+// Per-voxel maximum compositing weight of the pixel rays of one camera
+// through a dense density volume (grid_trace_ray / grid_weight_render_kernel,
+// misc_kernel.cu:184-334,418-445): every step of size step_size from the
+// volume's entry (no empty-space skip), trilinear density, a sample with
+// density > 1e-8 adds w = T (1 - exp(-ws sigma)) and raises the weight of the
+// 8 corners of its cell to at least w; the ray stops after the first sample
+// that leaves T below stop_thresh.  With last_sample_opaque the last sample
+// before tmax gets density + 1e9 and T reset to 1 (weight 1).
+//
+// 32 lanes per pixel take 32 consecutive steps at a time; the log
+// transmittance is replayed in step order with the sequential march's float
+// subtractions, so the stop decision and every weight are those of a
+// sequential walk.  Weights are non-negative floats, whose bit patterns order
+// as integers: the per-corner maximum is an integer atomic max.
+template <bool OPAQUE>
+SVOX2_FWD_KERNEL void k_grid_weight_render(GridView<float> g, OptView o, svox2_camera_t cam,
+                                           const float* __restrict__ data,
+                                           int* __restrict__ weight_out) {
+    const int64_t pix = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (pix >= static_cast<int64_t>(cam.width) * cam.height) return;
+    const int iy = static_cast<int>(pix / cam.width), ix = static_cast<int>(pix % cam.width);
+    float wd[3], wo[3];
+    camera_ray(cam, ix, iy, wd, wo);
+    Ray r;
+    setup_ray(g, o, wo, wd, nullptr, r);
+    if (r.tmin > r.tmax) return;
+    const int sx = g.stride_x, sy = g.stride_y;
+    float t0 = r.tmin, log_t = 0.f;
+    int guard = 0;
+    while (++guard < kMaxIters) {
+        const float t = chain_t(t0, lane, o);
+        const bool in_range = t <= r.tmax;
+        float p[3];
+        int l[3];
+        float sigma = 0.f;
+        int base = 0;
+        if (in_range) {
+            base = sample_cell(g, r, t, p, l);
+            const float v[8] = {data[base], data[base + 1], data[base + sy], data[base + sy + 1],
+                                data[base + sx], data[base + sx + 1], data[base + sx + sy],
+                                data[base + sx + sy + 1]};
+            sigma = trilerp8(v, p);
+        }
+        const bool last = OPAQUE && in_range && t + o.step > r.tmax;
+        if (last) sigma += 1e9f;
+        const bool contrib = in_range && sigma > 1e-8f;
+        const float pcnt = contrib ? r.ws * sigma : 0.f;
+        float lt = log_t;
+        const int n_rep = wave_last_set(contrib);
+        for (int j = 0; j < n_rep; ++j) {
+            const float pj = group_lane(pcnt, j);
+            if (j < lane) lt -= pj;
+        }
+        if (last) lt = 0.f;
+        const float lt_after = lt - pcnt;
+        const uint32_t stop_mask = group_ballot(contrib && lt_after < o.log_stop);
+        const int first_stop = stop_mask ? (__ffs(stop_mask) - 1) : 32;
+        if (contrib && lane <= first_stop) {
+            const float w = __expf(lt) * (1.f - __expf(-pcnt));
+            const int wb = __float_as_int(w);
+            int* q = weight_out + base;
+            atomicMax(q, wb);
+            atomicMax(q + 1, wb);
+            atomicMax(q + sy, wb);
+            atomicMax(q + sy + 1, wb);
+            atomicMax(q + sx, wb);
+            atomicMax(q + sx + 1, wb);
+            atomicMax(q + sx + sy, wb);
+            atomicMax(q + sx + sy + 1, wb);
+        }
+        const uint32_t range_mask = group_ballot(in_range);
+        if (first_stop < 32 || range_mask != 0xFFFFFFFFu) break;
+        log_t = group_lane(lt_after, 31);
+        t0 = group_lane(t, 31) + o.step;
+    }
+}
+
+
 template <typename DT, int B>
 SVOX2_FWD_KERNEL void k_render_image(
         GridView<DT> g, OptView o, svox2_camera_t cam, float* __restrict__ rgb_out,
@@ -2115,6 +2209,46 @@ int render_depth(const svox2_grid_t* grid, const float* origins, const float* di
         launch(make_view<float>(*grid), (float*)nullptr);
     return check_launch(mode == kDepthSigma ? "volume_render_sigma_thresh"
                                             : "volume_render_expected_term");
+}
+
+extern "C" int svox2_grid_weight_render(const float* data, const int32_t size[3],
+                                        const svox2_camera_t* cam, float step_size,
+                                        float stop_thresh, int32_t last_sample_opaque,
+                                        const float offset[3], const float scaling[3],
+                                        float* grid_weight_out, void* stream) {
+    SVOX2_REQUIRE(data != nullptr && size != nullptr && cam != nullptr && offset != nullptr &&
+                      scaling != nullptr && grid_weight_out != nullptr,
+                  "grid_weight_render: NULL argument");
+    for (int i = 0; i < 3; ++i)
+        SVOX2_REQUIRE(size[i] >= 2, "volume size[%d] = %d, must be >= 2", i, size[i]);
+    const int64_t cells = static_cast<int64_t>(size[0]) * size[1] * size[2];
+    SVOX2_REQUIRE(cells < INT32_MAX, "volume has too many cells");
+    SVOX2_REQUIRE(cam->width >= 0 && cam->height >= 0, "negative image size");
+    svox2_render_options_t ro{};
+    ro.step_size = step_size;
+    ro.stop_thresh = stop_thresh;
+    OptView o;
+    int st = make_opt(&ro, o);
+    if (st) return st;
+    const int64_t n_pix = static_cast<int64_t>(cam->width) * cam->height;
+    if (n_pix == 0) return ok();
+    GridView<float> g{};
+    for (int i = 0; i < 3; ++i) {
+        g.size[i] = size[i];
+        g.offset[i] = offset[i];
+        g.scaling[i] = scaling[i];
+    }
+    g.stride_x = size[1] * size[2];
+    g.stride_y = size[2];
+    const unsigned blocks = blocks_for(n_pix * kGroup, kBlock);
+    int* out = reinterpret_cast<int*>(grid_weight_out);
+    if (last_sample_opaque)
+        hipLaunchKernelGGL((k_grid_weight_render<true>), dim3(blocks), dim3(kBlock), 0,
+                           as_stream(stream), g, o, *cam, data, out);
+    else
+        hipLaunchKernelGGL((k_grid_weight_render<false>), dim3(blocks), dim3(kBlock), 0,
+                           as_stream(stream), g, o, *cam, data, out);
+    return check_launch("grid_weight_render");
 }
 
 extern "C" int svox2_volume_render_expected_term(const svox2_grid_t* grid, const float* origins,

@@ -12,6 +12,7 @@
 // point to [0, size-1], lower corner min(trunc, size-2), empty corners read 0.
 #include "common.hpp"
 
+#include <climits>
 #include <type_traits>
 
 namespace svox2amd {
@@ -161,6 +162,82 @@ __global__ __launch_bounds__(kBlock) void k_sample_density_backward(
         if (g.valid(lk[k])) atomic_add_f32(grad_d + static_cast<int64_t>(lk[k]) * gds, w[k]);
 }
 
+// Resampling (SparseGrid.resample, svox2.py:1223-1424).  The new grid's cell
+// (ix, iy, iz) sits at the old grid's coordinates (ax[ix], ay[iy], az[iz]):
+// the caller passes the three axis arrays, so the points are those of its own
+// linspace.  Grid coordinates map to themselves (sample(grid_coords=True)).
+struct NewGrid {
+    const float* ax;
+    const float* ay;
+    const float* az;
+    int n[3];
+};
+
+template <typename DT>
+__device__ __forceinline__ int new_point_cell(const GridView<DT>& g, const NewGrid& ng,
+                                              int64_t cell, float* p) {
+    const int iz = static_cast<int>(cell % ng.n[2]);
+    const int64_t ixy = cell / ng.n[2];
+    const int iy = static_cast<int>(ixy % ng.n[1]);
+    const int ix = static_cast<int>(ixy / ng.n[1]);
+    const float q[3] = {ng.ax[ix], ng.ay[iy], ng.az[iz]};
+    int l[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float x = fminf(fmaxf(q[a], 0.f), static_cast<float>(g.size[a]) - 1.f);
+        l[a] = min(static_cast<int>(x), g.size[a] - 2);
+        p[a] = x - static_cast<float>(l[a]);
+    }
+    return l[0] * g.stride_x + l[1] * g.stride_y + l[2];
+}
+
+// Pass 1: density of every new cell, one lane per cell, raster order.
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_resample_density(GridView<DT> g, NewGrid ng,
+                                                             int64_t n,
+                                                             float* __restrict__ out) {
+    const int64_t cell = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (cell >= n) return;
+    float p[3];
+    const int base = new_point_cell(g, ng, cell, p);
+    int lk[8];
+    corner_links(g, base, lk);
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        v[k] = g.valid(lk[k]) ? ld(g.density, static_cast<int64_t>(lk[k]) * g.dstride) : 0.f;
+    out[cell] = trilerp8(v, p);
+}
+
+// Pass 2: the rows of the kept cells.  One 32-lane group per new row: lane
+// c < C samples SH channel c at the cell's point, lane C copies the cell's
+// pass-1 density; the row is written in the output's storage type.
+template <typename DT, typename OT>
+__global__ __launch_bounds__(kBlock) void k_resample_rows(
+        GridView<DT> g, NewGrid ng, const int32_t* __restrict__ kept, int64_t n_rows,
+        const float* __restrict__ density_vol, OT* __restrict__ out_sh, int64_t sh_stride,
+        OT* __restrict__ out_density, int64_t density_stride) {
+    const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    const int C = g.sh_dim;
+    if (row >= n_rows || lane > C) return;
+    const int64_t cell = kept[row];
+    if (cell < 0 || cell >= static_cast<int64_t>(ng.n[0]) * ng.n[1] * ng.n[2]) return;
+    if (lane == C) {
+        st(out_density, row * density_stride, density_vol[cell]);
+        return;
+    }
+    float p[3];
+    const int base = new_point_cell(g, ng, cell, p);
+    int lk[8];
+    corner_links(g, base, lk);
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        v[k] = g.valid(lk[k]) ? ld(g.sh, static_cast<int64_t>(lk[k]) * g.sstride + lane) : 0.f;
+    st(out_sh, row * sh_stride + lane, trilerp8(v, p));
+}
+
 template <typename Fn>
 void with_view(const svox2_grid_t* grid, Fn&& fn) {
     if (grid->storage == SVOX2_STORAGE_BF16) fn(make_view<uint16_t>(*grid));
@@ -226,4 +303,71 @@ extern "C" int svox2_sample_grid_backward(const svox2_grid_t* grid, const float*
         }
     });
     return check_launch("sample_grid_backward");
+}
+
+static int check_new_grid(const float* axes, const int32_t reso[3], NewGrid& ng, int64_t& n) {
+    SVOX2_REQUIRE(axes != nullptr && reso != nullptr, "axes / reso are NULL");
+    n = 1;
+    for (int i = 0; i < 3; ++i) {
+        SVOX2_REQUIRE(reso[i] >= 1, "reso[%d] = %d", i, reso[i]);
+        ng.n[i] = reso[i];
+        n *= reso[i];
+    }
+    SVOX2_REQUIRE(n < INT32_MAX, "new grid has too many cells");
+    ng.ax = axes;
+    ng.ay = axes + reso[0];
+    ng.az = axes + reso[0] + reso[1];
+    return ok();
+}
+
+extern "C" int svox2_resample_density(const svox2_grid_t* grid, const float* axes,
+                                      const int32_t reso[3], float* density_out,
+                                      void* stream) {
+    int st = check_grid(grid, false);
+    if (st) return st;
+    NewGrid ng;
+    int64_t n;
+    if ((st = check_new_grid(axes, reso, ng, n))) return st;
+    SVOX2_REQUIRE(density_out != nullptr, "density_out is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    with_view(grid, [&](auto g) {
+        using DT = typename ViewDT<decltype(g)>::type;
+        k_resample_density<DT><<<blocks_for(n, kBlock), kBlock, 0, s>>>(g, ng, n, density_out);
+    });
+    return check_launch("resample_density");
+}
+
+extern "C" int svox2_resample_rows(const svox2_grid_t* grid, const float* axes,
+                                   const int32_t reso[3], const int32_t* kept_cells,
+                                   int64_t n_rows, const float* density_volume, void* sh_out,
+                                   int64_t sh_stride, void* density_out, int64_t density_stride,
+                                   int32_t out_storage, void* stream) {
+    int st = check_grid(grid, true);
+    if (st) return st;
+    NewGrid ng;
+    int64_t n;
+    if ((st = check_new_grid(axes, reso, ng, n))) return st;
+    SVOX2_REQUIRE(n_rows >= 0 && n_rows <= n, "row count %lld outside [0, %lld]",
+                  (long long)n_rows, (long long)n);
+    if (n_rows == 0) return ok();
+    SVOX2_REQUIRE(kept_cells != nullptr && density_volume != nullptr && sh_out != nullptr &&
+                      density_out != nullptr,
+                  "resample_rows: NULL argument");
+    SVOX2_REQUIRE(sh_stride >= grid->sh_data_dim && density_stride >= 1, "bad output strides");
+    SVOX2_REQUIRE(out_storage == SVOX2_STORAGE_F32 || out_storage == SVOX2_STORAGE_BF16,
+                  "unknown storage kind %d", out_storage);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned blocks = blocks_for(n_rows * kGroup, kBlock);
+    with_view(grid, [&](auto g) {
+        using DT = typename ViewDT<decltype(g)>::type;
+        if (out_storage == SVOX2_STORAGE_BF16)
+            k_resample_rows<DT, uint16_t><<<blocks, kBlock, 0, s>>>(
+                g, ng, kept_cells, n_rows, density_volume, static_cast<uint16_t*>(sh_out),
+                sh_stride, static_cast<uint16_t*>(density_out), density_stride);
+        else
+            k_resample_rows<DT, float><<<blocks, kBlock, 0, s>>>(
+                g, ng, kept_cells, n_rows, density_volume, static_cast<float*>(sh_out),
+                sh_stride, static_cast<float*>(density_out), density_stride);
+    });
+    return check_launch("resample_rows");
 }
