@@ -10,9 +10,10 @@ launch / average launch time from HIP events on the launch stream, against the
 8 TB/s HBM peak), side measurements (BF16 storage, 5,000-ray batches, the
 million-ray autograd step, the serving loop, the 256^3 training step with
 the TV, RMSprop, accel_dist_prop and sample kernels against their own
-algorithmic bytes, an 800x800 forward image) and a CPU baseline: the scalar
-C port of the reference's fused forward+backward (oracle/march_ref.c) timed on
-one host core on a bounded sample of the same batch.
+algorithmic bytes, an 800x800 forward image) and CPU baselines on the host's
+cores: the C port of the reference's fused forward+backward
+(oracle/march_ref.c) on the same 512^3 grid, and the restated reference CPU
+path (_volume_render_gradcheck_lerp + autograd) on the 64^3 dense workload.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -470,14 +471,44 @@ def serve_bench(wl: Workload, qps_list=(200.0, 600.0, 1000.0), n: int = 600):
     return out
 
 
-def cpu_baseline(wl: Workload, budget_s: float = 12.0):
-    """The scalar C port of the reference's fused forward + MSE backward
-    (oracle/march_ref.c: trace_ray_cuvol and trace_ray_cuvol_backward, one
-    thread, every contributing sample scattered into its 8 corner rows) on
-    this host, over whole 65536-ray batches of the same grid until about
-    `budget_s` seconds of CPU time are spent."""
+def host_threads() -> int:
+    """Threads the host leg may use: OMP_NUM_THREADS when set (the GPU box
+    sets it to its CPU share), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl: Workload, budget_s: float = 12.0, runs: int = 5):
+    """CPU baselines on this host's cores, median of `runs` timed runs after
+    one warm-up:
+
+    - the headline figure: the C port of the reference's fused forward +
+      MSE backward (oracle/march_ref.c ref_fused, the CUDA kernel's algorithm
+      ray by ray, rays split over OpenMP threads, gradient updates atomic) on
+      65536-ray batches of the benchmark's 512^3 grid;
+    - `reference_path`: the restatement of the reference's own CPU path,
+      SparseGrid._volume_render_gradcheck_lerp plus autograd for the MSE
+      gradient (svox2.py:658-884), float32, torch on all threads, at full size
+      on the 64^3 dense grid with 4096 rays.  At 512^3 that path is out of
+      reach on a CPU: every march step gathers rows with dense indexing whose
+      backward zero-fills a rows x 27 gradient table per corner (8 x 460 MB
+      per step at 4.26M rows, some 3000 steps per batch), and its autograd
+      state for a 65536-ray batch would be about 200 GB."""
     import oracle.oracle as O
     O.build(force=False)
+    threads = host_threads()
     grid = wl.grid(torch.float32)
     N = np.array(grid.links.shape, np.float64)
     hg = O.Grid(grid.links.cpu().numpy(), grid.density_data.data.cpu().double().numpy().reshape(-1),
@@ -486,20 +517,49 @@ def cpu_baseline(wl: Workload, budget_s: float = 12.0):
     opt = O.Opts(sigma_thresh=1e-8, stop_thresh=1e-7)
     march = O.march()
     gen = torch.Generator().manual_seed(123)
-    rays, spent, n_batches = 0, 0.0, 0
-    while spent < budget_s and n_batches < 64:
+    rates, spent = [], 0.0
+    for i in range(runs + 1):
         o, d = wl.cams.random_batch(BATCH, gen)
         gt = torch.rand(BATCH, 3, generator=gen)
         t = time.perf_counter()
-        march.fused(hg, o.numpy(), d.numpy(), opt, gt.numpy())
-        spent += time.perf_counter() - t
-        rays += BATCH
-        n_batches += 1
-        log(f"CPU baseline: batch {n_batches}, {rays} rays in {spent:.2f}s")
-    return dict(value=rays / spent, unit="rays/s", cores=1, kind="port",
-                sample=f"{n_batches} random 65536-ray batches on the same 512^3 grid "
-                       f"({rays} rays, {spent:.1f}s): scalar C port of the reference fused "
-                       f"forward+backward (one thread, float32)")
+        march.fused(hg, o.numpy(), d.numpy(), opt, gt.numpy(), threads=threads)
+        dt = time.perf_counter() - t
+        if i > 0:
+            rates.append(BATCH / dt)
+            spent += dt
+        log(f"CPU baseline (C port, {threads} threads): run {i} {BATCH / dt:.4g} rays/s")
+        if i >= runs and spent > budget_s:
+            break
+    del hg, grid
+    # the reference's own CPU path at full size on the 64^3 dense workload
+    from svox2_amd import scenes
+    torch.set_num_threads(threads)
+    sc, o, d, gt = scenes.workload_dense64(seed=0)
+    links = sc.links.long()
+    ref_rates = []
+    for i in range(runs + 1):
+        dens = sc.density.float().clone().requires_grad_(True)
+        sh = sc.sh.float().clone().requires_grad_(True)
+        t = time.perf_counter()
+        rgb = O.volume_render_gradcheck_lerp(links, dens, sh, 9, 1.0, 0.0, o, d,
+                                             dtype=torch.float32)
+        ((rgb - gt.float()) ** 2).mean().backward()
+        dt = time.perf_counter() - t
+        if i > 0:
+            ref_rates.append(o.shape[0] / dt)
+        log(f"CPU baseline (reference path, 64^3): run {i} {o.shape[0] / dt:.4g} rays/s")
+    model = cpu_model()
+    return dict(
+        value=float(np.median(rates)), unit="rays/s", cores=threads, kind="port",
+        cpu=model,
+        sample=f"median of {len(rates)} runs (after 1 warm-up) of random 65536-ray batches "
+               f"on the same 512^3 grid: C port of the reference's fused forward+backward, "
+               f"float32, {threads} OpenMP threads",
+        reference_path=dict(
+            value=float(np.median(ref_rates)), unit="rays/s", cores=threads, kind="port",
+            sample=f"median of {len(ref_rates)} runs (after 1 warm-up): restated "
+                   f"_volume_render_gradcheck_lerp + autograd MSE gradient, float32, "
+                   f"torch with {threads} threads, 64^3 dense grid, 4096 rays (full size)"))
 
 
 def pmc_run(reps: int = 3):

@@ -30,7 +30,8 @@
  *                trace_ray_cuvol_backward :227-383), scattering every
  *                contributing sample into the 8 corner rows as the reference
  *                does (render_util.cuh:93-153): the scalar CPU port timed as
- *                the CPU baseline.
+ *                the CPU baseline (rays split over n_threads OpenMP threads, the
+ *                gradient updates atomic; one thread is the sequential order).
  */
 #include <math.h>
 #include <stdint.h>
@@ -464,11 +465,13 @@ int ref_grid_weight_render(const float* data, const int32_t* size, const float* 
 int ref_fused(const ref_grid_t* g, const float* origins, const float* dirs, const float* clip,
               int64_t n, const ref_opt_t* o, const float* rgb_gt, float beta_loss,
               float sparsity_loss, float* rgb_out, float* grad_density, float* grad_sh,
-              uint8_t* mask) {
+              uint8_t* mask, int32_t n_threads) {
     const float log_stop = log_stop_threshold(o->stop_thresh);
     const int B = g->basis_dim, C = 3 * B;
     const float norm = 2.f / (float)(3 * n);
     const float beta = beta_loss / (float)n;
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads) if (n_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
         const float* wo = origins + 3 * i;
         const float* wd = dirs + 3 * i;
@@ -533,11 +536,17 @@ int ref_fused(const ref_grid_t* g, const float* origins, const float* dirs, cons
                     if (!valid_link(g, lk[k])) continue;
                     const float wk = ((k & 2) ? p[1] : ay) * ((k & 1) ? p[2] : az) *
                                      ((k & 4) ? p[0] : ax);
-                    if (grad_density) grad_density[lk[k]] += wk * gs;
+                    if (grad_density) {
+#pragma omp atomic
+                        grad_density[lk[k]] += wk * gs;
+                    }
                     if (mask) mask[lk[k]] = 1;
                     if (grad_sh)
-                        for (int ch = 0; ch < C; ++ch)
-                            grad_sh[(int64_t)lk[k] * C + ch] += wk * basis[ch % B] * gc[ch / B];
+                        for (int ch = 0; ch < C; ++ch) {
+                            const float v = wk * basis[ch % B] * gc[ch / B];
+#pragma omp atomic
+                            grad_sh[(int64_t)lk[k] * C + ch] += v;
+                        }
                 }
                 if (log_t < log_stop) break;
             }

@@ -61,7 +61,7 @@ def build(force: bool = True) -> str:
     src = os.path.join(HERE, "march_ref.c")
     tmp = LIB_PATH + ".tmp"
     cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-ffp-contract=off",
-           "-fno-fast-math", "-o", tmp, src, "-lm"]
+           "-fno-fast-math", "-fopenmp", "-o", tmp, src, "-lm"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
@@ -138,7 +138,7 @@ class MarchRef:
                                         ctypes.c_float, ctypes.c_int32, ctypes.c_int32, vp, vp]
         lib.ref_camera_rays.restype = ctypes.c_int
         lib.ref_fused.argtypes = [G, vp, vp, vp, i64, O, vp, ctypes.c_float, ctypes.c_float,
-                                  vp, vp, vp, vp]
+                                  vp, vp, vp, vp, ctypes.c_int32]
         lib.ref_fused.restype = ctypes.c_int
         f32 = ctypes.c_float
         lib.ref_grid_weight_render.argtypes = [vp, vp, vp, f32, f32, f32, f32, ctypes.c_int32,
@@ -253,9 +253,10 @@ class MarchRef:
         return out
 
     def fused(self, g: Grid, origins, dirs, opt: Opts, rgb_gt, clip=None, beta_loss=0.0,
-              sparsity_loss=0.0):
-        """FP32 scalar port of the fused forward + MSE backward: rgb (n,3),
-        grad_density (R,), grad_sh (R,C), touched mask (R,)."""
+              sparsity_loss=0.0, threads: int = 1):
+        """FP32 port of the fused forward + MSE backward: rgb (n,3),
+        grad_density (R,), grad_sh (R,C), touched mask (R,); rays split over
+        `threads` OpenMP threads (1: sequential, deterministic)."""
         keep: list = []
         gs = self._grid(g, keep)
         o, d, c = self._rays(origins, dirs, clip, keep)
@@ -269,7 +270,7 @@ class MarchRef:
         m = np.zeros(R, np.uint8)
         self.lib.ref_fused(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), _c_ptr(c), n,
                            ctypes.byref(self._opt(opt)), _c_ptr(gt), beta_loss, sparsity_loss,
-                           _c_ptr(rgb), _c_ptr(gd), _c_ptr(gsh), _c_ptr(m))
+                           _c_ptr(rgb), _c_ptr(gd), _c_ptr(gsh), _c_ptr(m), int(threads))
         return rgb, gd, gsh, m.astype(bool)
 
     def camera_rays(self, c2w, fx, fy, cx, cy, width, height):
