@@ -745,40 +745,54 @@ def tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray, 
                    end_dim: int, scale: float, ignore_edge: bool, ignore_last_z: bool):
     """Gradient and touched-row mask of the sparse TV term
     (loss_kernel.cu:178-246 with the launcher's scale / n_cells, :597),
-    including its edge rules.  Plain float64 loops over cells."""
+    including its edge rules, in float64 over all listed cells at once: a
+    neighbour past the grid's end is link 0; a missing neighbour value is 0,
+    or v000 with ignore_edge, which also skips cells whose link is 0;
+    ignore_last_z skips cells at z = size2 - 2; a term is added (and its row
+    marked) where its link is >= 0 and its scaled difference is nonzero."""
     data = np.asarray(data, np.float64)
     R, D = data.shape
     grad = np.zeros_like(data)
     mask = np.zeros(R, bool)
     s0, s1, s2 = links.shape
-    sc = np.array([s0, s1, s2], np.float64) / 256.0
-    n = len(rand_cells)
+    cells = np.asarray(rand_cells, np.int64)
+    n = cells.size
+    if n == 0:
+        return grad, mask
     scale_c = scale / n
-    for xyz in np.asarray(rand_cells, np.int64):
-        z = xyz % s2
-        y = (xyz // s2) % s1
-        x = xyz // (s1 * s2)
-        l000 = int(links[x, y, z])
-        if ignore_edge and l000 == 0:
-            continue
-        l001 = int(links[x, y, z + 1]) if (z + 1 < s2 and (not ignore_last_z or z != s2 - 2)) else 0
-        l010 = int(links[x, y + 1, z]) if y + 1 < s1 else 0
-        l100 = int(links[x + 1, y, z]) if x + 1 < s0 else 0
-        if ignore_last_z and z == s2 - 2:
-            continue
-        for idx in range(start_dim, end_dim):
-            v000 = data[l000, idx] if l000 >= 0 else 0.0
-            nul = v000 if ignore_edge else 0.0
-            v001 = data[l001, idx] if l001 >= 0 else nul
-            v010 = data[l010, idx] if l010 >= 0 else nul
-            v100 = data[l100, idx] if l100 >= 0 else nul
-            dx, dy, dz = v100 - v000, v010 - v000, v001 - v000
-            idelta = scale_c / math.sqrt(1e-9 + dx * dx + dy * dy + dz * dz)
-            dx, dy, dz = dx * sc[0], dy * sc[1], dz * sc[2]
-            for lk, val in ((l000, -(dx + dy + dz)), (l001, dz), (l010, dy), (l100, dx)):
-                if lk >= 0 and val != 0.0:
-                    grad[lk, idx] += val * idelta
-                    mask[lk] = True
+    z = cells % s2
+    y = (cells // s2) % s1
+    x = cells // (s1 * s2)
+    flat = links.reshape(-1).astype(np.int64)
+    l000 = flat[cells]
+    keep = np.ones(n, bool)
+    if ignore_edge:
+        keep &= l000 != 0
+    if ignore_last_z:
+        keep &= z != s2 - 2
+    x, y, z, l000 = x[keep], y[keep], z[keep], l000[keep]
+    base = (x * s1 + y) * s2 + z
+    l001 = np.where(z + 1 < s2, flat[np.minimum(base + 1, flat.size - 1)], 0)
+    l010 = np.where(y + 1 < s1, flat[np.minimum(base + s2, flat.size - 1)], 0)
+    l100 = np.where(x + 1 < s0, flat[np.minimum(base + s1 * s2, flat.size - 1)], 0)
+    ch = np.arange(start_dim, end_dim)
+
+    def vals(lk, null):
+        v = data[np.maximum(lk, 0)][:, ch]
+        return np.where((lk >= 0)[:, None], v, null)
+
+    v000 = vals(l000, 0.0)
+    nul = v000 if ignore_edge else 0.0
+    v001, v010, v100 = vals(l001, nul), vals(l010, nul), vals(l100, nul)
+    dx, dy, dz = v100 - v000, v010 - v000, v001 - v000
+    idelta = scale_c / np.sqrt(1e-9 + dx * dx + dy * dy + dz * dz)
+    dx, dy, dz = dx * (s0 / 256.0), dy * (s1 / 256.0), dz * (s2 / 256.0)
+    for lk, val in ((l000, -(dx + dy + dz)), (l001, dz), (l010, dy), (l100, dx)):
+        act = (lk >= 0)[:, None] & (val != 0.0)
+        rows = np.broadcast_to(lk[:, None], act.shape)[act]
+        cols = np.broadcast_to(ch[None, :], act.shape)[act]
+        np.add.at(grad, (rows, cols), (val * idelta)[act])
+        mask[rows] = True
     return grad, mask
 
 

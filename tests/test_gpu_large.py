@@ -294,3 +294,90 @@ def test_262144_fused_matches_autograd(gpu, big):
     assert rel(gd_f, grid.density_data.grad) < 1e-5 and rel(gs_f, grid.sh_data.grad) < 1e-5
     grid.density_data.grad = None
     grid.sh_data.grad = None
+
+
+def test_512_image_800(gpu, big, march):
+    """The 800x800 image of the benchmark's 512^3 grid (float32 and bf16
+    storage): bit-exact against the ray kernel on the oracle's float32 camera
+    rays of the same camera, and the colours of 4096 random pixels within
+    2e-5 of the FP64 oracle on the stored values."""
+    sc, grid32, links, o, d, gt = big
+    cams = scenes.CameraSet(scenes.hemisphere_cameras(3, torch.Generator().manual_seed(9)))
+    cam = cams.camera(1)
+    ro, rd = march.camera_rays(cam.c2w.numpy(), cam.fx_val, cam.fy_val, cam.cx_val, cam.cy_val,
+                               800, 800)
+    sel = np.random.default_rng(0).choice(800 * 800, 4096, replace=False)
+    for dtype in (torch.float32, torch.bfloat16):
+        grid = grid32 if dtype == torch.float32 else scenes.to_sparse_grid(sc, device=gpu,
+                                                                           dtype=dtype)
+        grid.opt.sigma_thresh, grid.opt.stop_thresh = OPT.sigma_thresh, OPT.stop_thresh
+        with torch.no_grad():
+            img = grid.volume_render_image(cam).reshape(-1, 3)
+            rgb = grid.volume_render(svox2_amd.Rays(torch.from_numpy(ro).to(gpu),
+                                                    torch.from_numpy(rd).to(gpu)))
+        assert torch.equal(img, rgb), dtype
+        g = host_grid(sc, dtype, links=links)
+        rgb64 = O.render_fp64(g, ro[sel], rd[sel], OPT, torch.from_numpy(g.density),
+                              torch.from_numpy(g.sh)).numpy()
+        err = np.abs(img.cpu().numpy()[sel] - rgb64).max()
+        print(f"800x800 {dtype}: colour error on 4096 pixels {err:.3e}")
+        assert err < 2e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_512_fused_5000(gpu, big, dtype):
+    """The reference's training batch size on the 512^3 grid: 5000 rays
+    (below the ordering threshold, so the forward records every contributing
+    sample and the backward replays the records) against the FP64 oracle."""
+    sc, grid32, links, o, d, gt = big
+    grid = grid32 if dtype == torch.float32 else scenes.to_sparse_grid(sc, device=gpu,
+                                                                       dtype=dtype)
+    grid.opt.sigma_thresh, grid.opt.stop_thresh = OPT.sigma_thresh, OPT.stop_thresh
+    n = 5000
+    grid.density_data.grad = None
+    grid.sh_data.grad = None
+    rgb = grid.volume_render_fused(svox2_amd.Rays(o[:n].to(gpu), d[:n].to(gpu)),
+                                   gt[:n].to(gpu)).cpu().numpy()
+    g = host_grid(sc, dtype, links=links)
+    rgb64, gd64, gs64 = O.mse_grads_fp64(g, o[:n].numpy(), d[:n].numpy(), OPT, gt[:n].numpy())
+    assert np.abs(rgb - rgb64).max() < 2e-5
+    assert_grad_close(grid.density_data.grad.cpu().numpy().reshape(-1), gd64,
+                      f"5000 rays {dtype} grad_density")
+    assert_grad_close(grid.sh_data.grad.cpu().numpy(), gs64, f"5000 rays {dtype} grad_sh")
+    grid.density_data.grad = None
+    grid.sh_data.grad = None
+
+
+def test_serving_mix_at_size(gpu, big):
+    """The serving configuration at its real size: four resident grids (two
+    256^3 about 8% occupied and two 512^3 about 3%, one of each in bf16),
+    image sides 128, 256, 512, 800 and 1024, the benchmark's server settings
+    (4096-pixel tiles, 128 tiles per launch, graph replay): every request's
+    image equals its lone volume_render_image bit for bit."""
+    from svox2_amd.serve import RenderRequest, RenderServer
+    sc512, grid512, _, _, _, _ = big
+    sc256, cams256, _ = scenes.workload_blobs(256, scenes.SPARSE256, seed=1, device=gpu)
+    grids = []
+    for sc, dt in ((sc256, torch.float32), (sc256, torch.bfloat16), (sc512, torch.float32),
+                   (sc512, torch.bfloat16)):
+        g = grid512 if (sc is sc512 and dt == torch.float32) else \
+            scenes.to_sparse_grid(sc, device=gpu, dtype=dt)
+        g.opt.sigma_thresh, g.opt.stop_thresh = OPT.sigma_thresh, OPT.stop_thresh
+        grids.append(g)
+    cam_sets = [cams256, cams256] + [scenes.CameraSet(
+        scenes.hemisphere_cameras(8, torch.Generator().manual_seed(13)))] * 2
+    rng = np.random.default_rng(2)
+    reqs = []
+    for side in (128, 256, 512, 800, 1024, 128, 512, 1024, 256, 800):
+        gi = int(rng.integers(4))
+        cams = cam_sets[gi]
+        cam = cams.camera(int(rng.integers(cams.c2w.shape[0])), width=side, height=side)
+        reqs.append(RenderRequest(grid=gi, camera=cam, arrival=0.0))
+    server = RenderServer(grids, hbm_budget=1 << 30, tile_pixels=4096, max_tiles=128,
+                          max_active=64, policy="fifo", depth=3)
+    stats = server.run(reqs, realtime=False)
+    assert stats["requests"] == len(reqs)
+    for r in reqs:
+        with torch.no_grad():
+            alone = grids[r.grid].volume_render_image(r.camera)
+        assert torch.equal(r.image, alone), (r.grid, r.camera.width)
