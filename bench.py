@@ -16,6 +16,7 @@ cores: the C port of the reference's fused forward+backward
 path (_volume_render_gradcheck_lerp + autograd) on the 64^3 dense workload.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+  python bench.py --only {small,side,image,serve,stress}   (one side measurement)
 """
 from __future__ import annotations
 
@@ -579,6 +580,32 @@ def pmc_run(reps: int = 3):
                                   "copy_bytes": 2 ** 30}}), flush=True)
 
 
+ONLY_PARTS = ("small", "side", "image", "serve", "stress")
+
+
+def run_only(part: str, dev, steps: int, warmup: int) -> dict:
+    """One side measurement of the full run, alone: `small` the 5000-ray
+    fused steps (float32 and bfloat16), `side` the 256^3 training step and
+    the TV / RMSprop / accel_dist_prop / sample kernels, `image` the 800x800
+    forward images, `serve` the serving loop at 200 requests/s, `stress` the
+    million-ray autograd step."""
+    wl = Workload(dev)
+    if part == "small":
+        small = wl.batches(5000, POOL)
+        return {"fp32_5000": measure(wl.grid(torch.float32), small, steps, warmup, 4, 4),
+                "bf16_5000": measure(wl.grid(torch.bfloat16), small, steps, warmup, 2, 2)}
+    if part == "side":
+        return side_kernels(wl, steps, warmup)
+    if part == "image":
+        return {"image_800_fp32": image_forward(wl.grid(torch.float32), wl.cams),
+                "image_800_bf16": image_forward(wl.grid(torch.bfloat16), wl.cams)}
+    if part == "serve":
+        return serve_bench(wl, qps_list=(200.0,), n=200)
+    if part == "stress":
+        return stress_1m(wl, wl.grid(torch.float32))
+    raise ValueError(part)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -588,6 +615,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline measurement only")
     ap.add_argument("--pmc", action="store_true",
                     help="short run for rocprofv3 counter passes (3 fused calls + 1 GiB copy)")
+    ap.add_argument("--only", choices=ONLY_PARTS,
+                    help="run one side measurement alone (for per-workload rocprofv3 "
+                         "summaries) and print its result as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the renderer runs on a single device "
@@ -603,6 +633,10 @@ def main():
         pmc_run()
         return
     dev = torch.device("cuda:0")
+    if args.only:
+        print(json.dumps({"only": args.only, "result": run_only(args.only, dev, args.steps,
+                                                                 args.warmup)}), flush=True)
+        return
 
     wl = Workload(dev)
     batches = wl.batches(BATCH, POOL)

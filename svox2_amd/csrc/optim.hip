@@ -69,24 +69,30 @@ __device__ __forceinline__ void update_one(const RowJob<DT>& J, int row, int col
 }
 
 // Candidate k of the job: its row (or -1) and whether groups A / B select it.
+// Written with one exit: every kind assigns the row on its own path.
 template <typename DT>
 __device__ __forceinline__ int candidate(const RowJob<DT>& J, int64_t k, int64_t ncand,
                                          bool& a, bool& b) {
-    a = b = false;
-    if (k >= ncand) return -1;
-    if (J.kind == SVOX2_INDEXER_INDEX) {
-        const int64_t r = J.index[k];
-        if (r < 0 || r >= J.rows) return -1;  // out of range: skipped
-        a = b = true;
-        return static_cast<int>(r);
+    int row = -1;
+    bool sa = false, sb = false;
+    if (k < ncand) {
+        if (J.kind == SVOX2_INDEXER_INDEX) {
+            const int64_t r = J.index[k];
+            const bool in = r >= 0 && r < J.rows;  // out of range: skipped
+            row = in ? static_cast<int>(r) : -1;
+            sa = sb = in;
+        } else if (J.kind == SVOX2_INDEXER_MASK) {
+            row = static_cast<int>(k);
+            sa = J.mask_a[k] != 0;
+            sb = J.mask_b[k] != 0;
+        } else {
+            row = static_cast<int>(k);
+            sa = sb = true;
+        }
     }
-    if (J.kind == SVOX2_INDEXER_ALL) {
-        a = b = true;
-    } else {
-        a = J.mask_a[k] != 0;
-        b = J.mask_b[k] != 0;
-    }
-    return static_cast<int>(k);
+    a = sa;
+    b = sb;
+    return row;
 }
 
 template <typename DT, bool SGD>

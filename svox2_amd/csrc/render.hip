@@ -957,19 +957,23 @@ struct GradOut {
 // Per-sample record a backward chunk stages in LDS for its scatter loop: every
 // lane of the group then reads the current sample's record at one address (a
 // broadcast read) instead of gathering each field from the sample's lane.
-// `move` says how the sample's cell relates to the previous live sample's:
-// 0 the same cell, kMoveJump a cell that shares no corner with it (or the
-// ray's first cell), otherwise kMoveStep | (d0 + 1) << 1 | (d1 + 1) << 3 |
-// (d2 + 1) << 5 for a step of d in {-1, 0, 1}^3.
+//
+// Corner quantities are kept in parity slots: grid point (X, Y, Z) lives in
+// slot (X & 1) << 2 | (Y & 1) << 1 | (Z & 1).  The 8 corners of a cell take
+// the 8 distinct slots, so corner k (bits dx dy dz) of the cell with lower
+// corner c sits in slot k ^ parity(c).  A step to an adjacent cell keeps the
+// shared grid points in their slots, and the grid points it enters take the
+// slots of those it leaves: no accumulator moves, only the departing slots
+// are flushed.  `w` and `lk` are stored in slot order; `depart` has bit s set
+// when the grid point in slot s before this sample is not a corner of its
+// cell (all 8 after a jump or for the ray's first live sample).
 struct __align__(16) SampleRec {
-    float w[8];    // trilinear weights (wy * wz) * wx of corners k = dx dy dz
+    float w[8];    // trilinear weight of the corner in slot s
     float val[4];  // colour factors R, G, B and the density gradient
-    int lk[8];
-    int move;
+    int lk[8];     // link of the corner in slot s
+    int depart;
     int pad[3];
 };
-constexpr int kMoveStep = 1;
-constexpr int kMoveJump = 0x80;
 
 // LDS of one backward workgroup: the sample records of its rays.
 struct BwdShared {
@@ -978,16 +982,16 @@ struct BwdShared {
 
 // Gradient scatter state of one ray.  Lane L of the 32-lane group owns one
 // gradient column of a row: SH channel L (c = L / B, j = L % B) for L < C,
-// the density for L == C; lanes above are idle.  For each corner k (bits dx
-// dy dz) of the cell the ray is in, the lane keeps in registers the
-// accumulated factor of its quantity - the colour factor of channel c (the SH
-// gradient is that times basis_j) or the density gradient - and the corner's
-// link.  Consecutive samples in one cell add into the same accumulators;
-// moving to an adjacent cell keeps the corners the two cells share (a
-// register permutation) and flushes the face left behind, so a grid point is
-// flushed once per visit.  A flush of corner k is one no-return atomic
-// instruction over lanes 0..C: the row's SH channels and its density, the
-// same 128-byte row in the interleaved layout.
+// the density for L == C; lanes above are idle.  For each parity slot s the
+// lane keeps in registers the accumulated factor of its quantity at the
+// slot's grid point - the colour factor of channel c (the SH gradient is that
+// times basis_j) or the density gradient - and the grid point's link.
+// Consecutive samples add into the same accumulators while their cells share
+// the grid point, so a grid point is flushed once per visit.  A flush of slot
+// s is one no-return atomic instruction over lanes 0..C: the row's SH
+// channels and its density, the same 128-byte row in the interleaved layout.
+// (The touched-row mask is marked when a sample enters a grid point, while
+// the records are built.)
 template <int B>
 struct Scatter {
     static constexpr int C = 3 * B;
@@ -1016,7 +1020,7 @@ struct Scatter {
         }
     }
 
-    // Adds the accumulated factor a of the corner with link lk to its row.
+    // Adds the accumulated factor a of the grid point with link lk to its row.
     template <typename DT>
     __device__ __forceinline__ void emit(const GridView<DT>& g, int lk, float a) const {
         if (col != nullptr && g.valid(lk))
@@ -1025,57 +1029,23 @@ struct Scatter {
                            mult * a);
     }
 
-    // Marks the rows of up to 4 flushed corners: lane C + 1 + i stores row i's.
+    // Flushes the slots of `out` (bit s: slot s) and clears them.
     template <typename DT>
-    __device__ __forceinline__ void mark(const GridView<DT>& g, const GradOut& go, int lane,
-                                         const int* lk4) const {
-        int mk = -1;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) mk = (lane == C + 1 + i) ? lk4[i] : mk;
-        if (go.mask != nullptr && g.valid(mk)) go.mask[mk] = 1;
-    }
-
-    // Flushes all 8 corners and clears them.
-    template <typename DT>
-    __device__ __forceinline__ void flush_all(const GridView<DT>& g, const GradOut& go,
-                                              int lane) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) emit(g, link[k], acc[k]);
-        mark(g, go, lane, link);
-        mark(g, go, lane, link + 4);
+    __device__ __forceinline__ void flush(const GridView<DT>& g, int out) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            acc[k] = 0.f;
-            link[k] = -1;
+            const bool go = (out >> k) & 1;
+            emit(g, go ? link[k] : -1, acc[k]);
+            acc[k] = go ? 0.f : acc[k];
         }
     }
 
-    // Step by d in {-1, 0, 1} along the axis of corner bit BIT: flush the face
-    // left behind, move the shared face's accumulators to their new corner
-    // index and clear the entering face.
-    template <int BIT, typename DT>
-    __device__ __forceinline__ void shift(int d, const GridView<DT>& g, const GradOut& go,
-                                          int lane) {
-        if (d == 0) return;
-        const bool up = d > 0;
-        int dep[4];
+    // Flushes all 8 slots and clears them.
+    template <typename DT>
+    __device__ __forceinline__ void flush_all(const GridView<DT>& g, const GradOut&, int) {
+        flush(g, 0xFF);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            // i-th corner without bit BIT, and its partner with it.
-            constexpr int kLow[3][4] = {{0, 2, 4, 6}, {0, 1, 4, 5}, {0, 1, 2, 3}};
-            const int lo = kLow[BIT == 1 ? 0 : (BIT == 2 ? 1 : 2)][i];
-            const int hi = lo | BIT;
-            const float a_out = up ? acc[lo] : acc[hi];
-            dep[i] = up ? link[lo] : link[hi];
-            emit(g, dep[i], a_out);
-            const float a_keep = up ? acc[hi] : acc[lo];
-            const int l_keep = up ? link[hi] : link[lo];
-            acc[lo] = up ? a_keep : 0.f;
-            acc[hi] = up ? 0.f : a_keep;
-            link[lo] = up ? l_keep : -1;
-            link[hi] = up ? -1 : l_keep;
-        }
-        mark(g, go, lane, dep);
+        for (int k = 0; k < 8; ++k) link[k] = -1;
     }
 };
 
@@ -1115,8 +1085,9 @@ struct BwdRay {
 // formed sample-parallel - the colour factors weight * [colour >= 0] * g and
 // the density gradient ws * (total * T_after - accum), with the running
 // `accum` as a prefix sum over the chunk - and staged as LDS records with each
-// sample's 8 trilinear corner weights and cell move; then the samples are
-// added in order into the corner accumulators, one fma per corner each.
+// sample's 8 trilinear weights and links in parity-slot order and the slots
+// its step leaves; then the samples are added in order into the slot
+// accumulators, one fma per corner each.
 template <typename DT, int B>
 __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradOut& go,
                                               BwdRay<B>& st, int lane, bool live,
@@ -1148,26 +1119,67 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
     const bool had = before != 0u || sc.have;
     if (live) {
         const int d0 = l[0] - pl[0], d1 = l[1] - pl[1], d2 = l[2] - pl[2];
-        int move;
-        if (!had || d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
-            move = kMoveJump;
-        else if ((d0 | d1 | d2) == 0)
-            move = 0;
-        else
-            move = kMoveStep | ((d0 + 1) << 1) | ((d1 + 1) << 3) | ((d2 + 1) << 5);
+        // Slots the step from the previous live sample's cell leaves: along an
+        // axis stepped by +1 the grid points at the old lower coordinate, by
+        // -1 those at the old upper one; their slots have that coordinate's
+        // parity in the axis' bit.
+        int depart;
+        if (!had || d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1) {
+            depart = 0xFF;
+        } else {
+            const int bx = (pl[0] & 1) ^ (d0 < 0 ? 1 : 0);
+            const int by = (pl[1] & 1) ^ (d1 < 0 ? 1 : 0);
+            const int bz = (pl[2] & 1) ^ (d2 < 0 ? 1 : 0);
+            depart = (d0 != 0 ? (bx ? 0xF0 : 0x0F) : 0) | (d1 != 0 ? (by ? 0xCC : 0x33) : 0) |
+                     (d2 != 0 ? (bz ? 0xAA : 0x55) : 0);
+        }
         SampleRec& R = st.rec[lane];
-        const float wx[2] = {1.f - p[0], p[0]};
-        const float wy[2] = {1.f - p[1], p[1]};
-        const float wz[2] = {1.f - p[2], p[2]};
+        // Slot s holds corner s ^ parity(l): swapping each axis' pair of
+        // weights (and links) by the cell's parity puts them in slot order.
+        const bool px = l[0] & 1, py = l[1] & 1, pz = l[2] & 1;
+        const float wx[2] = {px ? p[0] : 1.f - p[0], px ? 1.f - p[0] : p[0]};
+        const float wy[2] = {py ? p[1] : 1.f - p[1], py ? 1.f - p[1] : p[1]};
+        const float wz[2] = {pz ? p[2] : 1.f - p[2], pz ? 1.f - p[2] : p[2]};
 #pragma unroll
         for (int k = 0; k < 8; ++k) R.w[k] = wy[(k >> 1) & 1] * wz[k & 1] * wx[k >> 2];
+        int q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = lk[k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k & 4) continue;
+            const int a0 = q[k], a1 = q[k | 4];
+            q[k] = px ? a1 : a0;
+            q[k | 4] = px ? a0 : a1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k & 2) continue;
+            const int a0 = q[k], a1 = q[k | 2];
+            q[k] = py ? a1 : a0;
+            q[k | 2] = py ? a0 : a1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k & 1) continue;
+            const int a0 = q[k], a1 = q[k | 1];
+            q[k] = pz ? a1 : a0;
+            q[k | 1] = pz ? a0 : a1;
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             R.val[a] = weight * ((fmaxf(cs[a], 0.f) == cs[a]) ? 1.f : 0.f) * st.gq[a];
-        R.move = move;
+        R.depart = depart;
         R.val[3] = gsig;
 #pragma unroll
-        for (int kk = 0; kk < 8; ++kk) R.lk[kk] = lk[kk];
+        for (int kk = 0; kk < 8; ++kk) R.lk[kk] = q[kk];
+        // Each visit of a grid point is entered by exactly one sample: mark
+        // its row there (the scatter flushes it once when the visit ends).
+        if (go.mask != nullptr) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                if (((depart >> kk) & 1) && g.valid(q[kk])) go.mask[q[kk]] = 1;
+        }
     }
     // The scatter ends the chunk in the last live sample's cell.
     const int last = 31 - __clz(live_mask);
@@ -1184,7 +1196,7 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
         float4 wa, wb;
         int4 la, lb;
         float v;
-        int move;
+        int depart;
     };
     auto stage = [&](int s) {
         const SampleRec& R = st.rec[s];
@@ -1194,18 +1206,12 @@ __device__ __forceinline__ void scatter_chunk(const GridView<DT>& g, const GradO
         q.la = *reinterpret_cast<const int4*>(R.lk);
         q.lb = *reinterpret_cast<const int4*>(R.lk + 4);
         q.v = R.val[st.my_q];
-        q.move = R.move;
+        q.depart = R.depart;
         return q;
     };
     auto add = [&](const Staged& q) {
-        if (q.move != 0) {
-            if (q.move & kMoveJump) {
-                sc.flush_all(g, go, lane);
-            } else {
-                sc.template shift<4>(((q.move >> 1) & 3) - 1, g, go, lane);
-                sc.template shift<2>(((q.move >> 3) & 3) - 1, g, go, lane);
-                sc.template shift<1>(((q.move >> 5) & 3) - 1, g, go, lane);
-            }
+        if (q.depart != 0) {
+            sc.flush(g, q.depart);
             sc.link[0] = q.la.x; sc.link[1] = q.la.y; sc.link[2] = q.la.z; sc.link[3] = q.la.w;
             sc.link[4] = q.lb.x; sc.link[5] = q.lb.y; sc.link[6] = q.lb.z; sc.link[7] = q.lb.w;
         }
@@ -2500,3 +2506,4 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     if (st) return st;
     return check_launch("volume_render_cuvol_fused");
 }
+
