@@ -222,6 +222,26 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
+/* svox2_volume_render_cuvol_fused without a caller workspace and without any
+ * device allocation: the workspace is a fixed 40 MiB device array inside the
+ * library (one copy per device, looked up with hipGetSymbolAddress on the
+ * device's first call).  Same arguments, results and errors as the fused
+ * call, minus workspace / workspace_bytes.  Batches whose
+ * svox2_volume_render_cuvol_fused_workspace_bytes exceeds 40 MiB (above about
+ * 9000 rays; 8192 rays always fit) fail with SVOX2_ERR_INVALID_ARGUMENT.
+ * Calls that overlap on one device share the array and race: issue them on
+ * one stream.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under graph capture, allocation-free serving
+ * or training loops). */
+int svox2_volume_render_cuvol_fused_noalloc(const svox2_grid_t* grid,
+                                            const float* origins, const float* dirs,
+                                            const float* ray_clip, int64_t n_rays,
+                                            const svox2_render_options_t* opt,
+                                            const float* rgb_gt, float beta_loss,
+                                            float sparsity_loss, float* rgb_out,
+                                            float* log_transmit_out,
+                                            const svox2_grads_t* grads, void* stream);
+
 /* One tile of a multi-request render launch: n_pixels consecutive pixels
  * (row-major, starting at pixel0) of the image that camera `camera` sees of
  * grid `grid`, written to rgb_out[pixel][3] (the request's whole image). */

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol", "svox2_volume_render_cuvol_ordered",
     "svox2_volume_render_cuvol_image",
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
+    "svox2_volume_render_cuvol_fused_noalloc",
     "svox2_ray_order_workspace_bytes", "svox2_volume_render_cuvol_fused_workspace_bytes",
     "svox2_cell_map_bytes",
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
@@ -149,6 +150,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
                                                ctypes.c_size_t, vp],
         "svox2_volume_render_cuvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp, Gr,
                                             vp, ctypes.c_size_t, vp],
+        "svox2_volume_render_cuvol_fused_noalloc": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, vp,
+                                                    Gr, vp],
         "svox2_volume_render_cuvol_stats": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_volume_render_expected_term": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_volume_render_sigma_thresh": [G, vp, vp, vp, i64, O, f32, vp, vp],

@@ -36,6 +36,7 @@
 #include <math.h>
 #include <limits.h>
 #include <string.h>
+#include <mutex>
 #include <type_traits>
 
 namespace svox2amd {
@@ -2507,3 +2508,65 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     return check_launch("volume_render_cuvol_fused");
 }
 
+// ---------------------------------------------------------------------------
+// The fused call without a caller workspace or any allocation: its workspace
+// is a fixed device array in this library.  One copy of the array lives on
+// every device the library runs on, at its own address, which
+// hipGetSymbolAddress gives once per device; host code uses only that
+// address.  Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+namespace svox2amd {
+namespace {
+
+constexpr size_t kScratchBytes = size_t{40} << 20;  // 40 MiB per device
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up on its first use.
+int scratch_ptr(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(SVOX2_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    SVOX2_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(SVOX2_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ok();
+}
+
+}  // namespace
+}  // namespace svox2amd
+
+extern "C" int svox2_volume_render_cuvol_fused_noalloc(const svox2_grid_t* grid,
+                                                       const float* origins, const float* dirs,
+                                                       const float* ray_clip, int64_t n_rays,
+                                                       const svox2_render_options_t* opt,
+                                                       const float* rgb_gt, float beta_loss,
+                                                       float sparsity_loss, float* rgb_out,
+                                                       float* log_transmit_out,
+                                                       const svox2_grads_t* grads, void* stream) {
+    SVOX2_REQUIRE(n_rays >= 0, "n_rays = %lld", static_cast<long long>(n_rays));
+    const size_t need = svox2_volume_render_cuvol_fused_workspace_bytes(n_rays);
+    SVOX2_REQUIRE(need <= kScratchBytes,
+                  "%lld rays need %zu workspace bytes; the no-allocation fused call holds %zu "
+                  "(8192 rays at least)",
+                  static_cast<long long>(n_rays), need, kScratchBytes);
+    void* ws = nullptr;
+    if (n_rays > 0) {
+        const int st = scratch_ptr(&ws);
+        if (st) return st;
+    }
+    return svox2_volume_render_cuvol_fused(grid, origins, dirs, ray_clip, n_rays, opt, rgb_gt,
+                                           beta_loss, sparsity_loss, rgb_out, log_transmit_out,
+                                           grads, ws, n_rays > 0 ? kScratchBytes : 0, stream);
+}
