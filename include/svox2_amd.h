@@ -317,6 +317,20 @@ int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
                          int32_t ignore_last_z, float ndc_coeffx,
                          float ndc_coeffy, float* grad_data, void* stream);
 
+/* svox2_tv_grad_sparse over the contiguous run of n_cells cells starting at
+ * flat cell first_cell and wrapping past the last cell (the cells the
+ * reference's contiguous sparse regulariser draws, svox2.py:2224-2241),
+ * generated in the kernel instead of read from a cell list.  Requires
+ * 0 <= first_cell < cells and n_cells <= cells.  No counterpart in the
+ * reference, which materialises the run as a tensor. */
+int svox2_tv_grad_sparse_run(const int32_t* links, const int32_t size[3],
+                             const void* data, int32_t storage, int64_t rows,
+                             int32_t data_dim, int64_t data_stride,
+                             int64_t grad_stride, int64_t first_cell,
+                             int64_t n_cells, uint8_t* mask_out, int32_t start_dim,
+                             int32_t end_dim, float scale, int32_t ignore_edge,
+                             int32_t ignore_last_z, float* grad_data, void* stream);
+
 /* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
  * rms and grad; grad is zeroed where updated.  Row strides (elements, 0 =
  * dim) apply to data, rms and grad separately.  indexer_kind selects every row,

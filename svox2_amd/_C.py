@@ -25,7 +25,7 @@ __all__ = [
     "SparseGridSpec", "RaysSpec", "CameraSpec", "RenderOptions", "GridOutputGrads",
     "volume_render_cuvol", "volume_render_cuvol_image", "volume_render_cuvol_backward",
     "volume_render_cuvol_fused", "volume_render_cuvol_stats", "volume_render_expected_term",
-    "volume_render_sigma_thresh", "tv_grad_sparse",
+    "volume_render_sigma_thresh", "tv_grad_sparse", "tv_grad_sparse_run",
     "rmsprop_step", "sgd_step", "rmsprop_step_grouped", "dilate", "grid_weight_render",
     "resample_density", "resample_rows", "threshold_mask", "mask_to_links", "sample_grid", "sample_grid_backward", "accel_dist_prop",
     "library_path", "load_library", "EXPORTED_SYMBOLS",
@@ -51,7 +51,7 @@ EXPORTED_SYMBOLS = (
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_volume_render_expected_term",
     "svox2_volume_render_sigma_thresh", "svox2_sample_grid", "svox2_sample_grid_backward",
-    "svox2_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
+    "svox2_tv_grad_sparse", "svox2_tv_grad_sparse_run", "svox2_rmsprop_step", "svox2_sgd_step",
     "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_resample_density", "svox2_resample_rows",
     "svox2_grid_weight_render", "svox2_threshold_mask", "svox2_dilate",
@@ -159,6 +159,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
         "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, vp, i64,
                                  vp, i32, i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
+        "svox2_tv_grad_sparse_run": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, i64,
+                                     i64, vp, i32, i32, f32, i32, i32, vp, vp],
         "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
                                f32, f32, f32, vp],
         "svox2_sgd_step": [vp, i32, vp, i64, i32, i64, i64, i32, vp, i64, f32, f32, vp],
@@ -734,6 +736,34 @@ def tv_grad_sparse(links: torch.Tensor, data: torch.Tensor, rand_cells: torch.Te
               int(start_dim), int(end_dim), float(scale), int(bool(use_logalpha)),
               float(logalpha_delta), int(bool(ignore_edge)), int(bool(ignore_last_z)),
               float(ndc_coeffx), float(ndc_coeffy), _ptr(grad_data), _stream(data))
+
+
+def tv_grad_sparse_run(links: torch.Tensor, data: torch.Tensor, first_cell: int, n_cells: int,
+                       mask_out: Optional[torch.Tensor], start_dim: int, end_dim: int,
+                       scale: float, ignore_edge: bool, ignore_last_z: bool,
+                       grad_data: torch.Tensor) -> None:
+    """tv_grad_sparse over the cells first_cell, first_cell + 1, ... (n_cells
+    of them, wrapping past the last cell), generated in the kernel: the
+    contiguous run of the sparse regularisers without a cell tensor.  No
+    reference counterpart."""
+    _check(links, "links", (torch.int32,), 3)
+    _check_rows(data, "data", _DATA_DTYPES)
+    _check_rows(grad_data, "grad_data", (torch.float32,))
+    if grad_data.shape != data.shape:
+        raise RuntimeError("grad_data must match data's shape")
+    mask_ptr = None
+    if mask_out is not None and mask_out.dim() > 0 and mask_out.numel() > 0:
+        _check(mask_out, "mask_out", (torch.bool, torch.uint8))
+        if mask_out.numel() != data.size(0):
+            raise RuntimeError("mask_out must have one entry per row")
+        mask_ptr = _ptr(mask_out)
+    size = (ctypes.c_int32 * 3)(*links.shape)
+    with _on_device(data):
+        _call("svox2_tv_grad_sparse_run", _ptr(links), size, _ptr(data), _storage_of(data),
+              data.size(0), data.size(1), _row_stride(data), _row_stride(grad_data),
+              int(first_cell), int(n_cells), mask_ptr, int(start_dim), int(end_dim),
+              float(scale), int(bool(ignore_edge)), int(bool(ignore_last_z)), _ptr(grad_data),
+              _stream(data))
 
 
 def _indexer_args(indexer: torch.Tensor, rows: int):

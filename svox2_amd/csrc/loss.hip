@@ -43,8 +43,10 @@ struct TvJob {
     const DT* data;
     int rows;
     int64_t dst, gst;  // row strides of data and grad
-    const int32_t* cells;
+    const int32_t* cells;  // listed cells, or NULL for the run first_cell + k (mod n_total)
     int64_t n_cells;
+    int64_t first_cell;
+    int64_t n_total;
     int start_dim, nch;
     float scale;  // already divided by n_cells
     int ignore_edge, ignore_last_z;
@@ -68,7 +70,9 @@ template <typename DT>
 __device__ __forceinline__ CellLinks cell_links(const TvJob<DT>& J, int64_t k) {
     CellLinks c{-1, -1, -1, -1, false};
     if (k >= J.n_cells) return c;
-    const int xyz = J.cells[k];
+    int64_t run = J.first_cell + k;
+    if (run >= J.n_total) run -= J.n_total;
+    const int xyz = J.cells != nullptr ? J.cells[k] : static_cast<int>(run);
     if (xyz < 0) return c;
     uint32_t z, y;
     const uint32_t xy = J.div_z.divmod(static_cast<uint32_t>(xyz), z);
@@ -201,8 +205,9 @@ int launch(const TvJob<DT>& J, hipStream_t s) {
 template <typename DT>
 int run_tv(const DT* data, const int32_t* links, const int32_t size[3], int64_t rows,
            int32_t data_dim, int64_t data_stride, int64_t grad_stride, const int32_t* cells,
-           int64_t n_cells, uint8_t* mask, int32_t start_dim, int32_t end_dim, float scale,
-           int32_t ignore_edge, int32_t ignore_last_z, float* grad, hipStream_t s) {
+           int64_t n_cells, int64_t first_cell, uint8_t* mask, int32_t start_dim,
+           int32_t end_dim, float scale, int32_t ignore_edge, int32_t ignore_last_z, float* grad,
+           hipStream_t s) {
     TvJob<DT> J;
     J.links = links;
     J.s0 = size[0];
@@ -217,6 +222,8 @@ int run_tv(const DT* data, const int32_t* links, const int32_t size[3], int64_t 
     SVOX2_REQUIRE(J.dst >= data_dim && J.gst >= data_dim, "row stride below data_dim");
     J.cells = cells;
     J.n_cells = n_cells;
+    J.first_cell = first_cell;
+    J.n_total = static_cast<int64_t>(size[0]) * size[1] * size[2];
     J.start_dim = start_dim;
     J.nch = end_dim - start_dim;
     // The reference divides the scale by the number of cells (loss_kernel.cu:597).
@@ -236,6 +243,47 @@ int run_tv(const DT* data, const int32_t* links, const int32_t size[3], int64_t 
 
 using namespace svox2amd;
 
+namespace svox2amd {
+namespace {
+
+// Checks shared by both entry points; dispatches on the storage kind.
+int tv_entry(const int32_t* links, const int32_t size[3], const void* data, int32_t storage,
+             int64_t rows, int32_t data_dim, int64_t data_stride, int64_t grad_stride,
+             const int32_t* cells, int64_t n_cells, int64_t first_cell, uint8_t* mask_out,
+             int32_t start_dim, int32_t end_dim, float scale, int32_t ignore_edge,
+             int32_t ignore_last_z, float* grad_data, void* stream) {
+    SVOX2_REQUIRE(size != nullptr && links != nullptr, "links / size are NULL");
+    for (int i = 0; i < 3; ++i) SVOX2_REQUIRE(size[i] >= 1, "grid size[%d] = %d", i, size[i]);
+    const int64_t cells_total = static_cast<int64_t>(size[0]) * size[1] * size[2];
+    SVOX2_REQUIRE(cells_total < INT32_MAX, "too many cells");
+    SVOX2_REQUIRE(storage == SVOX2_STORAGE_F32 || storage == SVOX2_STORAGE_BF16,
+                  "unknown storage kind %d", storage);
+    SVOX2_REQUIRE(rows >= 0 && rows < INT32_MAX, "rows out of range");
+    SVOX2_REQUIRE(data_dim >= 1, "data_dim must be >= 1");
+    SVOX2_REQUIRE(0 <= start_dim && start_dim < end_dim && end_dim <= data_dim,
+                  "channel range [%d, %d) invalid for %d channels", start_dim, end_dim, data_dim);
+    SVOX2_REQUIRE(n_cells >= 0, "negative cell count");
+    if (cells == nullptr)
+        SVOX2_REQUIRE(0 <= first_cell && first_cell < cells_total && n_cells <= cells_total,
+                      "cell run [%lld, +%lld) invalid for %lld cells",
+                      static_cast<long long>(first_cell), static_cast<long long>(n_cells),
+                      static_cast<long long>(cells_total));
+    if (n_cells == 0 || rows == 0) return ok();
+    SVOX2_REQUIRE(data != nullptr && grad_data != nullptr, "data / grad_data are NULL");
+    SVOX2_REQUIRE(data_stride >= 0 && grad_stride >= 0, "negative row stride");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (storage == SVOX2_STORAGE_BF16)
+        return run_tv(static_cast<const uint16_t*>(data), links, size, rows, data_dim, data_stride,
+                      grad_stride, cells, n_cells, first_cell, mask_out, start_dim, end_dim,
+                      scale, ignore_edge, ignore_last_z, grad_data, s);
+    return run_tv(static_cast<const float*>(data), links, size, rows, data_dim, data_stride,
+                  grad_stride, cells, n_cells, first_cell, mask_out, start_dim, end_dim, scale,
+                  ignore_edge, ignore_last_z, grad_data, s);
+}
+
+}  // namespace
+}  // namespace svox2amd
+
 extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3], const void* data,
                                     int32_t storage, int64_t rows, int32_t data_dim,
                                     int64_t data_stride, int64_t grad_stride,
@@ -248,28 +296,21 @@ extern "C" int svox2_tv_grad_sparse(const int32_t* links, const int32_t size[3],
     (void)logalpha_delta;
     (void)ndc_coeffx;
     (void)ndc_coeffy;
-    SVOX2_REQUIRE(size != nullptr && links != nullptr, "links / size are NULL");
-    for (int i = 0; i < 3; ++i) SVOX2_REQUIRE(size[i] >= 1, "grid size[%d] = %d", i, size[i]);
-    const int64_t cells = static_cast<int64_t>(size[0]) * size[1] * size[2];
-    SVOX2_REQUIRE(cells < INT32_MAX, "too many cells");
     SVOX2_REQUIRE(!use_logalpha, "use_logalpha is no longer supported (as in the reference)");
-    SVOX2_REQUIRE(storage == SVOX2_STORAGE_F32 || storage == SVOX2_STORAGE_BF16,
-                  "unknown storage kind %d", storage);
-    SVOX2_REQUIRE(rows >= 0 && rows < INT32_MAX, "rows out of range");
-    SVOX2_REQUIRE(data_dim >= 1, "data_dim must be >= 1");
-    SVOX2_REQUIRE(0 <= start_dim && start_dim < end_dim && end_dim <= data_dim,
-                  "channel range [%d, %d) invalid for %d channels", start_dim, end_dim, data_dim);
-    SVOX2_REQUIRE(n_cells >= 0, "negative cell count");
-    if (n_cells == 0 || rows == 0) return ok();
-    SVOX2_REQUIRE(data != nullptr && rand_cells != nullptr && grad_data != nullptr,
-                  "data / rand_cells / grad_data are NULL");
-    SVOX2_REQUIRE(data_stride >= 0 && grad_stride >= 0, "negative row stride");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (storage == SVOX2_STORAGE_BF16)
-        return run_tv(static_cast<const uint16_t*>(data), links, size, rows, data_dim, data_stride,
-                      grad_stride, rand_cells, n_cells, mask_out, start_dim, end_dim, scale,
-                      ignore_edge, ignore_last_z, grad_data, s);
-    return run_tv(static_cast<const float*>(data), links, size, rows, data_dim, data_stride,
-                  grad_stride, rand_cells, n_cells, mask_out, start_dim, end_dim, scale,
-                  ignore_edge, ignore_last_z, grad_data, s);
+    SVOX2_REQUIRE(n_cells == 0 || rand_cells != nullptr, "rand_cells is NULL");
+    return tv_entry(links, size, data, storage, rows, data_dim, data_stride, grad_stride,
+                    rand_cells, n_cells, 0, mask_out, start_dim, end_dim, scale, ignore_edge,
+                    ignore_last_z, grad_data, stream);
+}
+
+extern "C" int svox2_tv_grad_sparse_run(const int32_t* links, const int32_t size[3],
+                                        const void* data, int32_t storage, int64_t rows,
+                                        int32_t data_dim, int64_t data_stride,
+                                        int64_t grad_stride, int64_t first_cell,
+                                        int64_t n_cells, uint8_t* mask_out, int32_t start_dim,
+                                        int32_t end_dim, float scale, int32_t ignore_edge,
+                                        int32_t ignore_last_z, float* grad_data, void* stream) {
+    return tv_entry(links, size, data, storage, rows, data_dim, data_stride, grad_stride,
+                    nullptr, n_cells, first_cell, mask_out, start_dim, end_dim, scale,
+                    ignore_edge, ignore_last_z, grad_data, stream);
 }

@@ -20,9 +20,12 @@ How a request flows:
 * Each step copies the tables to fixed device buffers and replays the group's
   graph. The graph is one ``svox2_volume_render_cuvol_tiles`` launch captured
   once, whose shape depends only on ``max_tiles`` and ``tile_pixels``.
-* Steps are pipelined over two table slots, each with its own buffers and
-  graphs: the host plans and uploads step k+1 while step k runs, and it
-  learns of completions from per-step events.
+* Steps are pipelined over ``depth`` table slots, each with its own buffers
+  and graphs: the host plans and uploads step k+1 while step k runs, and it
+  learns of completions from per-step events. By default the steps run in
+  order on one stream, so the device serves them first come, first served
+  (``concurrent=True`` gives each slot a stream of its own, and consecutive
+  steps then share the device).
 * A request completes once all its tiles have run. Its image is then
   bit-identical to ``volume_render_image`` of the same grid and camera,
   because every pixel runs the same march code.
@@ -154,8 +157,10 @@ class _Slot:
     """One in-flight step's tables: a device buffer [count | tiles | cameras]
     per launch group, its pinned staging copy, captured graphs and stream."""
 
-    def __init__(self, device):
-        self.stream = torch.cuda.Stream(device=device)  # default priority
+    def __init__(self, device, stream=None):
+        # default priority; steps share one stream unless the server runs
+        # them concurrently
+        self.stream = stream if stream is not None else torch.cuda.Stream(device=device)
         self.cam_version: List[int] = []  # per launch group: camera table version copied
         self.host: List[torch.Tensor] = []
         self.host_np: List[np.ndarray] = []
@@ -181,12 +186,16 @@ class RenderServer:
     :param max_tiles: tiles per launch and group.
     :param max_active: camera slots (requests in flight) at most.
     :param use_graph: replay captured graphs (else launch directly).
+    :param concurrent: run the table slots' steps on streams of their own, so
+        that consecutive steps share the device; by default they run in order
+        on one stream (first come, first served on the device, as one launch
+        per request is), while the host still prepares the next steps.
     """
 
     def __init__(self, grids: Sequence[SparseGrid], opt: Optional[RenderOptions] = None,
                  hbm_budget: int = 1 << 30, tile_pixels: int = 4096, max_tiles: int = 128,
                  max_active: int = 64, use_graph: bool = True, policy: str = "fifo",
-                 depth: int = 3):
+                 depth: int = 3, concurrent: bool = False):
         if not grids:
             raise ValueError("RenderServer needs at least one grid")
         self.grids = list(grids)
@@ -236,7 +245,9 @@ class RenderServer:
                 self._maps.append(spec.cell_map)
                 structs.append(_C.grid_struct(spec))
             grp.grids_dev = _C.pack_structs(structs).to(self.device)
-        self.slots = [_Slot(self.device) for _ in range(self.depth)]
+        self.concurrent = bool(concurrent)
+        shared = None if self.concurrent else torch.cuda.Stream(device=self.device)
+        self.slots = [_Slot(self.device, shared) for _ in range(self.depth)]
         for slot in self.slots:
             for _ in self.groups:
                 slot.cam_version.append(-1)
@@ -336,19 +347,21 @@ class RenderServer:
             done += self._retire(self._inflight.pop(0))
         self._next_slot = (self._next_slot + 1) % self.depth
         finishing: List[RenderRequest] = []
-        # Steps run on the two slots' streams, so a step can start while the
-        # previous one drains.  They share nothing a step writes: each slot
-        # has its own tables, tiles of one image are disjoint, and camera
-        # slots and images are reused only after the event of the step that
-        # last read them.  The slot stream follows the caller's stream, where
-        # the grids were written.
+        # Steps run on the slots' stream (one stream shared by all slots
+        # unless the server is concurrent, then a stream per slot, so a step
+        # can start while the previous one drains).  Steps share nothing a
+        # step writes: each slot has its own tables, tiles of one image are
+        # disjoint, and camera slots and images are reused only after the
+        # event of the step that last read them.  The slot stream follows the
+        # caller's stream, where the grids were written.
         stream = slot.stream
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
             self._fill_and_launch(slot, pending, finishing)
         slot.finishing = finishing
-        slot.event = torch.cuda.Event()
-        slot.event.record(stream)
+        if slot.event is None:
+            slot.event = torch.cuda.Event()
+        slot.event.record(stream)  # the slot's previous step has been retired
         slot.busy = True
         self._inflight.append(slot)
         self.steps += 1

@@ -118,6 +118,10 @@ def test_rand_cells_and_indexer():
         start = np.random.RandomState(seed).randint(0, 1000)
         got = g._get_rand_cells(0.05).numpy()
         assert np.array_equal(got, (start + np.arange(50)) % 1000)
+        # the run the TV kernel generates in place: the same draw
+        np.random.seed(seed)
+        assert g._rand_cell_run(0.05) == (start, 50)
+    assert g._rand_cell_run(1.0) is None
     assert g._get_rand_cells(1.0) is None
     assert g._get_rand_cells(1.0, force=True).numel() == 1000
     # the optimisers take the touched-row mask as it is (no host sync); no
