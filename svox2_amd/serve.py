@@ -11,6 +11,9 @@ How a request flows:
 * Admission: it is admitted when a camera slot is free and its output image
   (H x W x 3 float32) fits in the HBM budget next to the images of the
   requests in flight.
+* Idle dispatch: when no step is in flight and one fresh request is pending,
+  its whole image goes out at once as one image-kernel launch, with no tile
+  table to plan or upload (``idle_direct``).
 * Continuous batching: every ``step`` builds one tile table per launch group
   (grids of one storage kind and basis) from all admitted requests: their
   remaining tiles of ``tile_pixels`` pixels in admission order, until the
@@ -22,10 +25,9 @@ How a request flows:
   once, whose shape depends only on ``max_tiles`` and ``tile_pixels``.
 * Steps are pipelined over ``depth`` table slots, each with its own buffers
   and graphs: the host plans and uploads step k+1 while step k runs, and it
-  learns of completions from per-step events. By default the steps run in
-  order on one stream, so the device serves them first come, first served
-  (``concurrent=True`` gives each slot a stream of its own, and consecutive
-  steps then share the device).
+  learns of completions from per-step events. By default each slot has a
+  stream of its own, so consecutive steps share the device
+  (``concurrent=False`` runs them in order on one stream instead).
 * A request completes once all its tiles have run. Its image is then
   bit-identical to ``volume_render_image`` of the same grid and camera,
   because every pixel runs the same march code.
@@ -60,6 +62,7 @@ class RenderRequest:
     # scheduler state
     slot: int = -1
     next_pixel: int = 0
+    cam_struct: Optional[object] = None
 
     @property
     def pixels(self) -> int:
@@ -186,16 +189,20 @@ class RenderServer:
     :param max_tiles: tiles per launch and group.
     :param max_active: camera slots (requests in flight) at most.
     :param use_graph: replay captured graphs (else launch directly).
-    :param concurrent: run the table slots' steps on streams of their own, so
-        that consecutive steps share the device; by default they run in order
-        on one stream (first come, first served on the device, as one launch
-        per request is), while the host still prepares the next steps.
+    :param concurrent: run the table slots' steps on streams of their own
+        (the default), so that consecutive steps share the device; with False
+        they run in order on one stream (first come, first served on the
+        device, as one launch per request is).  Under overload the concurrent
+        steps give the lower latencies.
+    :param idle_direct: when the device is idle and one fresh request is
+        pending, render its whole image with one direct image-kernel launch
+        instead of a tile step (no batching delay when nothing queues).
     """
 
     def __init__(self, grids: Sequence[SparseGrid], opt: Optional[RenderOptions] = None,
                  hbm_budget: int = 1 << 30, tile_pixels: int = 4096, max_tiles: int = 128,
                  max_active: int = 64, use_graph: bool = True, policy: str = "fifo",
-                 depth: int = 3, concurrent: bool = False):
+                 depth: int = 3, concurrent: bool = True, idle_direct: bool = True):
         if not grids:
             raise ValueError("RenderServer needs at least one grid")
         self.grids = list(grids)
@@ -217,6 +224,7 @@ class RenderServer:
         self.free_cams = list(range(self.max_active))
         self.bytes_in_flight = 0
         self.steps = 0
+        self.direct_launches = 0  # steps that were one direct image launch
         self.rays_rendered = 0
         self.cam_size = ctypes.sizeof(_C._Cam)
         self.tile_size = ctypes.sizeof(_C._Tile)
@@ -237,6 +245,8 @@ class RenderServer:
         # The device grid tables hold raw pointers: the server keeps every
         # tensor they point at, the cell maps included, alive.
         self._maps = []
+        self._grid_structs: Dict[int, object] = {}  # host descriptors, for direct launches
+        self._opt_struct = _C._opt_struct(self.opt._to_cpp())
         for grp in self.groups:
             structs = []
             for i in grp.grid_ids:
@@ -244,8 +254,10 @@ class RenderServer:
                 spec.cell_map = _C.build_cell_map(spec.links, spec.density_data.size(0))
                 self._maps.append(spec.cell_map)
                 structs.append(_C.grid_struct(spec))
+                self._grid_structs[i] = structs[-1]
             grp.grids_dev = _C.pack_structs(structs).to(self.device)
         self.concurrent = bool(concurrent)
+        self.idle_direct = bool(idle_direct)
         shared = None if self.concurrent else torch.cuda.Stream(device=self.device)
         self.slots = [_Slot(self.device, shared) for _ in range(self.depth)]
         for slot in self.slots:
@@ -308,6 +320,7 @@ class RenderServer:
             req.admitted = now
             self.bytes_in_flight += req.image_bytes
             cam = _C.camera_struct(req.camera._to_cpp())
+            req.cam_struct = cam
             o = req.slot * self.cam_size
             self.cams[o:o + self.cam_size] = bytes(memoryview(cam))
             self.cam_version += 1
@@ -357,7 +370,11 @@ class RenderServer:
         stream = slot.stream
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
-            self._fill_and_launch(slot, pending, finishing)
+            if self.idle_direct and not self._inflight and len(pending) == 1 and \
+                    pending[0].next_pixel == 0:
+                self._launch_direct(pending[0], stream, finishing)
+            else:
+                self._fill_and_launch(slot, pending, finishing)
         slot.finishing = finishing
         if slot.event is None:
             slot.event = torch.cuda.Event()
@@ -366,6 +383,19 @@ class RenderServer:
         self._inflight.append(slot)
         self.steps += 1
         return done
+
+    def _launch_direct(self, req: RenderRequest, stream, finishing: List[RenderRequest]) -> None:
+        """A lone request on an idle device: its whole image in one launch of
+        the image kernel (the same per-pixel march as the tiles kernel), with
+        no tile table to plan or upload."""
+        _C._call("svox2_volume_render_cuvol_image", ctypes.byref(self._grid_structs[req.grid]),
+                 ctypes.byref(req.cam_struct), ctypes.byref(self._opt_struct),
+                 ctypes.c_void_p(req.image.data_ptr()), None,
+                 ctypes.c_void_p(stream.cuda_stream))
+        req.next_pixel = req.pixels
+        self.rays_rendered += req.pixels
+        self.direct_launches += 1
+        finishing.append(req)
 
     def _fill_and_launch(self, slot: _Slot, pending: List[RenderRequest],
                          finishing: List[RenderRequest]) -> None:

@@ -52,6 +52,30 @@ def test_mix_equals_alone(gpu, use_graph):
         assert torch.equal(r.image, alone), (r.grid, r.camera.width, r.camera.height)
 
 
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_idle_requests_go_direct(gpu, concurrent):
+    """Requests that arrive on an idle server are rendered by one direct
+    image launch each (no tile step), bit for bit as alone; with the tile
+    steps switched off for them (idle_direct=False) the same requests go
+    through tiles and still match."""
+    grids, cams = _grids(gpu)
+    reqs = _requests(cams, 5, seed=6)
+    for i, r in enumerate(reqs):
+        r.arrival = 0.03 * i  # far apart: each finds the device idle
+    server = RenderServer(grids, tile_pixels=700, max_tiles=5, concurrent=concurrent)
+    server.run(reqs, realtime=True)
+    assert server.direct_launches == len(reqs) == server.steps
+    tiled = RenderServer(grids, tile_pixels=700, max_tiles=5, concurrent=concurrent,
+                         idle_direct=False)
+    again = _requests(cams, 5, seed=6)
+    tiled.run(again, realtime=False)
+    assert tiled.direct_launches == 0
+    for r, q in zip(reqs, again):
+        with torch.no_grad():
+            alone = grids[r.grid].volume_render_image(r.camera)
+        assert torch.equal(r.image, alone) and torch.equal(q.image, alone)
+
+
 def test_one_per_launch_baseline(gpu):
     grids, cams = _grids(gpu)
     reqs = _requests(cams, 4, seed=5)
