@@ -793,15 +793,29 @@ constexpr int kLogWords = 36;
 // (n > 0, -) n records follow; (0, next) continue at slot next; (-1, -) end.
 // state[r] = 1 when the ray's stream is complete, 2 when the overflow region
 // ran out (the backward then re-marches the ray).
+//
+// Long rays are replayed in segments: a ray with at least kRecSplitMin
+// recorded chunks is listed (split, long_list, up to split_cap rays), and
+// seg[r * (kRecSegs - 1) + m - 1] holds the header slot of its recorded
+// chunk m * kRecSegChunks (-1 past its end).  The header of such a chunk
+// carries, in its second float4, the colour composited before the chunk, so
+// a segment's backward can start there.
 struct RecStream {
     float4* __restrict__ rec;
     int32_t* __restrict__ state;
-    uint32_t* __restrict__ next_block;  // overflow slots taken so far
+    uint32_t* __restrict__ next_block;  // [0] overflow slots taken so far, [1] long rays listed
+    int32_t* __restrict__ seg;
+    int32_t* __restrict__ split;        // per ray: its index in long_list, or -1
+    int32_t* __restrict__ long_list;
+    int64_t split_cap;
     int64_t fixed;                      // slots per ray in the fixed region
     int64_t overflow_base;              // first overflow slot
     int64_t overflow_slots;             // slots in the overflow region
 };
 constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
+constexpr int kRecSegChunks = 4;  // recorded chunks per replay segment
+constexpr int kRecSegs = 6;       // segments of a split ray (the last runs to its end)
+constexpr int kRecSplitMin = 2 * kRecSegChunks;  // recorded chunks that make a ray split
 
 __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int64_t next) {
     rec[6 * slot] = make_float4(__int_as_float(n), __int_as_float(static_cast<int>(next)), 0.f,
@@ -825,6 +839,7 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     int64_t cur = RECORD ? ray * rs->fixed : 0;
     int64_t end = RECORD ? cur + rs->fixed : 0;
     bool rec_ok = RECORD;
+    int n_rec_chunks = 0;  // chunks whose records were written
     float acc[3] = {0.f, 0.f, 0.f};  // this lane's share of the composited colour
     uint32_t n_iter = 0, n_eval = 0, n_hit = 0;
     uint32_t n_chunks = 0;
@@ -858,6 +873,19 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
                 ++n_live_chunks;
             }
             if (n_live > 0) {
+                // Recorded chunk k = n_rec_chunks opens replay segment k / kRecSegChunks:
+                // keep the colour composited so far (spread over lanes 0, 8, 16, 24).
+                const bool seg_start = RECORD && rec_ok && n_rec_chunks > 0 &&
+                                       n_rec_chunks % kRecSegChunks == 0 &&
+                                       n_rec_chunks / kRecSegChunks < kRecSegs;
+                float before[3] = {0.f, 0.f, 0.f};
+                if (seg_start) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        before[a] = acc[a] + __shfl_xor(acc[a], 8, kGroup);
+                        before[a] += __shfl_xor(before[a], 16, kGroup);
+                    }
+                }
                 float cs[3] = {0.f, 0.f, 0.f};
                 chunk_colours<DT, B, RECORD>(g, rc, c, lane, cs, acc);
                 if (RECORD && rec_ok) {
@@ -876,6 +904,15 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
                         }
                     }
                     if (rec_ok) {
+                        const int k = n_rec_chunks++;
+                        if (seg_start && lane == 0) {
+                            // A segment may start here: its header slot and the
+                            // colour composited before the chunk.
+                            rs->seg[ray * (kRecSegs - 1) + k / kRecSegChunks - 1] =
+                                static_cast<int32_t>(cur);
+                            rs->rec[6 * cur + 1] =
+                                make_float4(before[0], before[1], before[2], 0.f);
+                        }
                         if (lane == 0) put_header(rs->rec, cur, n_live, 0);
                         if (c.live) {
                             const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
@@ -924,6 +961,15 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     if (RECORD && lane == 0) {
         if (rec_ok) put_header(rs->rec, cur, -1, 0);
         rs->state[ray] = rec_ok ? 1 : 2;
+        for (int m = 1; m < kRecSegs; ++m)
+            if (!rec_ok || m * kRecSegChunks >= n_rec_chunks) rs->seg[ray * (kRecSegs - 1) + m - 1] = -1;
+        int32_t idx = -1;
+        if (rec_ok && n_rec_chunks >= kRecSplitMin) {
+            idx = static_cast<int32_t>(atomicAdd(rs->next_block + 1, 1u));
+            if (idx < rs->split_cap) rs->long_list[idx] = static_cast<int32_t>(ray);
+            else idx = -1;
+        }
+        rs->split[ray] = idx;
     }
     if (LOG) {
         const int n_logged = static_cast<int>(min(n_chunks, static_cast<uint32_t>(kLogChunks)));
@@ -1335,12 +1381,13 @@ __device__ __forceinline__ void march_backward(const GridView<DT>& g, const OptV
 
 // Backward of one ray from its record stream (the fused call's forward wrote
 // every contributing sample with its colour and transmittance): no link,
-// density or SH reads, only the records and the gradient scatter.
+// density or SH reads, only the records and the gradient scatter.  Starts at
+// the header slot pos and replays at most max_chunks recorded chunks.
 template <typename DT, int B>
 __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const RecStream& rs,
-                                                int64_t ray, int lane, BwdRay<B>& st,
-                                                const GradOut& go) {
-    int64_t pos = ray * rs.fixed;
+                                                int64_t pos, int max_chunks, int lane,
+                                                BwdRay<B>& st, const GradOut& go) {
+    int chunks = 0;
     for (int guard = 0; guard < kMaxIters; ++guard) {
         const float4 h = rs.rec[6 * pos];
         const int n = __float_as_int(h.x);
@@ -1349,6 +1396,7 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
             pos = __float_as_int(h.y);
             continue;
         }
+        if (chunks++ == max_chunks) break;  // the next segment's first chunk
         const bool live = lane < n;
         float p[3] = {0.f, 0.f, 0.f}, cs[3] = {0.f, 0.f, 0.f};
         int l[3] = {0, 0, 0}, lk[8];
@@ -1749,11 +1797,27 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
         const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
         const uint32_t* __restrict__ order, GradOut go, RecStream rs) {
-    const int64_t slot = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int64_t item = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
-    if (slot >= n) return;
-    const int64_t ray = order != nullptr ? static_cast<int64_t>(order[slot]) : slot;
-    if (rs.state[ray] != 1) return;  // re-marched by k_render_backward_mse
+    int64_t ray, pos;
+    int max_chunks = kMaxIters;
+    const float* before = nullptr;  // colour composited before the segment
+    if (item < n) {  // a ray from its start; a split ray's first segment only
+        ray = order != nullptr ? static_cast<int64_t>(order[item]) : item;
+        if (rs.state[ray] != 1) return;  // re-marched by k_render_backward_mse
+        pos = ray * rs.fixed;
+        if (rs.split[ray] >= 0) max_chunks = kRecSegChunks;
+    } else {  // segment m >= 1 of listed ray j
+        const int64_t e = item - n;
+        const int64_t j = e / (kRecSegs - 1);
+        const int m = 1 + static_cast<int>(e % (kRecSegs - 1));
+        if (j >= min(static_cast<int64_t>(rs.next_block[1]), rs.split_cap)) return;
+        ray = rs.long_list[j];
+        pos = rs.seg[ray * (kRecSegs - 1) + m - 1];
+        if (pos < 0) return;  // the ray ends before this segment
+        if (m < kRecSegs - 1) max_chunks = kRecSegChunks;
+        before = reinterpret_cast<const float*>(rs.rec + 6 * pos + 1);
+    }
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
     float c3[3], gq[3];
@@ -1767,9 +1831,11 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         const float tin = __expf(log_t_fwd[ray]);
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
+    if (before != nullptr)  // accum after the samples before the segment
+        accum -= fmaf(before[0], gq[0], fmaf(before[1], gq[1], before[2] * gq[2]));
     __shared__ BwdShared sh;
     BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
-    replay_backward<DT, B>(g, rs, ray, lane, st, go);
+    replay_backward<DT, B>(g, rs, pos, max_chunks, lane, st, go);
 }
 
 // Backward of the fused MSE step: dL/drgb = (rgb - gt) * 2 / (3n) from the
@@ -1951,12 +2017,18 @@ constexpr int64_t kRecSlotBytes = 96;
 
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+int64_t split_cap_of(int64_t n) { return (n + 7) / 8; }
+
+// [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | long_list | counters | records]
 size_t fused_workspace_bytes(int64_t n) {
     const int64_t order = static_cast<int64_t>(order_workspace_bytes(n));
     const int64_t state = align256(4 * n);
+    const int64_t seg = align256(4 * n * (kRecSegs - 1));
+    const int64_t split = align256(4 * n);
+    const int64_t list = align256(4 * split_cap_of(n));
     const int64_t counter = 256;
     const int64_t recs = n * (kRecFixed + kRecOverflow) * kRecSlotBytes;
-    return static_cast<size_t>(order + state + counter + recs);
+    return static_cast<size_t>(order + state + seg + split + list + counter + recs);
 }
 
 // Fused-call workspace with the chunk log (batches that re-march): the
@@ -1982,6 +2054,13 @@ RecStream record_stream(void* ws, int64_t n) {
     RecStream rs{};
     rs.state = reinterpret_cast<int32_t*>(p);
     p += align256(4 * n);
+    rs.seg = reinterpret_cast<int32_t*>(p);
+    p += align256(4 * n * (kRecSegs - 1));
+    rs.split = reinterpret_cast<int32_t*>(p);
+    p += align256(4 * n);
+    rs.long_list = reinterpret_cast<int32_t*>(p);
+    rs.split_cap = split_cap_of(n);
+    p += align256(4 * split_cap_of(n));
     rs.next_block = reinterpret_cast<uint32_t*>(p);
     p += 256;
     rs.rec = reinterpret_cast<float4*>(p);
@@ -2464,7 +2543,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
-                           reinterpret_cast<uint32_t*>(rs.next_block), 1);
+                           reinterpret_cast<uint32_t*>(rs.next_block), 2);
     }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
@@ -2489,9 +2568,11 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         // The key kernel cleared both histograms; this sort uses the second.
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s, 1, true);
         if (rs.rec != nullptr)
-            hipLaunchKernelGGL((k_render_backward_replay<DT, B>), dim3(blocks), dim3(kBlock), 0,
-                               s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta, sparsity_loss,
-                               norm, static_cast<const uint32_t*>(w.order), go, rs);
+            hipLaunchKernelGGL((k_render_backward_replay<DT, B>),
+                               dim3(blocks_for((n_rays + rs.split_cap * (kRecSegs - 1)) * kGroup,
+                                               kBlock)),
+                               dim3(kBlock), 0, s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta,
+                               sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs);
         // Batches of up to 4 device fills use the 3-wave build.
         if (n_rays <= 4 * order_min_rays())
             hipLaunchKernelGGL((k_render_backward_mse<DT, B, SVOX2_BWD_WAVES_FEW>), dim3(blocks),
