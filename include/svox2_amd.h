@@ -218,7 +218,7 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 /* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
  * fastest for an n_rays batch (0 for an empty batch): the record stream
  * (about 4.6 KiB per ray) below the ordering threshold, the ray-ordering
- * workspace plus the chunk log (about 2.5 KiB per ray) at or above it.  No
+ * workspace plus the chunk log (about 2.3 KiB per ray) at or above it.  No
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
