@@ -11,9 +11,9 @@ How a request flows:
 * Admission: it is admitted when a camera slot is free and its output image
   (H x W x 3 float32) fits in the HBM budget next to the images of the
   requests in flight.
-* Idle dispatch: when no step is in flight and one fresh request is pending,
-  its whole image goes out at once as one image-kernel launch, with no tile
-  table to plan or upload (``idle_direct``).
+* Direct dispatch: while at most ``direct_max`` fresh requests are pending,
+  each goes out at once as one image-kernel launch, in arrival order on one
+  stream, with no tile table to plan or upload; a longer queue is batched.
 * Continuous batching: every ``step`` builds one tile table per launch group
   (grids of one storage kind and basis) from all admitted requests: their
   remaining tiles of ``tile_pixels`` pixels in admission order, until the
@@ -194,15 +194,19 @@ class RenderServer:
         they run in order on one stream (first come, first served on the
         device, as one launch per request is).  Under overload the concurrent
         steps give the lower latencies.
-    :param idle_direct: when the device is idle and one fresh request is
-        pending, render its whole image with one direct image-kernel launch
-        instead of a tile step (no batching delay when nothing queues).
+    :param idle_direct: launch short queues directly (below); False batches
+        every request into tile steps.
+    :param direct_max: while at most this many requests are pending and none
+        has started, each is rendered by one direct image-kernel launch, in
+        arrival order on one stream, instead of a tile step: no batching delay
+        and no tile table when little queues; longer queues are batched.
     """
 
     def __init__(self, grids: Sequence[SparseGrid], opt: Optional[RenderOptions] = None,
                  hbm_budget: int = 1 << 30, tile_pixels: int = 4096, max_tiles: int = 128,
                  max_active: int = 64, use_graph: bool = True, policy: str = "fifo",
-                 depth: int = 3, concurrent: bool = True, idle_direct: bool = True):
+                 depth: int = 3, concurrent: bool = True, idle_direct: bool = True,
+                 direct_max: int = 4):
         if not grids:
             raise ValueError("RenderServer needs at least one grid")
         self.grids = list(grids)
@@ -257,7 +261,8 @@ class RenderServer:
                 self._grid_structs[i] = structs[-1]
             grp.grids_dev = _C.pack_structs(structs).to(self.device)
         self.concurrent = bool(concurrent)
-        self.idle_direct = bool(idle_direct)
+        self.direct_max = int(direct_max) if idle_direct else 0
+        self.direct_stream = torch.cuda.Stream(device=self.device)  # default priority
         shared = None if self.concurrent else torch.cuda.Stream(device=self.device)
         self.slots = [_Slot(self.device, shared) for _ in range(self.depth)]
         for slot in self.slots:
@@ -367,12 +372,16 @@ class RenderServer:
         # disjoint, and camera slots and images are reused only after the
         # event of the step that last read them.  The slot stream follows the
         # caller's stream, where the grids were written.
-        stream = slot.stream
+        # A short queue of fresh requests goes out as direct image launches,
+        # in arrival order on one stream (first come, first served, as one
+        # launch per request); a longer queue is batched into a tile step.
+        direct = len(pending) <= self.direct_max and all(r.next_pixel == 0 for r in pending)
+        stream = self.direct_stream if direct else slot.stream
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
-            if self.idle_direct and not self._inflight and len(pending) == 1 and \
-                    pending[0].next_pixel == 0:
-                self._launch_direct(pending[0], stream, finishing)
+            if direct:
+                for r in pending:
+                    self._launch_direct(r, stream, finishing)
             else:
                 self._fill_and_launch(slot, pending, finishing)
         slot.finishing = finishing
@@ -385,9 +394,9 @@ class RenderServer:
         return done
 
     def _launch_direct(self, req: RenderRequest, stream, finishing: List[RenderRequest]) -> None:
-        """A lone request on an idle device: its whole image in one launch of
-        the image kernel (the same per-pixel march as the tiles kernel), with
-        no tile table to plan or upload."""
+        """A request of a short queue: its whole image in one launch of the
+        image kernel (the same per-pixel march as the tiles kernel), with no
+        tile table to plan or upload."""
         _C._call("svox2_volume_render_cuvol_image", ctypes.byref(self._grid_structs[req.grid]),
                  ctypes.byref(req.cam_struct), ctypes.byref(self._opt_struct),
                  ctypes.c_void_p(req.image.data_ptr()), None,
