@@ -13,17 +13,17 @@
 // with ignore_edge, which also skips cells whose link is 0; ignore_last_z
 // skips cells with z = size2 - 2.
 //
-// Layout of the work.  A wavefront takes 64 listed cells at a time, one lane
+// Layout of the work.  A wavefront reads 64 listed cells at a time, one lane
 // per cell: it splits the flat index with invariant-divisor multiplies (no
 // integer division), reads the cell's four links and decides whether any of
 // them addresses a row.  At about 8 % occupancy most cells address none and
-// stop there.  A ballot compacts the live cells, and the wave then processes
-// them two at a time, one 32-lane half per cell and lane = channel: the
-// half's four data reads are each one coalesced run of a row, and each of its
-// four gradient updates is one atomic instruction into one row, so the
-// interleaved 128-byte voxel rows are touched once per cell and neighbour.
-// One-channel data (density) skips the compaction: each lane finishes its
-// own cell.
+// stop there.  With several channels a ballot lists the live cells in LDS,
+// and the workgroup's half-waves process them, one 32-lane half per cell and
+// lane = channel: the half's four data reads are each one coalesced run of a
+// row, and each of its four gradient updates is one atomic instruction into
+// one row, so the interleaved 128-byte voxel rows are touched once per cell
+// and neighbour.  One-channel data (density) skips the list: each lane
+// finishes its own cell.
 #include "common.hpp"
 #include "fastdiv.hpp"
 
@@ -120,6 +120,8 @@ __device__ __forceinline__ void cell_terms(const TvJob<DT>& J, const CellLinks& 
     }
 }
 
+// One channel (density): a wavefront takes 64 listed cells at a time and
+// each lane finishes its own cell.
 template <typename DT>
 __global__ __launch_bounds__(kBlock) void k_tv_cells(TvJob<DT> J) {
     const int lane = threadIdx.x & 63;
@@ -127,46 +129,51 @@ __global__ __launch_bounds__(kBlock) void k_tv_cells(TvJob<DT> J) {
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * kBlock) >> 6;
     for (int64_t base = wave * 64; base < J.n_cells; base += nwaves * 64) {
         const CellLinks c = cell_links(J, base + lane);
-        if (J.nch == 1) {  // one channel: the lane finishes its cell
-            if (c.live) {
-                float t[4];
-                bool act[4];
-                const int ch = J.start_dim;
-                cell_terms(J, c, ch, t, act);
-                const int ls[4] = {c.l000, c.l001, c.l010, c.l100};
+        if (!c.live) continue;
+        float t[4];
+        bool act[4];
+        const int ch = J.start_dim;
+        cell_terms(J, c, ch, t, act);
+        const int ls[4] = {c.l000, c.l001, c.l010, c.l100};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (act[i]) {
-                        atomic_add_f32(J.grad + ls[i] * J.gst + ch, t[i]);
-                        if (J.mask) J.mask[ls[i]] = 1;
-                    }
-                }
+        for (int i = 0; i < 4; ++i) {
+            if (act[i]) {
+                atomic_add_f32(J.grad + ls[i] * J.gst + ch, t[i]);
+                if (J.mask) J.mask[ls[i]] = 1;
             }
-            continue;
         }
-        uint64_t m = __ballot(c.live);
-        const int half = lane >> 5, hl = lane & 31;
-        while (m != 0) {  // uniform: two live cells per pass, one per half
-            const int j0 = __builtin_ctzll(m);
-            m &= m - 1;
-            int j1 = -1;
-            if (m != 0) {
-                j1 = __builtin_ctzll(m);
-                m &= m - 1;
-            }
-            const int j = half ? j1 : j0;
-            CellLinks h;
-            const int src0 = j0, src1 = j1 >= 0 ? j1 : j0;
-            h.l000 = half ? __builtin_amdgcn_readlane(c.l000, src1)
-                          : __builtin_amdgcn_readlane(c.l000, src0);
-            h.l001 = half ? __builtin_amdgcn_readlane(c.l001, src1)
-                          : __builtin_amdgcn_readlane(c.l001, src0);
-            h.l010 = half ? __builtin_amdgcn_readlane(c.l010, src1)
-                          : __builtin_amdgcn_readlane(c.l010, src0);
-            h.l100 = half ? __builtin_amdgcn_readlane(c.l100, src1)
-                          : __builtin_amdgcn_readlane(c.l100, src0);
-            if (j < 0) continue;
-            const int ls[4] = {h.l000, h.l001, h.l010, h.l100};
+    }
+}
+
+// Several channels (SH): a workgroup takes kTvWindow listed cells.  Its first
+// wavefront reads their links and lists the live ones in LDS (ballot
+// compaction); then the workgroup's 8 half-waves take the listed cells in
+// turn, lane = channel.  Live cells come in runs (a contiguous run of cells
+// crossing the occupied region), and spreading a run over 8 half-waves
+// shortens the longest chain of dependent row reads and atomics.
+constexpr int kTvWindow = 64;
+
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_tv_cells_multi(TvJob<DT> J) {
+    __shared__ int4 live[kTvWindow];
+    __shared__ int n_live;
+    const int hw = threadIdx.x >> 5, hl = threadIdx.x & 31;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTvWindow; base < J.n_cells;
+         base += static_cast<int64_t>(gridDim.x) * kTvWindow) {
+        if (threadIdx.x < 64) {
+            const CellLinks c = cell_links(J, base + threadIdx.x);
+            const uint64_t m = __ballot(c.live);
+            if (c.live)
+                live[__popcll(m & ((uint64_t{1} << threadIdx.x) - 1))] =
+                    make_int4(c.l000, c.l001, c.l010, c.l100);
+            if (threadIdx.x == 0) n_live = static_cast<int>(__popcll(m));
+        }
+        __syncthreads();
+        const int total = n_live;
+        for (int j = hw; j < total; j += kBlock / 32) {
+            const int4 L = live[j];
+            const CellLinks h{L.x, L.y, L.z, L.w, true};
+            const int ls[4] = {L.x, L.y, L.z, L.w};
             bool any[4] = {false, false, false, false};
             for (int k = hl; k < J.nch; k += 32) {
                 const int ch = J.start_dim + k;
@@ -184,21 +191,28 @@ __global__ __launch_bounds__(kBlock) void k_tv_cells(TvJob<DT> J) {
             if (J.mask) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    // one store per half and row that received a gradient
+                    // one store per half-wave and row that received a gradient
                     const uint64_t b = __ballot(any[i]);
-                    const uint32_t mine = static_cast<uint32_t>(b >> (32 * half));
+                    const uint32_t mine = static_cast<uint32_t>(b >> (32 * (hw & 1)));
                     if (mine != 0 && hl == __builtin_ctz(mine)) J.mask[ls[i]] = 1;
                 }
             }
         }
+        __syncthreads();  // the list is rewritten by the next window
     }
 }
 
 template <typename DT>
 int launch(const TvJob<DT>& J, hipStream_t s) {
-    const int64_t windows = (J.n_cells + 63) / 64;
-    const int64_t blocks = std::min<int64_t>((windows + 3) / 4, 256 * 8);
-    k_tv_cells<DT><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
+    if (J.nch == 1) {
+        const int64_t windows = (J.n_cells + 63) / 64;
+        const int64_t blocks = std::min<int64_t>((windows + 3) / 4, 256 * 8);
+        k_tv_cells<DT><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
+    } else {
+        const int64_t windows = (J.n_cells + kTvWindow - 1) / kTvWindow;
+        const int64_t blocks = std::min<int64_t>(windows, 256 * 32);
+        k_tv_cells_multi<DT><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
+    }
     return check_launch("tv_grad_sparse");
 }
 

@@ -142,8 +142,11 @@ template <typename DT>
 int launch(const RowJob<DT>& J, bool sgd, hipStream_t s) {
     const int64_t ncand = J.kind == SVOX2_INDEXER_INDEX ? J.n_index : J.rows;
     const int64_t windows = (ncand + 63) / 64;
-    // Up to 8 workgroups (32 waves) per CU; waves then loop over windows.
-    const int64_t blocks = std::min<int64_t>((windows + 3) / 4, 256 * 8);
+    // A wave per window of 64 candidates, up to 32 workgroups (128 waves)
+    // per CU; waves then loop over windows.  Selected rows are updated two
+    // at a time, so fewer windows per wave shorten each wave's chain of row
+    // reads and writes.
+    const int64_t blocks = std::min<int64_t>((windows + 3) / 4, 256 * 32);
     if (blocks == 0) return ok();
     if (sgd)
         k_update_rows<DT, true><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
