@@ -331,6 +331,26 @@ int svox2_tv_grad_sparse_run(const int32_t* links, const int32_t size[3],
                              int32_t end_dim, float scale, int32_t ignore_edge,
                              int32_t ignore_last_z, float* grad_data, void* stream);
 
+/* Lumisphere total variation of the SH colour: for each of the n_cells flat
+ * cell indices (split by the extents size - 1) the colours A = sum_k
+ * sh[c * basis_dim + k] * basis_fn[k] of the cell and its +x/+y/+z neighbours
+ * and U = the same with basis_fn_u (basis_dim device floats each) give
+ * per-channel differences (axis factor size / 256, dir_factor for U - A), whose
+ * normalised gradient, scaled by scale / n_cells, is accumulated into
+ * grads->grad_sh_out (grads->grad_sh_stride apart); grads->mask_out (NULL
+ * allowed) marks every row that receives a term.  Only the first 32
+ * coefficients of a row take part, as in the reference; a cell index whose x
+ * part is >= size0 - 1 is skipped (the reference would read past the links).
+ * ndc_coeffx / ndc_coeffy are accepted and unused, as in the reference.
+ * Replaces lumisphere_tv_grad_sparse (svox2.cpp:66, bound at :101;
+ * loss_kernel.cu:643). */
+int svox2_lumisphere_tv_grad_sparse(const svox2_grid_t* grid,
+                                    const int32_t* rand_cells, int64_t n_cells,
+                                    const float* basis_fn, const float* basis_fn_u,
+                                    float scale, float ndc_coeffx, float ndc_coeffy,
+                                    float dir_factor, const svox2_grads_t* grads,
+                                    void* stream);
+
 /* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
  * rms and grad; grad is zeroed where updated.  Row strides (elements, 0 =
  * dim) apply to data, rms and grad separately.  indexer_kind selects every row,

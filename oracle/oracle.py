@@ -21,7 +21,8 @@ reference lines it follows:
   renders over the same sample list.
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
-  (loss_kernel.cu:178-246,558-604), ``rmsprop_step`` (optim_kernel.cu:15-93),
+  (loss_kernel.cu:178-246,558-604), ``lumisphere_tv_grad_sparse``
+  (loss_kernel.cu:336-446,643-678), ``rmsprop_step`` (optim_kernel.cu:15-93),
   ``sgd_step`` (optim_kernel.cu:97-147), ``accel_dist_prop``
   (misc_kernel.cu:109-181,356-416).
 
@@ -791,6 +792,72 @@ def tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray, 
         act = (lk >= 0)[:, None] & (val != 0.0)
         rows = np.broadcast_to(lk[:, None], act.shape)[act]
         cols = np.broadcast_to(ch[None, :], act.shape)[act]
+        np.add.at(grad, (rows, cols), (val * idelta)[act])
+        mask[rows] = True
+    return grad, mask
+
+
+def lumisphere_tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray,
+                              basis_dim: int, sv, svu, scale: float, dir_factor: float):
+    """Gradient and touched-row mask of the lumisphere TV term
+    (loss_kernel.cu:336-446 with the launcher's scale / n_cells, :671), in
+    float64 over all listed cells at once.  The flat index splits by the
+    extents size - 1 (:360-365); a cell whose link is 0 is skipped (:378); an
+    index whose x part is >= size0 - 1 is skipped too (the reference reads
+    past the links array there); a missing v000 reads 0 and a missing
+    neighbour v000 (:385-392); only the first 32 coefficients of a row take
+    part, one per lane of the reference's warp (:351), and a channel's colour
+    sums the coefficients it has among them (:402-411); a term is added (and
+    its row marked) where its link is >= 0 and its value is nonzero
+    (:433-445)."""
+    data = np.asarray(data, np.float64)
+    R, D = data.shape
+    grad = np.zeros_like(data)
+    mask = np.zeros(R, bool)
+    s0, s1, s2 = links.shape
+    cells = np.asarray(rand_cells, np.int64)
+    n = cells.size
+    if n == 0 or min(s0, s1, s2) < 2:
+        return grad, mask
+    scale_c = scale / n
+    z = cells % (s2 - 1)
+    xy = cells // (s2 - 1)
+    y, x = xy % (s1 - 1), xy // (s1 - 1)
+    keep = (cells >= 0) & (x < s0 - 1)
+    base = ((x * s1 + y) * s2 + z)[keep]
+    flat = links.reshape(-1).astype(np.int64)
+    l000 = flat[base]
+    base, l000 = base[l000 != 0], l000[l000 != 0]
+    l001, l010, l100 = flat[base + 1], flat[base + s2], flat[base + s1 * s2]
+    dim = min(D, 32)
+    ent = np.arange(dim)
+    ch = ent // basis_dim
+    sv_e = np.asarray(sv, np.float64)[:basis_dim][ent % basis_dim]
+    svu_e = np.asarray(svu, np.float64)[:basis_dim][ent % basis_dim]
+    onehot = (ch[:, None] == np.arange(ch.max() + 1)[None, :]).astype(np.float64)
+
+    def vals(lk, null):
+        v = data[np.maximum(lk, 0)][:, :dim]
+        return np.where((lk >= 0)[:, None], v, null)
+
+    def colour(v, w):  # per-channel sums, broadcast back to the coefficients
+        return ((v * w) @ onehot)[:, ch]
+
+    v000 = vals(l000, 0.0)
+    v001, v010, v100 = vals(l001, v000), vals(l010, v000), vals(l100, v000)
+    a000 = colour(v000, sv_e)
+    f0, f1, f2 = s0 / 256.0, s1 / 256.0, s2 / 256.0
+    dx = (colour(v100, sv_e) - a000) * f0
+    dy = (colour(v010, sv_e) - a000) * f1
+    dz = (colour(v001, sv_e) - a000) * f2
+    du = (colour(v000, svu_e) - a000) * dir_factor
+    idelta = scale_c / np.sqrt(1e-9 + dx * dx + dy * dy + dz * dz + du * du)
+    dx, dy, dz, du = dx * f0, dy * f1, dz * f2, du * dir_factor
+    sm = -dx * sv_e - dy * sv_e - dz * sv_e + du * (svu_e - sv_e)
+    for lk, val in ((l000, sm), (l001, dz * sv_e), (l010, dy * sv_e), (l100, dx * sv_e)):
+        act = (lk >= 0)[:, None] & (val != 0.0)
+        rows = np.broadcast_to(lk[:, None], act.shape)[act]
+        cols = np.broadcast_to(ent[None, :], act.shape)[act]
         np.add.at(grad, (rows, cols), (val * idelta)[act])
         mask[rows] = True
     return grad, mask

@@ -51,7 +51,8 @@ EXPORTED_SYMBOLS = (
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_volume_render_expected_term",
     "svox2_volume_render_sigma_thresh", "svox2_sample_grid", "svox2_sample_grid_backward",
-    "svox2_tv_grad_sparse", "svox2_tv_grad_sparse_run", "svox2_rmsprop_step", "svox2_sgd_step",
+    "svox2_tv_grad_sparse", "svox2_tv_grad_sparse_run", "svox2_lumisphere_tv_grad_sparse",
+    "svox2_rmsprop_step", "svox2_sgd_step",
     "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_resample_density", "svox2_resample_rows",
     "svox2_grid_weight_render", "svox2_threshold_mask", "svox2_dilate",
@@ -161,6 +162,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
                                  vp, i32, i32, f32, i32, f32, i32, i32, f32, f32, vp, vp],
         "svox2_tv_grad_sparse_run": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, i64,
                                      i64, vp, i32, i32, f32, i32, i32, vp, vp],
+        "svox2_lumisphere_tv_grad_sparse": [G, vp, i64, vp, vp, f32, f32, f32, f32, Gr, vp],
         "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
                                f32, f32, f32, vp],
         "svox2_sgd_step": [vp, i32, vp, i64, i32, i64, i64, i32, vp, i64, f32, f32, vp],
@@ -764,6 +766,34 @@ def tv_grad_sparse_run(links: torch.Tensor, data: torch.Tensor, first_cell: int,
               int(first_cell), int(n_cells), mask_ptr, int(start_dim), int(end_dim),
               float(scale), int(bool(ignore_edge)), int(bool(ignore_last_z)), _ptr(grad_data),
               _stream(data))
+
+
+def lumisphere_tv_grad_sparse(grid: SparseGridSpec, rand_cells: torch.Tensor,
+                              basis_fn: torch.Tensor, basis_fn_u: torch.Tensor, scale: float,
+                              ndc_coeffx: float, ndc_coeffy: float, dir_factor: float,
+                              grads: GridOutputGrads) -> None:
+    """lumisphere_tv_grad_sparse (loss_kernel.cu:643-678): the lumisphere TV
+    gradient of grid.sh_data at the listed cells into grads.grad_sh_out (and
+    grads.mask_out).  basis_fn / basis_fn_u hold at least basis_dim float32
+    values on the grid's device (the kernel reads the first basis_dim)."""
+    g = _grid_struct(grid, need_sh=True, need_map=False)
+    _check(rand_cells, "rand_cells", (torch.int32,))
+    for name, t in (("basis_fn", basis_fn), ("basis_fn_u", basis_fn_u)):
+        _check(t, name, (torch.float32,))
+        if t.numel() < g.basis_dim:
+            raise RuntimeError(f"{name} must hold basis_dim = {g.basis_dim} values")
+        if t.device != grid.sh_data.device:
+            raise RuntimeError(f"{name} must be on the grid's device")
+    if basis_fn.dim() != 1:
+        raise RuntimeError("basis_fn must have 1 dim")
+    if grads.grad_sh_out is None:
+        raise RuntimeError("grads.grad_sh_out must be defined")
+    gr = _grads_struct(grads, g.rows, g.sh_data_dim)
+    with _on_device(grid.sh_data):
+        _call("svox2_lumisphere_tv_grad_sparse", ctypes.byref(g), _ptr(rand_cells),
+              rand_cells.numel(), _ptr(basis_fn), _ptr(basis_fn_u), float(scale),
+              float(ndc_coeffx), float(ndc_coeffy), float(dir_factor), ctypes.byref(gr),
+              _stream(grid.sh_data))
 
 
 def _indexer_args(indexer: torch.Tensor, rows: int):

@@ -328,3 +328,240 @@ extern "C" int svox2_tv_grad_sparse_run(const int32_t* links, const int32_t size
                     nullptr, n_cells, first_cell, mask_out, start_dim, end_dim, scale,
                     ignore_edge, ignore_last_z, grad_data, stream);
 }
+
+// ---------------------------------------------------------------------------
+// Lumisphere total variation (lumisphere_tv_grad_sparse_kernel,
+// svox2/csrc/loss_kernel.cu:336-446, and its launcher :643-678).  The SH
+// coefficients of a cell are turned into one colour per channel by the basis
+// values of a direction, A(v) = sum_k v[c B + k] sv[k], and of a perturbed
+// direction, U(v) = sum_k v[c B + k] svu[k]; per listed cell and channel
+//   dx, dy, dz = (A(v100 / v010 / v001) - A(v000)) * size_axis / 256,
+//   du = (U(v000) - A(v000)) * dir_factor,
+//   idelta = (scale / n_cells) / sqrt(1e-9 + dx^2 + dy^2 + dz^2 + du^2),
+// and coefficient k of channel c receives
+//   v000: (-dx sv - dy sv - dz sv + du (svu - sv)) s idelta, v001: dz sv s idelta,
+//   v010: dy sv s idelta, v100: dx sv s idelta   (s = the same axis factor again),
+// the gradient of sqrt(1e-9 + dx^2 + dy^2 + dz^2 + du^2) scaled by scale / n_cells.
+// The reference's rules are kept: the flat cell index is split by the
+// (size - 1) extents; a cell whose link is 0 is skipped; a missing v000 reads
+// 0 and a missing neighbour reads v000; a term is added (and its row marked)
+// where its link addresses a row and its value is nonzero; only the first 32
+// coefficients of a row take part (the reference gives each cell one 32-lane
+// warp, one lane per coefficient), so a channel cut by that limit sums the
+// coefficients it has.  One rule is added: a flat index whose x part falls on
+// or past the last slab (x >= size0 - 1; the reference's cell list spans all
+// size0 * size1 * size2 indices) would read past the links array in the
+// reference and is skipped here.
+//
+// Work layout as for the several-channel TV above: wave 0 of a workgroup reads
+// 64 listed cells' links and lists the live ones in LDS; the workgroup's 8
+// half-waves take them in turn, lane = coefficient.  The per-channel sums are
+// segmented inclusive scans over the half-wave (5 shuffle steps) read back at
+// the channel's last lane.
+namespace svox2amd {
+namespace {
+
+constexpr int kLumiLanes = 32;
+
+template <typename DT>
+struct LumiJob {
+    const int32_t* links;
+    int s0, s1, s2;
+    FastDiv div_z;  // by s2 - 1
+    FastDiv div_y;  // by s1 - 1
+    const DT* data;
+    int rows;
+    int64_t dst, gst;
+    const int32_t* cells;
+    int64_t n_cells;
+    int dim;    // coefficients taking part: min(sh_data_dim, 32)
+    int basis;  // basis_dim
+    const float* sv;
+    const float* svu;
+    float scale;  // already divided by n_cells
+    float sc0, sc1, sc2, su;
+    uint8_t* mask;
+    float* grad;
+};
+
+template <typename DT>
+__device__ __forceinline__ bool lumi_row(const LumiJob<DT>& J, int l) {
+    return static_cast<uint32_t>(l) < static_cast<uint32_t>(J.rows);
+}
+
+template <typename DT>
+__device__ __forceinline__ CellLinks lumi_cell(const LumiJob<DT>& J, int64_t k) {
+    CellLinks c{-1, -1, -1, -1, false};
+    if (k >= J.n_cells) return c;
+    const int xyz = J.cells[k];
+    if (xyz < 0) return c;
+    uint32_t z, y;
+    const uint32_t xy = J.div_z.divmod(static_cast<uint32_t>(xyz), z);
+    const uint32_t x = J.div_y.divmod(xy, y);
+    if (x >= static_cast<uint32_t>(J.s0 - 1)) return c;
+    const int32_t* lp = J.links + (static_cast<int64_t>(x) * J.s1 + y) * J.s2 + z;
+    c.l000 = lp[0];
+    if (c.l000 == 0) return c;
+    c.l001 = lp[1];
+    c.l010 = lp[J.s2];
+    c.l100 = lp[static_cast<int64_t>(J.s1) * J.s2];
+    c.live = lumi_row(J, c.l000) || lumi_row(J, c.l001) || lumi_row(J, c.l010) ||
+             lumi_row(J, c.l100);
+    return c;
+}
+
+// Sum of v over the lanes [c0, last] of this lane's 32-lane half (c0 <= hl <=
+// last): a segmented Hillis-Steele scan, then the segment's last lane.
+__device__ __forceinline__ float channel_sum(float v, int hl, int c0, int last) {
+#pragma unroll
+    for (int d = 1; d < kLumiLanes; d <<= 1) {
+        const float o = __shfl_up(v, d, kLumiLanes);
+        if (hl - d >= c0) v += o;
+    }
+    return __shfl(v, last, kLumiLanes);
+}
+
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_tv_lumisphere(LumiJob<DT> J) {
+    __shared__ int4 live[kTvWindow];
+    __shared__ int n_live;
+    const int hw = threadIdx.x >> 5, hl = threadIdx.x & 31;
+    // This lane's coefficient, its channel's lanes and basis values.
+    const bool mine = hl < J.dim;
+    const int c0 = (hl / J.basis) * J.basis;
+    const int last = min(c0 + J.basis, J.dim) - 1;
+    const int kb = hl - c0;
+    const float sv = mine ? J.sv[kb] : 0.f;
+    const float svu = mine ? J.svu[kb] : 0.f;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTvWindow; base < J.n_cells;
+         base += static_cast<int64_t>(gridDim.x) * kTvWindow) {
+        if (threadIdx.x < 64) {
+            const CellLinks c = lumi_cell(J, base + threadIdx.x);
+            const uint64_t m = __ballot(c.live);
+            if (c.live)
+                live[__popcll(m & ((uint64_t{1} << threadIdx.x) - 1))] =
+                    make_int4(c.l000, c.l001, c.l010, c.l100);
+            if (threadIdx.x == 0) n_live = static_cast<int>(__popcll(m));
+        }
+        __syncthreads();
+        const int total = n_live;
+        for (int j = hw; j < total; j += kBlock / 32) {
+            const int4 L = live[j];
+            const int ls[4] = {L.x, L.y, L.z, L.w};
+            const bool ok000 = mine && lumi_row(J, L.x);
+            const float v000 = ok000 ? ld(J.data, L.x * J.dst + hl) : 0.f;
+            const float v001 = mine && lumi_row(J, L.y) ? ld(J.data, L.y * J.dst + hl) : v000;
+            const float v010 = mine && lumi_row(J, L.z) ? ld(J.data, L.z * J.dst + hl) : v000;
+            const float v100 = mine && lumi_row(J, L.w) ? ld(J.data, L.w * J.dst + hl) : v000;
+            // every lane of the half takes part in the scans (idle lanes add 0)
+            const float a000 = channel_sum(v000 * sv, hl, c0, last);
+            const float a001 = channel_sum(v001 * sv, hl, c0, last);
+            const float a010 = channel_sum(v010 * sv, hl, c0, last);
+            const float a100 = channel_sum(v100 * sv, hl, c0, last);
+            const float u000 = channel_sum(v000 * svu, hl, c0, last);
+            float dx = (a100 - a000) * J.sc0;
+            float dy = (a010 - a000) * J.sc1;
+            float dz = (a001 - a000) * J.sc2;
+            float du = (u000 - a000) * J.su;
+            const float idelta = J.scale * rsqrtf(1e-9f + dx * dx + dy * dy + dz * dz + du * du);
+            dx *= J.sc0;
+            dy *= J.sc1;
+            dz *= J.sc2;
+            du *= J.su;
+            const float val[4] = {-dx * sv - dy * sv - dz * sv + du * (svu - sv), dz * sv,
+                                  dy * sv, dx * sv};
+            bool any[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                any[i] = mine && lumi_row(J, ls[i]) && val[i] != 0.f;
+                if (any[i]) atomic_add_f32(J.grad + ls[i] * J.gst + hl, val[i] * idelta);
+            }
+            if (J.mask) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint64_t b = __ballot(any[i]);
+                    const uint32_t half = static_cast<uint32_t>(b >> (32 * (hw & 1)));
+                    if (half != 0 && hl == __builtin_ctz(half)) J.mask[ls[i]] = 1;
+                }
+            }
+        }
+        __syncthreads();  // the list is rewritten by the next window
+    }
+}
+
+template <typename DT>
+int run_lumisphere(const svox2_grid_t& g, const DT* data, const int32_t* cells, int64_t n_cells,
+                   const float* sv, const float* svu, float scale, float dir_factor,
+                   const svox2_grads_t& gr, hipStream_t s) {
+    LumiJob<DT> J;
+    J.links = g.links;
+    J.s0 = g.size[0];
+    J.s1 = g.size[1];
+    J.s2 = g.size[2];
+    J.div_z = FastDiv(static_cast<uint32_t>(g.size[2] - 1));
+    J.div_y = FastDiv(static_cast<uint32_t>(g.size[1] - 1));
+    J.data = data;
+    J.rows = static_cast<int>(g.rows);
+    J.dst = g.sh_stride > 0 ? g.sh_stride : g.sh_data_dim;
+    J.gst = gr.grad_sh_stride > 0 ? gr.grad_sh_stride : g.sh_data_dim;
+    J.cells = cells;
+    J.n_cells = n_cells;
+    J.dim = std::min(g.sh_data_dim, kLumiLanes);
+    J.basis = g.basis_dim;
+    J.sv = sv;
+    J.svu = svu;
+    // The reference divides the scale by the number of cells (loss_kernel.cu:671).
+    J.scale = scale / static_cast<float>(n_cells);
+    J.sc0 = static_cast<float>(g.size[0]) * (1.f / 256.f);
+    J.sc1 = static_cast<float>(g.size[1]) * (1.f / 256.f);
+    J.sc2 = static_cast<float>(g.size[2]) * (1.f / 256.f);
+    J.su = dir_factor;
+    J.mask = gr.mask_out;
+    J.grad = gr.grad_sh_out;
+    const int64_t windows = (n_cells + kTvWindow - 1) / kTvWindow;
+    const int64_t blocks = std::min<int64_t>(windows, 256 * 32);
+    k_tv_lumisphere<DT><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(J);
+    return check_launch("lumisphere_tv_grad_sparse");
+}
+
+}  // namespace
+}  // namespace svox2amd
+
+extern "C" int svox2_lumisphere_tv_grad_sparse(const svox2_grid_t* grid,
+                                               const int32_t* rand_cells, int64_t n_cells,
+                                               const float* basis_fn, const float* basis_fn_u,
+                                               float scale, float ndc_coeffx, float ndc_coeffy,
+                                               float dir_factor, const svox2_grads_t* grads,
+                                               void* stream) {
+    (void)ndc_coeffx;  // unused by the reference's axis factors (loss_kernel.cu:21-54)
+    (void)ndc_coeffy;
+    SVOX2_REQUIRE(grid != nullptr && grads != nullptr, "grid / grads are NULL");
+    const svox2_grid_t& g = *grid;
+    SVOX2_REQUIRE(g.links != nullptr, "links is NULL");
+    for (int i = 0; i < 3; ++i) SVOX2_REQUIRE(g.size[i] >= 1, "grid size[%d] = %d", i, g.size[i]);
+    SVOX2_REQUIRE(static_cast<int64_t>(g.size[0]) * g.size[1] * g.size[2] < INT32_MAX,
+                  "too many cells");
+    SVOX2_REQUIRE(g.storage == SVOX2_STORAGE_F32 || g.storage == SVOX2_STORAGE_BF16,
+                  "unknown storage kind %d", g.storage);
+    SVOX2_REQUIRE(g.rows >= 0 && g.rows < INT32_MAX, "rows out of range");
+    SVOX2_REQUIRE(g.basis_dim >= 1 && g.sh_data_dim >= g.basis_dim,
+                  "basis_dim %d invalid for sh_data_dim %d", g.basis_dim, g.sh_data_dim);
+    SVOX2_REQUIRE(n_cells >= 0, "negative cell count");
+    SVOX2_REQUIRE(g.sh_stride >= 0 && grads->grad_sh_stride >= 0, "negative row stride");
+    SVOX2_REQUIRE(g.sh_stride == 0 || g.sh_stride >= g.sh_data_dim, "sh row stride too small");
+    SVOX2_REQUIRE(grads->grad_sh_stride == 0 || grads->grad_sh_stride >= g.sh_data_dim,
+                  "grad_sh row stride too small");
+    // A unit extent leaves no cells between grid points.
+    if (n_cells == 0 || g.rows == 0 || g.size[0] < 2 || g.size[1] < 2 || g.size[2] < 2)
+        return ok();
+    SVOX2_REQUIRE(rand_cells != nullptr, "rand_cells is NULL");
+    SVOX2_REQUIRE(g.sh_data != nullptr && grads->grad_sh_out != nullptr,
+                  "sh_data / grad_sh_out are NULL");
+    SVOX2_REQUIRE(basis_fn != nullptr && basis_fn_u != nullptr, "basis values are NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (g.storage == SVOX2_STORAGE_BF16)
+        return run_lumisphere(g, static_cast<const uint16_t*>(g.sh_data), rand_cells, n_cells,
+                              basis_fn, basis_fn_u, scale, dir_factor, *grads, s);
+    return run_lumisphere(g, static_cast<const float*>(g.sh_data), rand_cells, n_cells,
+                          basis_fn, basis_fn_u, scale, dir_factor, *grads, s);
+}

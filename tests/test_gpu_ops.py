@@ -124,6 +124,112 @@ def test_tv_grad_sparse_run_matches_cell_list(gpu, small_sparse, dtype):
                               torch.zeros(R, 1, device=gpu))
 
 
+def _spec(links, sh, basis_dim):
+    spec = _C.SparseGridSpec()
+    spec.links = links
+    spec.sh_data = sh
+    spec.density_data = torch.zeros(sh.size(0), 1, dtype=sh.dtype, device=sh.device)
+    spec._offset = torch.zeros(3)
+    spec._scaling = torch.ones(3)
+    spec.basis_dim = basis_dim
+    return spec
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_lumisphere_tv_grad_sparse(gpu, small_sparse, dtype):
+    """lumisphere_tv_grad_sparse against the oracle: the blob grid's SH rows
+    (basis 9) with every flat index listed (those past the size - 1 extents
+    included) and a random list with repeats; an odd random grid with basis
+    1, 4, 16 (48 coefficients, the third channel past the 32nd) and 25 (the
+    second channel cut at the 32nd), in interleaved rows of a wider buffer;
+    dir_factor 0 and scale 0 (rows still marked where a term is nonzero).
+    Masks bit-exact, gradients within 1e-5 of their max (float32 colour sums
+    and atomic order)."""
+    sc, _ = small_sparse
+    rng = np.random.default_rng(31)
+    shape = (19, 13, 10)
+    R = 700
+    odd = rng.integers(-3, R, size=shape).astype(np.int32)
+    odd[rng.random(shape) < 0.4] = -1
+    cases = []
+    for cells in (np.arange(32 ** 3, dtype=np.int32),
+                  rng.integers(0, 32 ** 3, size=5000).astype(np.int32)):
+        cases.append((sc.links.numpy(), sc.sh.numpy(), 9, cells, 0.7, 0.5))
+    odd_cells = rng.integers(0, int(np.prod(shape)), size=3000).astype(np.int32)
+    for B, f, scale in ((1, 0.3, 1.0), (4, 0.0, 2.0), (16, 0.8, 0.5), (25, 1.2, 0.5),
+                        (9, 0.4, 0.0)):
+        cases.append((odd, rng.standard_normal((R, 3 * B)), B, odd_cells, f, scale))
+    for links, data, B, cells, f, scale in cases:
+        D = data.shape[1]
+        host = torch.from_numpy(np.ascontiguousarray(data)).float().to(dtype)
+        buf = torch.zeros(host.size(0), D + 5, dtype=dtype, device=gpu)
+        buf[:, 2:2 + D] = host.to(gpu)
+        gbuf = torch.zeros(host.size(0), D + 3, device=gpu)
+        mask = torch.zeros(host.size(0), dtype=torch.bool, device=gpu)
+        sv = torch.from_numpy(rng.standard_normal(B)).float()
+        svu = torch.from_numpy(rng.standard_normal(B)).float()
+        grads = _C.GridOutputGrads()
+        grads.grad_sh_out = gbuf[:, 1:1 + D]
+        grads.mask_out = mask
+        lk = torch.from_numpy(links).to(gpu)
+        _C.lumisphere_tv_grad_sparse(_spec(lk, buf[:, 2:2 + D], B), torch.from_numpy(cells).to(gpu),
+                                     sv.to(gpu), svu.to(gpu), scale, -1.0, -1.0, f, grads)
+        rg, rm = O.lumisphere_tv_grad_sparse(links, host.double().numpy(), cells, B,
+                                             sv.double().numpy(), svu.double().numpy(), scale, f)
+        g = gbuf.cpu().numpy()
+        tag = (B, D, len(cells), f, scale)
+        assert np.abs(g[:, 1:1 + D] - rg).max() <= 1e-5 * max(np.abs(rg).max(), 1e-30), tag
+        assert not g[:, 0].any() and not g[:, 1 + D:].any(), tag
+        assert np.array_equal(mask.cpu().numpy(), rm) and rm.any(), tag
+        if scale == 0.0:
+            assert not g.any(), tag
+
+
+def test_inplace_tv_lumisphere_grad_draws(gpu, small_sparse):
+    """The facade draws the reference's random numbers in its order (run start
+    from np.random, the direction from torch.randn on the device, one rotation
+    axis per listed cell from torch.randn on the CPU; svox2.py:1847-1884) and
+    adds the oracle's gradient for the first axis's basis values, which are the
+    ones the reference's kernel reads; the mask goes to the SH indexer."""
+    from scipy.spatial.transform import Rotation
+
+    sc, _ = small_sparse
+    grid = scenes.to_sparse_grid(sc, device=gpu, dtype=torch.float32, accelerate=False)
+    grad = grid._get_data_grads()[1]
+    grid.sparse_sh_grad_indexer = torch.zeros(grid.sh_data.size(0), dtype=torch.bool,
+                                              device=gpu)
+    for f in (0.5, 0.0):
+        grad.zero_()
+        grid.sparse_sh_grad_indexer.zero_()
+        np.random.seed(11)  # a run that crosses the blobs
+        torch.manual_seed(6)
+        grid.inplace_tv_lumisphere_grad(grad, scaling=0.8, sparse_frac=0.05, dir_factor=f,
+                                        dir_perturb_radians=0.3)
+        np.random.seed(11)
+        torch.manual_seed(6)
+        n = 32 ** 3
+        count = int(0.05 * n)
+        cells = (np.random.randint(0, n) + np.arange(count)) % n
+        dirs = torch.randn(3, device=gpu)
+        dirs /= torch.norm(dirs)
+        if f > 0:
+            axis = torch.randn((count, 3))
+            axis /= torch.norm(axis, dim=-1, keepdim=True)
+            axis *= 0.3
+            rot = torch.from_numpy(Rotation.from_rotvec(axis.numpy()).as_matrix()).float()
+            dirs_u = (rot.to(gpu) * dirs.unsqueeze(-2)).sum(-1)  # every cell's, as the reference
+        else:
+            dirs_u = dirs[None]
+        sv = O.eval_sh_bases(9, dirs.double().cpu()[None])[0].numpy()
+        svu = O.eval_sh_bases(9, dirs_u.double().cpu())[0].numpy()
+        rg, rm = O.lumisphere_tv_grad_sparse(sc.links.numpy(),
+                                             grid.sh_data.data.double().cpu().numpy(), cells, 9,
+                                             sv, svu, 0.8, f)
+        g = grad.cpu().numpy()
+        assert np.abs(g - rg).max() <= 1e-5 * np.abs(rg).max(), f
+        assert np.array_equal(grid.sparse_sh_grad_indexer.cpu().numpy(), rm) and rm.any(), f
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
 def test_rmsprop_all_mask_index(gpu, dtype):
     gen = torch.Generator().manual_seed(9)

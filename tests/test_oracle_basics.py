@@ -264,6 +264,116 @@ def test_tv_grad_axis_factor():
     assert np.array_equal(g128, 0.5 * g256)
 
 
+def _lumi_energy(links, data, cells, B, sv, svu, scale, f):
+    """Lumisphere TV energy whose coefficient gradient the reference kernel
+    evaluates (loss_kernel.cu:336-446, scale / n_cells from :671): per listed
+    cell and channel, (scale / n) * sqrt(1e-9 + dx^2 + dy^2 + dz^2 + du^2) of
+    the colour differences A(neighbour) - A(v000) times size_axis / 256 and
+    (U(v000) - A(v000)) * f, with A / U the coefficient sums weighted by sv /
+    svu, over the first 32 coefficients; cells split by the extents size - 1,
+    link-0 cells and x >= size0 - 1 skipped, missing v000 = 0, missing
+    neighbour = v000."""
+    s0, s1, s2 = links.shape
+    n = len(cells)
+    dim = min(data.shape[1], 32)
+    e = data.sum() * 0.0
+    zero = torch.zeros(dim, dtype=data.dtype)
+    for xyz in cells:
+        z, y, x = xyz % (s2 - 1), (xyz // (s2 - 1)) % (s1 - 1), xyz // ((s1 - 1) * (s2 - 1))
+        if x >= s0 - 1 or int(links[x, y, z]) == 0:
+            continue
+        l000 = int(links[x, y, z])
+        v000 = data[l000, :dim] if l000 >= 0 else zero
+        nb = [data[int(lk), :dim] if int(lk) >= 0 else v000
+              for lk in (links[x + 1, y, z], links[x, y + 1, z], links[x, y, z + 1])]
+        for c0 in range(0, dim, B):
+            c1 = min(c0 + B, dim)
+            w, wu = sv[:c1 - c0], svu[:c1 - c0]
+            a = (v000[c0:c1] * w).sum()
+            dx = ((nb[0][c0:c1] * w).sum() - a) * (s0 / 256)
+            dy = ((nb[1][c0:c1] * w).sum() - a) * (s1 / 256)
+            dz = ((nb[2][c0:c1] * w).sum() - a) * (s2 / 256)
+            du = ((v000[c0:c1] * wu).sum() - a) * f
+            e = e + (scale / n) * torch.sqrt(1e-9 + dx * dx + dy * dy + dz * dz + du * du)
+    return e
+
+
+def _lumi_case(shape, seed, occ=0.6):
+    """Random links of `shape` (occupied with probability occ, rows numbered
+    in random order so that row 0 sits at some cell) and every flat index
+    0 .. prod(shape) - 1 as the cell list (those past the (size - 1) extents
+    included)."""
+    g = np.random.default_rng(seed)
+    links = np.where(g.random(shape) < occ, 0, -1).astype(np.int32)
+    links[2, 2, 2] = 0
+    idx = np.flatnonzero(links.reshape(-1) >= 0)
+    g.shuffle(idx)
+    links.reshape(-1)[idx] = np.arange(len(idx), dtype=np.int32)
+    zero_at = tuple(np.argwhere(links == 0)[0])
+    links[zero_at], links[2, 2, 2] = links[2, 2, 2], 0  # row 0 at an interior cell
+    return links, len(idx), np.arange(int(np.prod(shape)), dtype=np.int64)
+
+
+@pytest.mark.parametrize("B,D", [(4, 12), (9, 27), (16, 48), (25, 75)])
+def test_lumisphere_grad_is_energy_gradient(B, D):
+    """The lumisphere gradient the oracle forms is the autograd gradient of the
+    lumisphere energy on an odd grid (its normaliser uses the scaled
+    differences, so this holds at any size), including channels cut at the 32nd
+    coefficient (B = 16, 25); its mask is the set of rows with a nonzero
+    gradient; coefficients past the 32nd receive nothing."""
+    links, R, cells = _lumi_case((7, 6, 8), 20 + B)
+    g = np.random.default_rng(B)
+    data = g.normal(size=(R, D))
+    sv, svu = g.normal(size=B), g.normal(size=B)
+    grad, mask = O.lumisphere_tv_grad_sparse(links, data, cells, B, sv, svu, 0.3, 0.7)
+    t = torch.tensor(data, requires_grad=True)
+    _lumi_energy(links, t, cells, B, torch.tensor(sv), torch.tensor(svu), 0.3, 0.7).backward()
+    ref = t.grad.numpy()
+    assert np.abs(grad - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(mask, np.abs(grad).sum(1) > 0)
+    assert mask.sum() > R // 2
+    assert (grad[:, 32:] == 0).all()
+    # the cell at (2, 2, 2) holds row 0 and is skipped (its occupied neighbours
+    # would give terms); the cell below it in z is not, and row 0 receives
+    # its +z neighbour term
+    g0, m0 = O.lumisphere_tv_grad_sparse(links, data, [(2 * 5 + 2) * 7 + 2], B, sv, svu, 1.0,
+                                         0.7)
+    assert not g0.any() and not m0.any()
+    g1, m1 = O.lumisphere_tv_grad_sparse(links, data, [(2 * 5 + 2) * 7 + 1], B, sv, svu, 1.0,
+                                         0.7)
+    assert m1[0] and g1[0, :min(D, 32)].any()
+
+
+def test_lumisphere_reduces_to_tv():
+    """One basis function of value 1 for both directions (so du = 0) makes the
+    lumisphere term the plain TV term of each channel with the ignore_edge rules;
+    a one-hot basis of 9 makes it the plain TV term of coefficients 0, 9 and 18.
+    Both against tv_grad_sparse (itself pinned to a TV energy) on a 256-cube,
+    where the axis factor is 1, with the cell lists mapped between the size - 1
+    and size splits."""
+    links, R, _ = _tv_case(256, 11, 90)
+    pts = [(x, y, z) for x in range(90, 102) for y in range(90, 102) for z in range(90, 102)]
+    pts += [(x, y, z) for x in range(250, 255) for y in range(250, 255) for z in range(250, 255)]
+    lumi_cells = np.array([(x * 255 + y) * 255 + z for x, y, z in pts], np.int64)
+    tv_cells = np.array([(x * 256 + y) * 256 + z for x, y, z in pts], np.int64)
+    g = np.random.default_rng(12)
+    data = g.normal(size=(R, 3))
+    grad, mask = O.lumisphere_tv_grad_sparse(links, data, lumi_cells, 1, [1.0], [1.0], 0.4, 0.9)
+    ref, rmask = O.tv_grad_sparse(links, data, tv_cells, 0, 3, 0.4, True, False)
+    assert np.abs(grad - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.array_equal(mask, rmask) and mask.sum() > 100
+    data27 = g.normal(size=(R, 27))
+    e0 = np.zeros(9)
+    e0[0] = 1.0
+    grad, mask = O.lumisphere_tv_grad_sparse(links, data27, lumi_cells, 9, e0, e0, 0.4, 0.9)
+    ref3, rmask = O.tv_grad_sparse(links, data27[:, [0, 9, 18]], tv_cells, 0, 3, 0.4, True,
+                                   False)
+    assert np.abs(grad[:, [0, 9, 18]] - ref3).max() <= 1e-13 * np.abs(ref3).max()
+    others = np.delete(np.arange(27), [0, 9, 18])
+    assert (grad[:, others] == 0).all()
+    assert np.array_equal(mask, rmask)
+
+
 def test_sgd_closed_form_known_answer():
     """data - lr * g per selected row, lr_last on the last column, the
     gradient zeroed only where applied: a hand-computed case."""
