@@ -236,12 +236,6 @@ __device__ __forceinline__ float group_scan(float v) {
     return v;
 }
 
-// Value of lane - 1 (wave shift right by one; lane 0 of each group must not
-// use it: it reads the previous group's last lane or nothing).
-__device__ __forceinline__ int prev_lane(int v) {
-    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
-}
-
 // Value of lane ^ BIT (BIT = 1, 2 or 4): quad permutes for 1 and 2, a
 // bit-mode swizzle (xor mask 4 within 32 lanes) for 4.
 template <int BIT>
@@ -599,12 +593,9 @@ __device__ __forceinline__ void march_chunk(const GridView<DT>& g, const OptView
 }
 
 
-// Projections of row lk on the SH basis: col[c] = sum_j basis[j] * sh[c * B +
-// j], accumulated in j order.  Rows that are 16-byte aligned with a stride of
-// whole 16-byte units are read with 16-byte loads (one aligned 128-byte line
-// for a float32 SH-2 row of stride 32, 64 bytes in bf16), and each loaded
-// vector is consumed as it arrives (bf16 channels are widened one by one
-// into the products), so the row is never held as C floats.
+// Adds coefficient ch of a row to its channel's projection on the SH basis
+// (c0, c1, c2 for the three channels; coefficient j of a channel is weighted
+// by basis[j]).
 template <int B>
 __device__ __forceinline__ void project_channel(int ch, float x, const float* basis, float& c0,
                                                 float& c1, float& c2) {
@@ -614,6 +605,12 @@ __device__ __forceinline__ void project_channel(int ch, float x, const float* ba
     else c2 = fmaf(basis[j], x, c2);
 }
 
+// Projections of row lk on the SH basis: col[c] = sum_j basis[j] * sh[c * B +
+// j], accumulated in j order.  Rows that are 16-byte aligned with a stride of
+// whole 16-byte units are read with 16-byte loads (one aligned 128-byte line
+// for a float32 SH-2 row of stride 32, 64 bytes in bf16), and each loaded
+// vector is consumed as it arrives (bf16 channels are widened one by one
+// into the products), so the row is never held as C floats.
 template <int B>
 __device__ __forceinline__ void project_row(const GridView<float>& g, int lk, const float* basis,
                                             float& c0, float& c1, float& c2) {
@@ -727,12 +724,15 @@ __device__ __forceinline__ void chunk_run_colors(const GridView<DT>& g, const Ra
     }
 }
 
-// Run heads of a chunk: live samples whose previous lane is not a live
-// sample of the same cell.
+// Run heads of a chunk: live samples whose previous live sample (in lane
+// order) is not in the same cell.  Runs are formed over the live samples
+// alone, so a chunk whose live samples are compacted into lanes 0 .. n - 1
+// (the records) has the same runs, passes and sums.
 __device__ __forceinline__ uint32_t run_heads(const Chunk& c, int lane) {
-    const int prev_cell = prev_lane(c.base);
-    const bool prev_live = lane > 0 && ((c.live_mask >> (lane - 1)) & 1u);
-    return group_ballot(c.live && !(prev_live && prev_cell == c.base));
+    const uint32_t below = c.live_mask & ((1u << lane) - 1u);
+    const int prev = below ? 31 - static_cast<int>(__clz(below)) : 0;
+    const int prev_cell = __shfl(c.base, prev, kGroup);
+    return group_ballot(c.live && !(below != 0u && prev_cell == c.base));
 }
 
 // Colours of a chunk's live samples: each live lane receives its own sample's
@@ -803,7 +803,8 @@ constexpr int kLogWords = 36;
 struct RecStream {
     float4* __restrict__ rec;
     int32_t* __restrict__ state;
-    uint32_t* __restrict__ next_block;  // [0] overflow slots taken so far, [1] long rays listed
+    uint32_t* __restrict__ next_block;  // [0] overflow slots taken, [1] long rays listed,
+                                        // [2] chunk list entries taken
     int32_t* __restrict__ seg;
     int32_t* __restrict__ split;        // per ray: its index in long_list, or -1
     int32_t* __restrict__ long_list;
@@ -811,8 +812,16 @@ struct RecStream {
     int64_t fixed;                      // slots per ray in the fixed region
     int64_t overflow_base;              // first overflow slot
     int64_t overflow_slots;             // slots in the overflow region
+    // Deferred colours: the recorded chunks of rays whose colours are formed
+    // by k_rec_colours, next_block[2] entries taken of clist_cap.
+    int2* __restrict__ clist;           // (ray, header slot), or (-1, -1)
+    float4* __restrict__ chunk_rgb;     // per entry: the chunk's colour
+    float4* __restrict__ ray_info;      // per ray: (first entry, entries, background term)
+    int64_t clist_cap;
 };
 constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
+constexpr int kDeferChunks = 64;  // recorded chunks a ray can list for deferred colours
+constexpr int64_t kDeferCap = 8;  // chunk list entries per ray of the batch
 constexpr int kRecSegChunks = 4;  // recorded chunks per replay segment
 constexpr int kRecSegs = 6;       // segments of a split ray (the last runs to its end)
 constexpr int kRecSplitMin = 2 * kRecSegChunks;  // recorded chunks that make a ray split
@@ -822,9 +831,33 @@ __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int
                                 0.f);
 }
 
+// Adds a chunk's colour to the ray's: the run groups' partial sums (lanes 0,
+// 8, 16 and 24 of cacc) reduced across the group, then added to total in
+// march order (every lane of the group gets the same sums).
+__device__ __forceinline__ void add_chunk_colour(float* total, const float* cacc, float* sum) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v = cacc[a];
+        v += __shfl_xor(v, 8, kGroup);
+        v += __shfl_xor(v, 16, kGroup);
+        sum[a] = group_lane(v, 0);
+        total[a] += sum[a];
+    }
+}
+
 // Forward march of one ray.  out[3] = composited colour (all lanes),
-// returns the final log transmittance.
-template <typename DT, int B, bool STATS, bool RECORD = false, bool LOG = false>
+// returns the final log transmittance.  A chunk's colour sums its live
+// samples' weighted colours (per run group, then across the group); the ray's
+// colour is the sum of its chunks' in march order plus the background.
+//
+// DEFER (recording forward): the chunks' colours are left to k_rec_colours.
+// The samples are recorded with zero colours, each recorded chunk's header
+// slot is staged in slot_lds, and at the ray's end the chunks are listed in
+// rs->clist; k_rec_colours forms the same colours and sums.  A ray whose
+// records or list entries do not fit is marched again with the colours
+// formed here and no records (state 2, re-marched by the backward).
+template <typename DT, int B, bool STATS, bool RECORD = false, bool LOG = false,
+          bool DEFER = false>
 __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptView& o,
                                                const Ray& r, const RayCtx<B>& rc, int lane,
                                                float* out, int32_t* stats,
@@ -832,7 +865,8 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
                                                const RecStream* rs = nullptr,
                                                int64_t ray = 0,
                                                const ChunkLog* log = nullptr,
-                                               float* log_lds = nullptr) {
+                                               float* log_lds = nullptr,
+                                               int32_t* slot_lds = nullptr) {
     bool log_more = false;   // the march continued after the last logged chunk
     float log_next_t = 0.f;  // and where
     // Record stream position of this ray: next free slot and end of block.
@@ -840,120 +874,145 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     int64_t end = RECORD ? cur + rs->fixed : 0;
     bool rec_ok = RECORD;
     int n_rec_chunks = 0;  // chunks whose records were written
-    float acc[3] = {0.f, 0.f, 0.f};  // this lane's share of the composited colour
+    float total[3] = {0.f, 0.f, 0.f};  // colour composited so far
     uint32_t n_iter = 0, n_eval = 0, n_hit = 0;
     uint32_t n_chunks = 0;
     uint32_t n_live_all = 0, n_live_chunks = 0;  // replay work (recording only)
     Fnv fnv;
     float log_t = 0.f;
-    if (r.tmin <= r.tmax) {
-        float t = r.tmin;
-        int guard = 0;
-        bool more = true;
-        while (more && ++guard < kMaxIters) {
-            Chunk c;
-            march_chunk(g, o, r, lane, t, log_t, c);
-            const int n_live = __popc(c.live_mask);
-            if (LOG && n_chunks < static_cast<uint32_t>(kLogChunks)) {
-                // Staged in LDS and written out when the ray ends: global
-                // stores here would hold up the next chunk's loads (a load's
-                // wait also waits for every earlier store).
-                float* rec = log_lds + n_chunks * kLogWords;
-                rec[lane] = c.t;
-                if (lane == 0) {
-                    rec[32] = __int_as_float(c.n_place);
-                    rec[33] = __uint_as_float(c.load_mask);
-                }
-                log_more = c.more;
-                log_next_t = c.next_t;
-            }
-            ++n_chunks;
-            if (RECORD && n_live > 0) {
-                n_live_all += n_live;
-                ++n_live_chunks;
-            }
-            if (n_live > 0) {
-                // Recorded chunk k = n_rec_chunks opens replay segment k / kRecSegChunks:
-                // keep the colour composited so far (spread over lanes 0, 8, 16, 24).
-                const bool seg_start = RECORD && rec_ok && n_rec_chunks > 0 &&
-                                       n_rec_chunks % kRecSegChunks == 0 &&
-                                       n_rec_chunks / kRecSegChunks < kRecSegs;
-                float before[3] = {0.f, 0.f, 0.f};
-                if (seg_start) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        before[a] = acc[a] + __shfl_xor(acc[a], 8, kGroup);
-                        before[a] += __shfl_xor(before[a], 16, kGroup);
-                    }
-                }
-                float cs[3] = {0.f, 0.f, 0.f};
-                chunk_colours<DT, B, RECORD>(g, rc, c, lane, cs, acc);
-                if (RECORD && rec_ok) {
-                    // Header + n_live records; keep one slot for a jump/end header.
-                    if (cur + n_live + 2 > end) {
-                        uint32_t nb = 0;
-                        if (lane == 0) nb = atomicAdd(rs->next_block, static_cast<uint32_t>(kRecBlock));
-                        nb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), 0, kGroup));
-                        if (static_cast<int64_t>(nb) + kRecBlock > rs->overflow_slots) {
-                            rec_ok = false;
-                        } else {
-                            const int64_t nxt = rs->overflow_base + nb;
-                            if (lane == 0) put_header(rs->rec, cur, 0, nxt);
-                            cur = nxt;
-                            end = nxt + kRecBlock;
-                        }
-                    }
-                    if (rec_ok) {
-                        const int k = n_rec_chunks++;
-                        if (seg_start && lane == 0) {
-                            // A segment may start here: its header slot and the
-                            // colour composited before the chunk.
-                            rs->seg[ray * (kRecSegs - 1) + k / kRecSegChunks - 1] =
-                                static_cast<int32_t>(cur);
-                            rs->rec[6 * cur + 1] =
-                                make_float4(before[0], before[1], before[2], 0.f);
-                        }
-                        if (lane == 0) put_header(rs->rec, cur, n_live, 0);
-                        if (c.live) {
-                            const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
-                            float4* q = rs->rec + 6 * (cur + 1 + rank);
-                            q[0] = make_float4(c.p[0], c.p[1], c.p[2], __int_as_float(c.base));
-                            q[1] = make_float4(__int_as_float(c.l[0]), __int_as_float(c.l[1]),
-                                               __int_as_float(c.l[2]), c.weight);
-                            q[2] = make_float4(__int_as_float(c.lk[0]), __int_as_float(c.lk[1]),
-                                               __int_as_float(c.lk[2]), __int_as_float(c.lk[3]));
-                            q[3] = make_float4(__int_as_float(c.lk[4]), __int_as_float(c.lk[5]),
-                                               __int_as_float(c.lk[6]), __int_as_float(c.lk[7]));
-                            q[4] = make_float4(cs[0], cs[1], cs[2], c.lt_after);
-                            q[5] = make_float4(c.ws, c.sigma, 0.f, 0.f);
-                        }
-                        cur += n_live + 1;
-                    }
-                }
-            }
-            if (STATS) {
-                n_iter += c.n_iter;
-                n_eval += c.n_eval;
-                n_hit += n_live;
-                for (int j = 0; j < c.n_eval; ++j) {
-                    fnv.mix(static_cast<uint32_t>(__shfl(c.base, j, kGroup)));
-                    fnv.mix(__float_as_uint(__shfl(c.t, j, kGroup)));
-                    fnv.mix(__shfl(static_cast<int>(c.contrib), j, kGroup) ? 1u : 0u);
-                }
-            }
-            log_t = c.log_t_end;
-            t = c.next_t;
-            more = c.more;
+    bool defer = DEFER;
+    for (int attempt = 0; attempt < (DEFER ? 2 : 1); ++attempt) {
+        if (DEFER && attempt == 1) {  // the deferred march did not fit
+            defer = false;
+            total[0] = total[1] = total[2] = 0.f;
+            n_chunks = n_live_all = n_live_chunks = 0;
         }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        acc[a] += __shfl_xor(acc[a], 8, kGroup);
-        acc[a] += __shfl_xor(acc[a], 16, kGroup);
+        log_t = 0.f;
+        if (r.tmin <= r.tmax) {
+            float t = r.tmin;
+            int guard = 0;
+            bool more = true;
+            while (more && ++guard < kMaxIters) {
+                Chunk c;
+                march_chunk(g, o, r, lane, t, log_t, c);
+                const int n_live = __popc(c.live_mask);
+                if (LOG && n_chunks < static_cast<uint32_t>(kLogChunks)) {
+                    // Staged in LDS and written out when the ray ends: global
+                    // stores here would hold up the next chunk's loads (a load's
+                    // wait also waits for every earlier store).
+                    float* rec = log_lds + n_chunks * kLogWords;
+                    rec[lane] = c.t;
+                    if (lane == 0) {
+                        rec[32] = __int_as_float(c.n_place);
+                        rec[33] = __uint_as_float(c.load_mask);
+                    }
+                    log_more = c.more;
+                    log_next_t = c.next_t;
+                }
+                ++n_chunks;
+                if (RECORD && n_live > 0) {
+                    n_live_all += n_live;
+                    ++n_live_chunks;
+                }
+                if (n_live > 0) {
+                    // Recorded chunk k = n_rec_chunks opens replay segment
+                    // k / kRecSegChunks when k is a multiple of kRecSegChunks.
+                    const bool seg_start = RECORD && rec_ok && n_rec_chunks > 0 &&
+                                           n_rec_chunks % kRecSegChunks == 0 &&
+                                           n_rec_chunks / kRecSegChunks < kRecSegs;
+                    float cs[3] = {0.f, 0.f, 0.f};
+                    if (!defer) {
+                        float cacc[3] = {0.f, 0.f, 0.f}, sum[3];
+                        chunk_colours<DT, B, RECORD>(g, rc, c, lane, cs, cacc);
+                        add_chunk_colour(total, cacc, sum);
+                    }
+                    if (RECORD && rec_ok) {
+                        // Header + n_live records; keep one slot for a jump/end header.
+                        if (cur + n_live + 2 > end) {
+                            uint32_t nb = 0;
+                            if (lane == 0) nb = atomicAdd(rs->next_block, static_cast<uint32_t>(kRecBlock));
+                            nb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), 0, kGroup));
+                            if (static_cast<int64_t>(nb) + kRecBlock > rs->overflow_slots) {
+                                rec_ok = false;
+                            } else {
+                                const int64_t nxt = rs->overflow_base + nb;
+                                if (lane == 0) put_header(rs->rec, cur, 0, nxt);
+                                cur = nxt;
+                                end = nxt + kRecBlock;
+                            }
+                        }
+                        if (DEFER && defer && n_rec_chunks >= kDeferChunks) rec_ok = false;
+                        if (rec_ok) {
+                            const int k = n_rec_chunks++;
+                            if (seg_start && lane == 0)  // a replay segment may start here
+                                rs->seg[ray * (kRecSegs - 1) + k / kRecSegChunks - 1] =
+                                    static_cast<int32_t>(cur);
+                            if (DEFER && defer && lane == 0) slot_lds[k] = static_cast<int32_t>(cur);
+                            if (lane == 0) put_header(rs->rec, cur, n_live, 0);
+                            if (c.live) {
+                                const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
+                                float4* q = rs->rec + 6 * (cur + 1 + rank);
+                                q[0] = make_float4(c.p[0], c.p[1], c.p[2], __int_as_float(c.base));
+                                q[1] = make_float4(__int_as_float(c.l[0]), __int_as_float(c.l[1]),
+                                                   __int_as_float(c.l[2]), c.weight);
+                                q[2] = make_float4(__int_as_float(c.lk[0]), __int_as_float(c.lk[1]),
+                                                   __int_as_float(c.lk[2]), __int_as_float(c.lk[3]));
+                                q[3] = make_float4(__int_as_float(c.lk[4]), __int_as_float(c.lk[5]),
+                                                   __int_as_float(c.lk[6]), __int_as_float(c.lk[7]));
+                                q[4] = make_float4(cs[0], cs[1], cs[2], c.lt_after);
+                                q[5] = make_float4(c.ws, c.sigma, 0.f, 0.f);
+                            }
+                            cur += n_live + 1;
+                        }
+                    }
+                }
+                if (STATS) {
+                    n_iter += c.n_iter;
+                    n_eval += c.n_eval;
+                    n_hit += n_live;
+                    for (int j = 0; j < c.n_eval; ++j) {
+                        fnv.mix(static_cast<uint32_t>(__shfl(c.base, j, kGroup)));
+                        fnv.mix(__float_as_uint(__shfl(c.t, j, kGroup)));
+                        fnv.mix(__shfl(static_cast<int>(c.contrib), j, kGroup) ? 1u : 0u);
+                    }
+                }
+                log_t = c.log_t_end;
+                t = c.next_t;
+                more = c.more;
+                if (DEFER && defer && !rec_ok) break;  // march again, colours here
+            }
+        }
+        if (!DEFER || !defer) break;
+        if (!rec_ok) continue;  // the records did not fit
+        if (n_rec_chunks > 0) {
+            // List the recorded chunks: one reservation per ray.
+            int first = 0;
+            if (lane == 0) first = static_cast<int>(atomicAdd(rs->next_block + 2,
+                                                              static_cast<uint32_t>(n_rec_chunks)));
+            first = __shfl(first, 0, kGroup);
+            const bool fits = static_cast<int64_t>(first) + n_rec_chunks <= rs->clist_cap;
+            wave_lds_fence();
+            for (int k = lane; k < n_rec_chunks; k += kGroup) {
+                const int64_t e = static_cast<int64_t>(first) + k;
+                if (e < rs->clist_cap)
+                    rs->clist[e] = fits ? make_int2(static_cast<int>(ray), slot_lds[k])
+                                        : make_int2(-1, -1);
+            }
+            if (!fits) {
+                rec_ok = false;
+                continue;
+            }
+            if (lane == 0)
+                rs->ray_info[ray] = make_float4(__int_as_float(first), __int_as_float(n_rec_chunks),
+                                                __expf(log_t) * o.bg, 0.f);
+        } else if (lane == 0) {
+            rs->ray_info[ray] = make_float4(0.f, 0.f, __expf(log_t) * o.bg, 0.f);
+        }
+        break;
     }
     const float tb = __expf(log_t) * o.bg;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[a] = group_lane(acc[a], 0) + tb;
+    for (int a = 0; a < 3; ++a) out[a] = total[a] + tb;
     // A replayed backward works per record and per recorded chunk, a
     // re-marching one per chunk.
     if (cost != nullptr)
@@ -1487,11 +1546,76 @@ SVOX2_REC_KERNEL void k_render_rays_rec(
     setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
     float out[3];
     uint32_t cost = 0;
-    const float log_t = march_forward<DT, B, false, true>(
-        g, o, r, rc, lane, out, nullptr, cost_out != nullptr ? &cost : nullptr, &rs, ray);
+    // LDS staging of this group's recorded chunk header slots.
+    __shared__ int32_t slot_lds[(kBlock / kGroup) * kDeferChunks];
+    const float log_t = march_forward<DT, B, false, true, false, true>(
+        g, o, r, rc, lane, out, nullptr, cost_out != nullptr ? &cost : nullptr, &rs, ray,
+        nullptr, nullptr, slot_lds + (threadIdx.x / kGroup) * kDeferChunks);
+    // A ray whose chunks were listed gets its colour from k_rec_colours.
     if (lane < 3) rgb_out[3 * ray + lane] = pick3(out, lane);
     if (log_t_out != nullptr && lane == 0) log_t_out[ray] = log_t;
     if (cost_out != nullptr && lane == 0) cost_out[ray] = cost;
+}
+
+// Colours of the chunks the recording forward listed, one group per chunk
+// (the chunks of a ray run on different groups at once).  A group reads the
+// chunk's records (positions, cells, corner links, weights: its live samples
+// compacted into lanes 0 .. n - 1, which gives the forward's runs), forms the
+// colours as the forward does, writes each sample's colour + 0.5 into its
+// record and the chunk's colour into chunk_rgb.  The replay sums a ray's
+// chunk colours in march order (no group here waits for another).
+template <typename DT, int B>
+__global__ __launch_bounds__(kBlock) void k_rec_colours(GridView<DT> g,
+                                                        const float* __restrict__ dirs,
+                                                        RecStream rs) {
+    const int lane = threadIdx.x % kGroup;
+    const int64_t gid = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int64_t ngroups = static_cast<int64_t>(gridDim.x) * (kBlock / kGroup);
+    const int64_t total = min(static_cast<int64_t>(rs.next_block[2]), rs.clist_cap);
+    for (int64_t j = gid; j < total; j += ngroups) {
+        const int2 e = rs.clist[j];
+        if (e.x < 0) continue;  // entries of a ray whose list did not fit
+        const int64_t ray = e.x, slot = e.y;
+        const int n = __float_as_int(rs.rec[6 * slot].x);
+        Chunk c;
+        c.live = lane < n;
+        c.live_mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+        float4* q = rs.rec + 6 * (slot + 1 + lane);
+        if (c.live) {
+            const float4 a = q[0], b = q[1], c2 = q[2], d = q[3];
+            c.p[0] = a.x;
+            c.p[1] = a.y;
+            c.p[2] = a.z;
+            c.base = __float_as_int(a.w);
+            c.weight = b.w;
+            c.lk[0] = __float_as_int(c2.x);
+            c.lk[1] = __float_as_int(c2.y);
+            c.lk[2] = __float_as_int(c2.z);
+            c.lk[3] = __float_as_int(c2.w);
+            c.lk[4] = __float_as_int(d.x);
+            c.lk[5] = __float_as_int(d.y);
+            c.lk[6] = __float_as_int(d.z);
+            c.lk[7] = __float_as_int(d.w);
+        } else {
+            c.p[0] = c.p[1] = c.p[2] = 0.f;
+            c.base = -1;
+            c.weight = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) c.lk[kk] = -1;
+        }
+        const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+        const RayCtx<B> rc(wd);
+        float cs[3] = {0.f, 0.f, 0.f}, cacc[3] = {0.f, 0.f, 0.f}, sum[3], unused[3] = {0.f, 0.f, 0.f};
+        chunk_colours<DT, B, true>(g, rc, c, lane, cs, cacc);
+        add_chunk_colour(unused, cacc, sum);
+        if (c.live) {
+            float* q4 = reinterpret_cast<float*>(q + 4);
+            q4[0] = cs[0];
+            q4[1] = cs[1];
+            q4[2] = cs[2];
+        }
+        if (lane == 0) rs.chunk_rgb[j] = make_float4(sum[0], sum[1], sum[2], 0.f);
+    }
 }
 
 template <typename DT, int B, bool STATS, bool LOG = false>
@@ -1791,17 +1915,21 @@ SVOX2_BWD_KERNEL void k_render_backward(
 // Backward of the fused MSE step for the rays whose samples the forward
 // recorded (state 1): the same gradient as k_render_backward_mse, from the
 // records, in a kernel without the march code (fewer registers, more waves).
+// The ray's colour is the sum of its chunk colours (k_rec_colours) in march
+// order plus the background term, as the forward forms it; the group of item
+// < n writes it to rgb_out.  A segment m >= 1 of a split ray starts from the
+// colour composited before its first chunk.
 template <typename DT, int B>
 SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         GridView<DT> g, const float* __restrict__ dirs, int64_t n,
-        const float* __restrict__ rgb_gt, const float* __restrict__ rgb,
+        const float* __restrict__ rgb_gt, float* __restrict__ rgb_out,
         const float* __restrict__ log_t_fwd, float beta_per_ray, float sparsity, float norm,
         const uint32_t* __restrict__ order, GradOut go, RecStream rs) {
     const int64_t item = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int lane = threadIdx.x % kGroup;
     int64_t ray, pos;
     int max_chunks = kMaxIters;
-    const float* before = nullptr;  // colour composited before the segment
+    int seg_chunk = -1;  // recorded chunks before the segment (segments m >= 1)
     if (item < n) {  // a ray from its start; a split ray's first segment only
         ray = order != nullptr ? static_cast<int64_t>(order[item]) : item;
         if (rs.state[ray] != 1) return;  // re-marched by k_render_backward_mse
@@ -1816,22 +1944,40 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         pos = rs.seg[ray * (kRecSegs - 1) + m - 1];
         if (pos < 0) return;  // the ray ends before this segment
         if (m < kRecSegs - 1) max_chunks = kRecSegChunks;
-        before = reinterpret_cast<const float*>(rs.rec + 6 * pos + 1);
+        seg_chunk = m * kRecSegChunks;
     }
     const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
     const RayCtx<B> rc(wd);
-    float c3[3], gq[3];
+    // The ray's colour, and the colour composited before the segment.
+    const float4 info = rs.ray_info[ray];
+    const int first = __float_as_int(info.x), cnt = __float_as_int(info.y);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v0 = lane < cnt ? rs.chunk_rgb[first + lane] : zero4;
+    const float4 v1 = lane + kGroup < cnt ? rs.chunk_rgb[first + kGroup + lane] : zero4;
+    float c3[3] = {0.f, 0.f, 0.f}, before[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < cnt; ++k) {
+        if (k == seg_chunk) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) before[c] = c3[c];
+        }
+        const float4 v = k < kGroup ? v0 : v1;
+        c3[0] += group_bcast(v.x, k & (kGroup - 1));
+        c3[1] += group_bcast(v.y, k & (kGroup - 1));
+        c3[2] += group_bcast(v.z, k & (kGroup - 1));
+    }
+    float gq[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        c3[c] = rgb[3 * ray + c];
+        c3[c] += info.z;
         gq[c] = (c3[c] - rgb_gt[3 * ray + c]) * norm;
     }
+    if (item < n && lane < 3) rgb_out[3 * ray + lane] = pick3(c3, lane);
     float accum = fmaf(c3[0], gq[0], fmaf(c3[1], gq[1], c3[2] * gq[2]));
     if (beta_per_ray > 0.f) {
         const float tin = __expf(log_t_fwd[ray]);
         accum += beta_per_ray * (1.f - tin / (1.f - tin + 1e-3f));
     }
-    if (before != nullptr)  // accum after the samples before the segment
+    if (seg_chunk >= 0)  // accum after the samples before the segment
         accum -= fmaf(before[0], gq[0], fmaf(before[1], gq[1], before[2] * gq[2]));
     __shared__ BwdShared sh;
     BwdRay<B> st(rc, lane, gq, accum, sparsity, &sh, go);
@@ -2019,7 +2165,8 @@ inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
 int64_t split_cap_of(int64_t n) { return (n + 7) / 8; }
 
-// [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | long_list | counters | records]
+// [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | long_list | counters |
+//  ray_info (n) | clist (kDeferCap n) | chunk_rgb (kDeferCap n) | records]
 size_t fused_workspace_bytes(int64_t n) {
     const int64_t order = static_cast<int64_t>(order_workspace_bytes(n));
     const int64_t state = align256(4 * n);
@@ -2027,8 +2174,12 @@ size_t fused_workspace_bytes(int64_t n) {
     const int64_t split = align256(4 * n);
     const int64_t list = align256(4 * split_cap_of(n));
     const int64_t counter = 256;
+    const int64_t info = align256(16 * n);
+    const int64_t clist = align256(8 * kDeferCap * n);
+    const int64_t crgb = align256(16 * kDeferCap * n);
     const int64_t recs = n * (kRecFixed + kRecOverflow) * kRecSlotBytes;
-    return static_cast<size_t>(order + state + seg + split + list + counter + recs);
+    return static_cast<size_t>(order + state + seg + split + list + counter + info + clist +
+                               crgb + recs);
 }
 
 // Fused-call workspace with the chunk log (batches that re-march): the
@@ -2063,6 +2214,13 @@ RecStream record_stream(void* ws, int64_t n) {
     p += align256(4 * split_cap_of(n));
     rs.next_block = reinterpret_cast<uint32_t*>(p);
     p += 256;
+    rs.ray_info = reinterpret_cast<float4*>(p);
+    p += align256(16 * n);
+    rs.clist = reinterpret_cast<int2*>(p);
+    rs.clist_cap = kDeferCap * n;
+    p += align256(8 * kDeferCap * n);
+    rs.chunk_rgb = reinterpret_cast<float4*>(p);
+    p += align256(16 * kDeferCap * n);
     rs.rec = reinterpret_cast<float4*>(p);
     rs.fixed = kRecFixed;
     rs.overflow_base = n * kRecFixed;
@@ -2543,7 +2701,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
-                           reinterpret_cast<uint32_t*>(rs.next_block), 2);
+                           reinterpret_cast<uint32_t*>(rs.next_block), 3);
     }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
@@ -2553,10 +2711,14 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         // through w.cost / w.order; the forward rewrites w.cost by ray).
         if (w.order != nullptr) launch_length_order(grid, o, origins, dirs, ray_clip, n_rays, w, s);
         const uint32_t* order = w.order;
-        if (rs.rec != nullptr)
+        if (rs.rec != nullptr) {
             hipLaunchKernelGGL((k_render_rays_rec<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
                                o, origins, dirs, ray_clip, n_rays, rgb_out, lt, w.cost, rs,
                                order);
+            hipLaunchKernelGGL((k_rec_colours<DT, B>),
+                               dim3(blocks_for(std::min(rs.clist_cap, 2 * n_rays) * kGroup, kBlock)),
+                               dim3(kBlock), 0, s, g, dirs, rs);
+        }
         else if (log.rec != nullptr)
             hipLaunchKernelGGL((k_render_rays<DT, B, false, true>), dim3(blocks), dim3(kBlock), 0,
                                s, g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
