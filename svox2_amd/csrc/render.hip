@@ -832,14 +832,16 @@ __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int
 }
 
 // Adds a chunk's colour to the ray's: the run groups' partial sums (lanes 0,
-// 8, 16 and 24 of cacc) reduced across the group, then added to total in
-// march order (every lane of the group gets the same sums).
+// 8, 16 and 24 of cacc) reduced across the group as (a0 + a8) + (a16 + a24),
+// lane ^ 8 by a row rotation and lane ^ 16 by a bit-mode swizzle (no LDS
+// addresses), then added to total in march order (every lane of the group
+// gets the same sums).
 __device__ __forceinline__ void add_chunk_colour(float* total, const float* cacc, float* sum) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float v = cacc[a];
-        v += __shfl_xor(v, 8, kGroup);
-        v += __shfl_xor(v, 16, kGroup);
+        v += dpp<0x128>(v);  // row_ror:8 - lane ^ 8 within each 16-lane row
+        v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));  // lane ^ 16
         sum[a] = group_lane(v, 0);
         total[a] += sum[a];
     }
