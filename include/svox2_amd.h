@@ -227,7 +227,7 @@ size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
  * library (one copy per device, looked up with hipGetSymbolAddress on the
  * device's first call).  Same arguments, results and errors as the fused
  * call, minus workspace / workspace_bytes.  Batches whose
- * svox2_volume_render_cuvol_fused_workspace_bytes exceeds 40 MiB (above 8640
+ * svox2_volume_render_cuvol_fused_workspace_bytes exceeds 40 MiB (above 8585
  * rays; 8192 rays always fit) fail with SVOX2_ERR_INVALID_ARGUMENT.
  * Calls that overlap on one device share the array and race: issue them on
  * one stream.  No counterpart in the reference: for callers that cannot

@@ -795,20 +795,19 @@ constexpr int kLogWords = 36;
 // ran out (the backward then re-marches the ray).
 //
 // Long rays are replayed in segments: a ray with at least kRecSplitMin
-// recorded chunks is listed (split, long_list, up to split_cap rays), and
-// seg[r * (kRecSegs - 1) + m - 1] holds the header slot of its recorded
-// chunk m * kRecSegChunks (-1 past its end).  The header of such a chunk
-// carries, in its second float4, the colour composited before the chunk, so
-// a segment's backward can start there.
+// recorded chunks lists its segments m >= 1 in seg_list (one reservation per
+// ray, seg_cap entries in all) and sets split; seg[r * (kRecSegs - 1) + m -
+// 1] holds the header slot of its recorded chunk m * kRecSegChunks (-1 past
+// its end), where segment m's backward starts.
 struct RecStream {
     float4* __restrict__ rec;
     int32_t* __restrict__ state;
-    uint32_t* __restrict__ next_block;  // [0] overflow slots taken, [1] long rays listed,
-                                        // [2] chunk list entries taken
+    uint32_t* __restrict__ next_block;  // [0] overflow slots taken, [1] segment list
+                                        // entries taken, [2] chunk list entries taken
     int32_t* __restrict__ seg;
-    int32_t* __restrict__ split;        // per ray: its index in long_list, or -1
-    int32_t* __restrict__ long_list;
-    int64_t split_cap;
+    int32_t* __restrict__ split;        // per ray: 1 if its segments are listed, else -1
+    int2* __restrict__ seg_list;        // (ray, segment m >= 1), or (-1, -1)
+    int64_t seg_cap;
     int64_t fixed;                      // slots per ray in the fixed region
     int64_t overflow_base;              // first overflow slot
     int64_t overflow_slots;             // slots in the overflow region
@@ -822,8 +821,8 @@ struct RecStream {
 constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
 constexpr int kDeferChunks = 64;  // recorded chunks a ray can list for deferred colours
 constexpr int64_t kDeferCap = 8;  // chunk list entries per ray of the batch
-constexpr int kRecSegChunks = 4;  // recorded chunks per replay segment
-constexpr int kRecSegs = 6;       // segments of a split ray (the last runs to its end)
+constexpr int kRecSegChunks = 2;  // recorded chunks per replay segment
+constexpr int kRecSegs = 12;      // segments of a split ray (the last runs to its end)
 constexpr int kRecSplitMin = 2 * kRecSegChunks;  // recorded chunks that make a ray split
 
 __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int64_t next) {
@@ -1019,18 +1018,29 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
     // re-marching one per chunk.
     if (cost != nullptr)
         *cost = backward_cost(RECORD ? n_live_all + 4u * n_live_chunks : n_chunks);
-    if (RECORD && lane == 0) {
-        if (rec_ok) put_header(rs->rec, cur, -1, 0);
-        rs->state[ray] = rec_ok ? 1 : 2;
-        for (int m = 1; m < kRecSegs; ++m)
-            if (!rec_ok || m * kRecSegChunks >= n_rec_chunks) rs->seg[ray * (kRecSegs - 1) + m - 1] = -1;
-        int32_t idx = -1;
-        if (rec_ok && n_rec_chunks >= kRecSplitMin) {
-            idx = static_cast<int32_t>(atomicAdd(rs->next_block + 1, 1u));
-            if (idx < rs->split_cap) rs->long_list[idx] = static_cast<int32_t>(ray);
-            else idx = -1;
+    if (RECORD) {
+        // A long ray lists its segments m >= 1 for the replay (one reservation).
+        int n_seg = 1;
+        if (rec_ok && n_rec_chunks >= kRecSplitMin)
+            n_seg = min((n_rec_chunks + kRecSegChunks - 1) / kRecSegChunks, kRecSegs);
+        int base = 0;
+        if (lane == 0 && n_seg > 1)
+            base = static_cast<int>(atomicAdd(rs->next_block + 1, static_cast<uint32_t>(n_seg - 1)));
+        base = __shfl(base, 0, kGroup);
+        const bool listed = n_seg > 1 && static_cast<int64_t>(base) + n_seg - 1 <= rs->seg_cap;
+        for (int m = 1 + lane; m < n_seg; m += kGroup) {
+            const int64_t e = static_cast<int64_t>(base) + m - 1;
+            if (e < rs->seg_cap)
+                rs->seg_list[e] = listed ? make_int2(static_cast<int>(ray), m) : make_int2(-1, -1);
         }
-        rs->split[ray] = idx;
+        if (lane == 0) {
+            if (rec_ok) put_header(rs->rec, cur, -1, 0);
+            rs->state[ray] = rec_ok ? 1 : 2;
+            for (int m = 1; m < kRecSegs; ++m)
+                if (!rec_ok || m * kRecSegChunks >= n_rec_chunks)
+                    rs->seg[ray * (kRecSegs - 1) + m - 1] = -1;
+            rs->split[ray] = listed ? 1 : -1;
+        }
     }
     if (LOG) {
         const int n_logged = static_cast<int>(min(n_chunks, static_cast<uint32_t>(kLogChunks)));
@@ -1937,12 +1947,13 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         if (rs.state[ray] != 1) return;  // re-marched by k_render_backward_mse
         pos = ray * rs.fixed;
         if (rs.split[ray] >= 0) max_chunks = kRecSegChunks;
-    } else {  // segment m >= 1 of listed ray j
+    } else {  // a listed segment m >= 1 of a split ray
         const int64_t e = item - n;
-        const int64_t j = e / (kRecSegs - 1);
-        const int m = 1 + static_cast<int>(e % (kRecSegs - 1));
-        if (j >= min(static_cast<int64_t>(rs.next_block[1]), rs.split_cap)) return;
-        ray = rs.long_list[j];
+        if (e >= min(static_cast<int64_t>(rs.next_block[1]), rs.seg_cap)) return;
+        const int2 sm = rs.seg_list[e];
+        if (sm.x < 0) return;  // entries of a ray whose segments did not fit
+        ray = sm.x;
+        const int m = sm.y;
         pos = rs.seg[ray * (kRecSegs - 1) + m - 1];
         if (pos < 0) return;  // the ray ends before this segment
         if (m < kRecSegs - 1) max_chunks = kRecSegChunks;
@@ -2165,16 +2176,18 @@ constexpr int64_t kRecSlotBytes = 96;
 
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-int64_t split_cap_of(int64_t n) { return (n + 7) / 8; }
+// Segment list entries: one extra segment per ray of the batch (rays whose
+// segments do not fit are replayed whole).
+int64_t seg_cap_of(int64_t n) { return n; }
 
-// [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | long_list | counters |
+// [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | seg_list | counters |
 //  ray_info (n) | clist (kDeferCap n) | chunk_rgb (kDeferCap n) | records]
 size_t fused_workspace_bytes(int64_t n) {
     const int64_t order = static_cast<int64_t>(order_workspace_bytes(n));
     const int64_t state = align256(4 * n);
     const int64_t seg = align256(4 * n * (kRecSegs - 1));
     const int64_t split = align256(4 * n);
-    const int64_t list = align256(4 * split_cap_of(n));
+    const int64_t list = align256(8 * seg_cap_of(n));
     const int64_t counter = 256;
     const int64_t info = align256(16 * n);
     const int64_t clist = align256(8 * kDeferCap * n);
@@ -2211,9 +2224,9 @@ RecStream record_stream(void* ws, int64_t n) {
     p += align256(4 * n * (kRecSegs - 1));
     rs.split = reinterpret_cast<int32_t*>(p);
     p += align256(4 * n);
-    rs.long_list = reinterpret_cast<int32_t*>(p);
-    rs.split_cap = split_cap_of(n);
-    p += align256(4 * split_cap_of(n));
+    rs.seg_list = reinterpret_cast<int2*>(p);
+    rs.seg_cap = seg_cap_of(n);
+    p += align256(8 * seg_cap_of(n));
     rs.next_block = reinterpret_cast<uint32_t*>(p);
     p += 256;
     rs.ray_info = reinterpret_cast<float4*>(p);
@@ -2733,7 +2746,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s, 1, true);
         if (rs.rec != nullptr)
             hipLaunchKernelGGL((k_render_backward_replay<DT, B>),
-                               dim3(blocks_for((n_rays + rs.split_cap * (kRecSegs - 1)) * kGroup,
+                               dim3(blocks_for((n_rays + rs.seg_cap) * kGroup,
                                                kBlock)),
                                dim3(kBlock), 0, s, g, dirs, n_rays, rgb_gt, rgb_out, lt, beta,
                                sparsity_loss, norm, static_cast<const uint32_t*>(w.order), go, rs);
