@@ -217,7 +217,7 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 
 /* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
  * fastest for an n_rays batch (0 for an empty batch): the record stream
- * and chunk list (about 4.7 KiB per ray) below the ordering threshold, the ray-ordering
+ * and chunk list (about 4.8 KiB per ray) below the ordering threshold, the ray-ordering
  * workspace plus the chunk log (about 2.3 KiB per ray) at or above it.  No
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
@@ -227,7 +227,7 @@ size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
  * library (one copy per device, looked up with hipGetSymbolAddress on the
  * device's first call).  Same arguments, results and errors as the fused
  * call, minus workspace / workspace_bytes.  Batches whose
- * svox2_volume_render_cuvol_fused_workspace_bytes exceeds 40 MiB (above 8585
+ * svox2_volume_render_cuvol_fused_workspace_bytes exceeds 40 MiB (above 8474
  * rays; 8192 rays always fit) fail with SVOX2_ERR_INVALID_ARGUMENT.
  * Calls that overlap on one device share the array and race: issue them on
  * one stream.  No counterpart in the reference: for callers that cannot

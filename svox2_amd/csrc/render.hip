@@ -821,8 +821,8 @@ struct RecStream {
 constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
 constexpr int kDeferChunks = 64;  // recorded chunks a ray can list for deferred colours
 constexpr int64_t kDeferCap = 8;  // chunk list entries per ray of the batch
-constexpr int kRecSegChunks = 2;  // recorded chunks per replay segment
-constexpr int kRecSegs = 12;      // segments of a split ray (the last runs to its end)
+constexpr int kRecSegChunks = 1;  // recorded chunks per replay segment
+constexpr int kRecSegs = 24;      // segments of a split ray (the last runs to its end)
 constexpr int kRecSplitMin = 2 * kRecSegChunks;  // recorded chunks that make a ray split
 
 __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int64_t next) {
@@ -2176,9 +2176,9 @@ constexpr int64_t kRecSlotBytes = 96;
 
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-// Segment list entries: one extra segment per ray of the batch (rays whose
-// segments do not fit are replayed whole).
-int64_t seg_cap_of(int64_t n) { return n; }
+// Segment list entries: three extra segments per ray of the batch (rays
+// whose segments do not fit are replayed whole).
+int64_t seg_cap_of(int64_t n) { return 3 * n; }
 
 // [order | state (n) | seg (n (kRecSegs - 1)) | split (n) | seg_list | counters |
 //  ray_info (n) | clist (kDeferCap n) | chunk_rgb (kDeferCap n) | records]
