@@ -720,9 +720,10 @@ def main():
             "traffic_source": "profiles/fused_512_f32_65536.md (rocprofv3 --pmc FETCH_SIZE, "
                               "WRITE_SIZE of `bench.py --pmc`, calibrated on a 1 GiB copy)",
             "kernel": "fused call (float32 storage, SH-2): forward ray order "
-                      "(k_ray_length_key, k_cost_hist, k_cost_scatter), k_render_rays "
-                      "(logging chunk placements), backward ray order (k_cost_hist, "
-                      "k_cost_scatter), k_render_backward_mse",
+                      "(k_ray_length_key, k_cost_hist, k_cost_scatter), k_render_rays_rec "
+                      "(recording the samples), k_rec_colours, backward ray order "
+                      "(k_cost_hist, k_cost_scatter), k_render_backward_replay, "
+                      "k_render_backward_mse (rays whose records overflowed)",
             "kernel_ms": head["kernel_ms"],
             "alg_bytes_per_launch": head["alg_bytes_per_launch"],
             "counters_per_launch": head["counters"],

@@ -802,8 +802,13 @@ constexpr int kLogWords = 36;
 struct RecStream {
     float4* __restrict__ rec;
     int32_t* __restrict__ state;
-    uint32_t* __restrict__ next_block;  // [0] overflow slots taken, [1] segment list
-                                        // entries taken, [2] chunk list entries taken
+    // Counters, one per partition of rays (ray % parts), so that no single
+    // address takes every ray's atomic: [p] overflow slots taken of opart,
+    // [kRecParts + p] segment list entries of spart, [2 kRecParts + p] chunk
+    // list entries of cpart; partition p owns the p-th slice of each region.
+    uint32_t* __restrict__ next_block;
+    int parts;
+    int64_t opart, spart, cpart;
     int32_t* __restrict__ seg;
     int32_t* __restrict__ split;        // per ray: 1 if its segments are listed, else -1
     int2* __restrict__ seg_list;        // (ray, segment m >= 1), or (-1, -1)
@@ -812,18 +817,28 @@ struct RecStream {
     int64_t overflow_base;              // first overflow slot
     int64_t overflow_slots;             // slots in the overflow region
     // Deferred colours: the recorded chunks of rays whose colours are formed
-    // by k_rec_colours, next_block[2] entries taken of clist_cap.
+    // by k_rec_colours.
     int2* __restrict__ clist;           // (ray, header slot), or (-1, -1)
     float4* __restrict__ chunk_rgb;     // per entry: the chunk's colour
     float4* __restrict__ ray_info;      // per ray: (first entry, entries, background term)
     int64_t clist_cap;
 };
 constexpr int kRecBlock = 64;  // >= 32 records + header + the end/jump header
+constexpr int kRecParts = 64;  // counter partitions at most
 constexpr int kDeferChunks = 64;  // recorded chunks a ray can list for deferred colours
 constexpr int64_t kDeferCap = 8;  // chunk list entries per ray of the batch
 constexpr int kRecSegChunks = 1;  // recorded chunks per replay segment
 constexpr int kRecSegs = 24;      // segments of a split ray (the last runs to its end)
 constexpr int kRecSplitMin = 2 * kRecSegChunks;  // recorded chunks that make a ray split
+
+// A chunk of n records occupies slots slot .. slot + n (the header, then n
+// records), stored field-major after the header's slot: field j (0..5) of
+// record r is rec_field0(rec, slot)[j * n + r], so that each field's stores
+// and loads over a chunk's records are one contiguous run.
+__device__ __forceinline__ float4* rec_field0(float4* rec, int64_t slot) { return rec + 6 * slot + 6; }
+__device__ __forceinline__ const float4* rec_field0(const float4* rec, int64_t slot) {
+    return rec + 6 * slot + 6;
+}
 
 __device__ __forceinline__ void put_header(float4* rec, int64_t slot, int n, int64_t next) {
     rec[6 * slot] = make_float4(__int_as_float(n), __int_as_float(static_cast<int>(next)), 0.f,
@@ -930,13 +945,15 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
                     if (RECORD && rec_ok) {
                         // Header + n_live records; keep one slot for a jump/end header.
                         if (cur + n_live + 2 > end) {
+                            const int part = static_cast<int>(ray % rs->parts);
                             uint32_t nb = 0;
-                            if (lane == 0) nb = atomicAdd(rs->next_block, static_cast<uint32_t>(kRecBlock));
+                            if (lane == 0)
+                                nb = atomicAdd(rs->next_block + part, static_cast<uint32_t>(kRecBlock));
                             nb = static_cast<uint32_t>(__shfl(static_cast<int>(nb), 0, kGroup));
-                            if (static_cast<int64_t>(nb) + kRecBlock > rs->overflow_slots) {
+                            if (static_cast<int64_t>(nb) + kRecBlock > rs->opart) {
                                 rec_ok = false;
                             } else {
-                                const int64_t nxt = rs->overflow_base + nb;
+                                const int64_t nxt = rs->overflow_base + part * rs->opart + nb;
                                 if (lane == 0) put_header(rs->rec, cur, 0, nxt);
                                 cur = nxt;
                                 end = nxt + kRecBlock;
@@ -952,16 +969,16 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
                             if (lane == 0) put_header(rs->rec, cur, n_live, 0);
                             if (c.live) {
                                 const int rank = __popc(c.live_mask & ((1u << lane) - 1u));
-                                float4* q = rs->rec + 6 * (cur + 1 + rank);
+                                float4* q = rec_field0(rs->rec, cur) + rank;
                                 q[0] = make_float4(c.p[0], c.p[1], c.p[2], __int_as_float(c.base));
-                                q[1] = make_float4(__int_as_float(c.l[0]), __int_as_float(c.l[1]),
-                                                   __int_as_float(c.l[2]), c.weight);
-                                q[2] = make_float4(__int_as_float(c.lk[0]), __int_as_float(c.lk[1]),
-                                                   __int_as_float(c.lk[2]), __int_as_float(c.lk[3]));
-                                q[3] = make_float4(__int_as_float(c.lk[4]), __int_as_float(c.lk[5]),
-                                                   __int_as_float(c.lk[6]), __int_as_float(c.lk[7]));
-                                q[4] = make_float4(cs[0], cs[1], cs[2], c.lt_after);
-                                q[5] = make_float4(c.ws, c.sigma, 0.f, 0.f);
+                                q[n_live] = make_float4(__int_as_float(c.l[0]), __int_as_float(c.l[1]),
+                                                        __int_as_float(c.l[2]), c.weight);
+                                q[2 * n_live] = make_float4(__int_as_float(c.lk[0]), __int_as_float(c.lk[1]),
+                                                            __int_as_float(c.lk[2]), __int_as_float(c.lk[3]));
+                                q[3 * n_live] = make_float4(__int_as_float(c.lk[4]), __int_as_float(c.lk[5]),
+                                                            __int_as_float(c.lk[6]), __int_as_float(c.lk[7]));
+                                q[4 * n_live] = make_float4(cs[0], cs[1], cs[2], c.lt_after);
+                                q[5 * n_live] = make_float4(c.ws, c.sigma, 0.f, 0.f);
                             }
                             cur += n_live + 1;
                         }
@@ -987,17 +1004,19 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
         if (!rec_ok) continue;  // the records did not fit
         if (n_rec_chunks > 0) {
             // List the recorded chunks: one reservation per ray.
-            int first = 0;
-            if (lane == 0) first = static_cast<int>(atomicAdd(rs->next_block + 2,
-                                                              static_cast<uint32_t>(n_rec_chunks)));
-            first = __shfl(first, 0, kGroup);
-            const bool fits = static_cast<int64_t>(first) + n_rec_chunks <= rs->clist_cap;
+            const int part = static_cast<int>(ray % rs->parts);
+            int local = 0;
+            if (lane == 0)
+                local = static_cast<int>(atomicAdd(rs->next_block + 2 * kRecParts + part,
+                                                   static_cast<uint32_t>(n_rec_chunks)));
+            local = __shfl(local, 0, kGroup);
+            const bool fits = static_cast<int64_t>(local) + n_rec_chunks <= rs->cpart;
+            const int first = static_cast<int>(part * rs->cpart + local);
             wave_lds_fence();
             for (int k = lane; k < n_rec_chunks; k += kGroup) {
-                const int64_t e = static_cast<int64_t>(first) + k;
-                if (e < rs->clist_cap)
-                    rs->clist[e] = fits ? make_int2(static_cast<int>(ray), slot_lds[k])
-                                        : make_int2(-1, -1);
+                if (static_cast<int64_t>(local) + k < rs->cpart)
+                    rs->clist[static_cast<int64_t>(first) + k] =
+                        fits ? make_int2(static_cast<int>(ray), slot_lds[k]) : make_int2(-1, -1);
             }
             if (!fits) {
                 rec_ok = false;
@@ -1023,15 +1042,18 @@ __device__ __forceinline__ float march_forward(const GridView<DT>& g, const OptV
         int n_seg = 1;
         if (rec_ok && n_rec_chunks >= kRecSplitMin)
             n_seg = min((n_rec_chunks + kRecSegChunks - 1) / kRecSegChunks, kRecSegs);
-        int base = 0;
+        const int part = static_cast<int>(ray % rs->parts);
+        int local = 0;
         if (lane == 0 && n_seg > 1)
-            base = static_cast<int>(atomicAdd(rs->next_block + 1, static_cast<uint32_t>(n_seg - 1)));
-        base = __shfl(base, 0, kGroup);
-        const bool listed = n_seg > 1 && static_cast<int64_t>(base) + n_seg - 1 <= rs->seg_cap;
+            local = static_cast<int>(atomicAdd(rs->next_block + kRecParts + part,
+                                               static_cast<uint32_t>(n_seg - 1)));
+        local = __shfl(local, 0, kGroup);
+        const bool listed = n_seg > 1 && static_cast<int64_t>(local) + n_seg - 1 <= rs->spart;
         for (int m = 1 + lane; m < n_seg; m += kGroup) {
-            const int64_t e = static_cast<int64_t>(base) + m - 1;
-            if (e < rs->seg_cap)
-                rs->seg_list[e] = listed ? make_int2(static_cast<int>(ray), m) : make_int2(-1, -1);
+            const int64_t l = static_cast<int64_t>(local) + m - 1;
+            if (l < rs->spart)
+                rs->seg_list[part * rs->spart + l] =
+                    listed ? make_int2(static_cast<int>(ray), m) : make_int2(-1, -1);
         }
         if (lane == 0) {
             if (rec_ok) put_header(rs->rec, cur, -1, 0);
@@ -1473,8 +1495,8 @@ __device__ __forceinline__ void replay_backward(const GridView<DT>& g, const Rec
         int l[3] = {0, 0, 0}, lk[8];
         float weight = 0.f, lt_after = 0.f, ws = 0.f, sigma = 0.f;
         if (live) {
-            const float4* q = rs.rec + 6 * (pos + 1 + lane);
-            const float4 a = q[0], b = q[1], c2 = q[2], d = q[3], e = q[4], f = q[5];
+            const float4* q = rec_field0(rs.rec, pos) + lane;
+            const float4 a = q[0], b = q[n], c2 = q[2 * n], d = q[3 * n], e = q[4 * n], f = q[5 * n];
             p[0] = a.x; p[1] = a.y; p[2] = a.z;
             l[0] = __float_as_int(b.x); l[1] = __float_as_int(b.y); l[2] = __float_as_int(b.z);
             weight = b.w;
@@ -1583,8 +1605,12 @@ __global__ __launch_bounds__(kBlock) void k_rec_colours(GridView<DT> g,
     const int lane = threadIdx.x % kGroup;
     const int64_t gid = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
     const int64_t ngroups = static_cast<int64_t>(gridDim.x) * (kBlock / kGroup);
-    const int64_t total = min(static_cast<int64_t>(rs.next_block[2]), rs.clist_cap);
+    const int64_t total = rs.parts * rs.cpart;
     for (int64_t j = gid; j < total; j += ngroups) {
+        const int64_t part = j / rs.cpart;
+        if (j - part * rs.cpart >= min(static_cast<int64_t>(rs.next_block[2 * kRecParts + part]),
+                                       rs.cpart))
+            continue;  // past the partition's entries
         const int2 e = rs.clist[j];
         if (e.x < 0) continue;  // entries of a ray whose list did not fit
         const int64_t ray = e.x, slot = e.y;
@@ -1592,9 +1618,9 @@ __global__ __launch_bounds__(kBlock) void k_rec_colours(GridView<DT> g,
         Chunk c;
         c.live = lane < n;
         c.live_mask = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-        float4* q = rs.rec + 6 * (slot + 1 + lane);
+        float4* q = rec_field0(rs.rec, slot) + lane;
         if (c.live) {
-            const float4 a = q[0], b = q[1], c2 = q[2], d = q[3];
+            const float4 a = q[0], b = q[n], c2 = q[2 * n], d = q[3 * n];
             c.p[0] = a.x;
             c.p[1] = a.y;
             c.p[2] = a.z;
@@ -1621,7 +1647,7 @@ __global__ __launch_bounds__(kBlock) void k_rec_colours(GridView<DT> g,
         chunk_colours<DT, B, true>(g, rc, c, lane, cs, cacc);
         add_chunk_colour(unused, cacc, sum);
         if (c.live) {
-            float* q4 = reinterpret_cast<float*>(q + 4);
+            float* q4 = reinterpret_cast<float*>(q + 4 * n);
             q4[0] = cs[0];
             q4[1] = cs[1];
             q4[2] = cs[2];
@@ -1949,7 +1975,11 @@ SVOX2_REPLAY_KERNEL void k_render_backward_replay(
         if (rs.split[ray] >= 0) max_chunks = kRecSegChunks;
     } else {  // a listed segment m >= 1 of a split ray
         const int64_t e = item - n;
-        if (e >= min(static_cast<int64_t>(rs.next_block[1]), rs.seg_cap)) return;
+        if (e >= rs.parts * rs.spart) return;
+        const int64_t part = e / rs.spart;
+        if (e - part * rs.spart >= min(static_cast<int64_t>(rs.next_block[kRecParts + part]),
+                                       rs.spart))
+            return;  // past the partition's entries
         const int2 sm = rs.seg_list[e];
         if (sm.x < 0) return;  // entries of a ray whose segments did not fit
         ray = sm.x;
@@ -2188,7 +2218,7 @@ size_t fused_workspace_bytes(int64_t n) {
     const int64_t seg = align256(4 * n * (kRecSegs - 1));
     const int64_t split = align256(4 * n);
     const int64_t list = align256(8 * seg_cap_of(n));
-    const int64_t counter = 256;
+    const int64_t counter = align256(4 * 3 * kRecParts);
     const int64_t info = align256(16 * n);
     const int64_t clist = align256(8 * kDeferCap * n);
     const int64_t crgb = align256(16 * kDeferCap * n);
@@ -2228,7 +2258,7 @@ RecStream record_stream(void* ws, int64_t n) {
     rs.seg_cap = seg_cap_of(n);
     p += align256(8 * seg_cap_of(n));
     rs.next_block = reinterpret_cast<uint32_t*>(p);
-    p += 256;
+    p += align256(4 * 3 * kRecParts);
     rs.ray_info = reinterpret_cast<float4*>(p);
     p += align256(16 * n);
     rs.clist = reinterpret_cast<int2*>(p);
@@ -2240,6 +2270,15 @@ RecStream record_stream(void* ws, int64_t n) {
     rs.fixed = kRecFixed;
     rs.overflow_base = n * kRecFixed;
     rs.overflow_slots = n * kRecOverflow;
+    // Counter partitions: one per 1024 rays of batches of 16384 rays or more,
+    // at most kRecParts, so that each partition's share of the overflow
+    // region and lists evens out over its rays; smaller batches share one
+    // counter each (their atomics are few, and their overflow region is too
+    // small to split).
+    rs.parts = n < 16384 ? 1 : static_cast<int>(std::min<int64_t>(n / 1024, kRecParts));
+    rs.opart = rs.overflow_slots / rs.parts;
+    rs.spart = rs.seg_cap / rs.parts;
+    rs.cpart = rs.clist_cap / rs.parts;
     return rs;
 }
 
@@ -2663,12 +2702,14 @@ extern "C" size_t svox2_ray_order_workspace_bytes(int64_t n_rays) {
 }
 
 // Whether the fused call records its samples for a batch of n rays (given a
-// workspace of fused_workspace_bytes(n)).  Records pay off below the ordering
-// threshold: there the backward's rays run in one round and replaying saves
-// each one's re-march.  Slot indices must stay below 2^31 (headers hold them
-// as int32).
+// workspace of fused_workspace_bytes(n)).  Replaying records saves each ray's
+// re-march; it pays up to about two ordering thresholds (measured on one
+// MI355X: 0.780 against 0.855 ms at 65536 rays, 1.41 against 1.20 ms at
+// 131072, where the records' traffic outweighs the re-march).  Larger
+// batches re-march from the chunk log.  Slot indices must stay below 2^31
+// (headers hold them as int32).
 bool fused_records(int64_t n) {
-    return (SVOX2_RECORD_ALL || n < order_min_rays()) &&
+    return (SVOX2_RECORD_ALL || n <= 2 * order_min_rays()) &&
            n * (kRecFixed + kRecOverflow) < INT32_MAX;
 }
 
@@ -2716,7 +2757,7 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
         workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
-                           reinterpret_cast<uint32_t*>(rs.next_block), 3);
+                           reinterpret_cast<uint32_t*>(rs.next_block), 3 * kRecParts);
     }
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;

@@ -77,7 +77,8 @@ def test_512_colours_and_grads(gpu, big, march, dtype):
 
 
 def test_512_chunk_log(gpu, big):
-    """The fused call on a full 65,536-ray batch with the workspace it asks for
+    """The fused call on a 131,072-ray batch (the 65,536 rays twice, above the
+    batch sizes that record their samples) with the workspace it asks for
     (ray order plus the chunk log: the backward evaluates the forward's logged
     chunk placements, the first 16 chunks of each ray, and places the rest
     itself) against the call without a workspace (input order, every chunk
@@ -89,9 +90,9 @@ def test_512_chunk_log(gpu, big):
     import ctypes
     sc, grid, links, o, d, gt = big
     lib = _C.load_library()
-    n = 65536
-    rays = svox2_amd.Rays(o.to(gpu), d.to(gpu))
-    gt_t = gt.to(gpu)
+    n = 2 * 65536
+    rays = svox2_amd.Rays(torch.cat([o, o]).to(gpu), torch.cat([d, d]).to(gpu))
+    gt_t = torch.cat([gt, gt]).to(gpu)
     nbytes = lib.svox2_volume_render_cuvol_fused_workspace_bytes(n)
     order_bytes = lib.svox2_ray_order_workspace_bytes(n)
     rec_bytes = (n * 16 * 36 * 4 + 255) // 256 * 256
@@ -165,11 +166,12 @@ def test_512_ordered_forward(gpu, big):
 
 
 def test_512_fused_matches_autograd(gpu, big):
-    """The fused call on a full 65,536-ray batch (its backward adds each
-    chunk's samples into corner slots in parallel, for chunks narrow enough)
-    against the autograd path on the same rays (volume_render, MSE loss,
-    backward; its scatter adds samples one by one): colours bit-exact and
-    gradients within 1e-5 of each other (atomic order differs)."""
+    """The fused call on a full 65,536-ray batch (it records its samples,
+    forms their colours chunk-parallel and replays the records, with the
+    counters split over 64 partitions) against the autograd path on the same
+    rays (volume_render, MSE loss, backward; it re-marches every ray):
+    colours bit-exact and gradients within 1e-5 of each other (atomic order
+    differs)."""
     sc, grid, links, o, d, gt = big
     n = 65536
     rays = svox2_amd.Rays(o.to(gpu), d.to(gpu))
