@@ -217,8 +217,9 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
 
 /* Device workspace bytes with which svox2_volume_render_cuvol_fused runs
  * fastest for an n_rays batch (0 for an empty batch): the record stream
- * and chunk list (about 4.8 KiB per ray) below the ordering threshold, the ray-ordering
- * workspace plus the chunk log (about 2.3 KiB per ray) at or above it.  No
+ * and chunk list (about 4.8 KiB per ray) up to twice the ordering threshold
+ * (81920 rays on a 256-CU MI355X), the ray-ordering workspace plus the chunk
+ * log (about 2.3 KiB per ray) above it.  No
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
 
