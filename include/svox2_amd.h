@@ -160,6 +160,43 @@ int svox2_volume_render_sigma_thresh(const svox2_grid_t* grid, const float* orig
                                      int64_t n_rays, const svox2_render_options_t* opt,
                                      float sigma_thresh, float* depth_out, void* stream);
 
+/* nvol backend forward: the samples of svox2_volume_render_cuvol (same
+ * placement and densities, no transmittance stop, no opaque last sample)
+ * composited by accumulated opacity A = min(A + 1 - exp(-ws sigma), 1) over
+ * the samples with sigma > sigma_thresh, colour weight = the increase of A,
+ * stopping where A reaches 1, background (1 - A) * background_brightness;
+ * rgb_out [n][3].
+ * Replaces volume_render_nvol (svox2.cpp:32, bound at :85;
+ * render_lerp_kernel_nvol.cu:411). */
+int svox2_volume_render_nvol(const svox2_grid_t* grid, const float* origins,
+                             const float* dirs, const float* ray_clip,
+                             int64_t n_rays, const svox2_render_options_t* opt,
+                             float* rgb_out, void* stream);
+
+/* Backward of svox2_volume_render_nvol given dL/drgb (grad_out [n][3]),
+ * accumulated into grads (the reference's two passes, including their density
+ * terms for samples at or below sigma_thresh before the saturating one).
+ * color_cache is accepted and unread, as in the reference.
+ * Replaces volume_render_nvol_backward (svox2.cpp:33, bound at :86;
+ * render_lerp_kernel_nvol.cu:431). */
+int svox2_volume_render_nvol_backward(const svox2_grid_t* grid, const float* origins,
+                                      const float* dirs, const float* ray_clip,
+                                      int64_t n_rays, const svox2_render_options_t* opt,
+                                      const float* grad_out, const float* color_cache,
+                                      const svox2_grads_t* grads, void* stream);
+
+/* svox2_volume_render_nvol into rgb_out, then its backward for the MSE loss
+ * against rgb_gt (dL/drgb = (rgb - rgb_gt) * 2 / (3 n)) plus the Cauchy
+ * sparsity term; beta_loss is accepted and unused, as in the reference.
+ * Replaces volume_render_nvol_fused (svox2.cpp:35, bound at :87;
+ * render_lerp_kernel_nvol.cu:461). */
+int svox2_volume_render_nvol_fused(const svox2_grid_t* grid, const float* origins,
+                                   const float* dirs, const float* ray_clip,
+                                   int64_t n_rays, const svox2_render_options_t* opt,
+                                   const float* rgb_gt, float beta_loss,
+                                   float sparsity_loss, float* rgb_out,
+                                   const svox2_grads_t* grads, void* stream);
+
 /* Backward of svox2_volume_render_cuvol given dL/drgb (grad_out [n][3]) and
  * the forward colours (color_cache [n][3]); accumulates into grads.  With the
  * forward's ray_cost_out (ray_cost) and a device workspace of

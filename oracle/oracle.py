@@ -19,6 +19,9 @@ reference lines it follows:
   grid passed explicitly; the CPU baseline and a cross-check of render_fp64.
 * ``depth_fp64`` (render_lerp_kernel_cuvol.cu:122-225), the depth
   renders over the same sample list.
+* ``nvol_fp64`` / ``nvol_grads_fp64`` (render_lerp_kernel_nvol.cu:31-292),
+  the nvol backend's compositing (differentiable) and its two-pass backward
+  (restated), over the kernel's sample list without the transmittance stop.
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
   (loss_kernel.cu:178-246,558-604), ``lumisphere_tv_grad_sparse``
@@ -518,6 +521,163 @@ def _blocks(grid: Grid, origins, dirs, opt: Opts, clip, samples, max_samples: in
     else:
         for r0, r1, sub in _ray_chunks(samples, np.asarray(origins).shape[0], max_samples):
             yield (r0, r1, *post(sub))
+
+
+def nvol_samples(grid: Grid, origins, dirs, opt: Opts, clip=None) -> dict:
+    """The nvol kernel's samples: the colour march's placement to tmax
+    (render_lerp_kernel_nvol.cu:52-71), without an opaque last sample; flag bit
+    0 is the FP32 decision sigma > sigma_thresh."""
+    import dataclasses
+    return march().samples(grid, origins, dirs, dataclasses.replace(opt, last_sample_opaque=False),
+                           clip=clip)
+
+
+def _nvol_sample_terms(grid: Grid, dirs, smp: dict, density, sh):
+    """Per-sample FP64 density (S,), colour + 0.5 before clamping (S, 3),
+    corner links (S, 8) with their validity and trilinear weights, and the SH
+    basis per ray."""
+    B = grid.basis_dim
+    lk = corner_gather(grid.links, smp["l"]).astype(np.int64)
+    rows = density.shape[0]
+    valid = (lk >= 0) & (lk < rows)
+    lk0 = np.where(valid, lk, 0)
+    dt = density.dtype if isinstance(density, torch.Tensor) else torch.float64
+    w = corner_weights(torch.from_numpy(smp["p"]).to(torch.float64)) * \
+        torch.from_numpy(valid).to(torch.float64)
+    basis = eval_sh_bases(B, torch.from_numpy(np.asarray(dirs, np.float32)).to(torch.float64))
+    ray = torch.from_numpy(smp["ray"].astype(np.int64))
+    lkt = torch.from_numpy(lk0)
+    sigma = (w * density[lkt]).sum(1)
+    shs = (w[..., None] * sh[lkt]).sum(1)
+    col = (shs.view(-1, 3, B) * basis[ray][:, None, :]).sum(-1) + 0.5
+    return sigma, col, lk, valid, w, basis, dt
+
+
+def nvol_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor, sh: torch.Tensor,
+              clip=None, samples=None) -> torch.Tensor:
+    """FP64 colours (n, 3) of the nvol compositing (render_lerp_kernel_nvol.cu:
+    31-107) over the kernel's exact samples, differentiable in ``density``
+    (R,) and ``sh`` (R, C): over the samples with FP32 sigma > sigma_thresh,
+    A = min(A + 1 - exp(-ws sigma), 1), weight = the increase of A, colour =
+    max(sh . basis + 0.5, 0); after A reaches 1 nothing is added; the
+    background adds (1 - A) bg."""
+    origins = np.asarray(origins, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    n = origins.shape[0]
+    smp = nvol_samples(grid, origins, dirs, opt, clip) if samples is None else samples
+    bg = float(opt.background_brightness)
+    S = smp["ray"].size
+    if S == 0:
+        return torch.full((n, 3), bg, dtype=torch.float64) + 0 * (density.sum() + sh.sum())
+    sigma, col, _, _, _, _, _ = _nvol_sample_terms(grid, dirs, smp, density, sh)
+    ws = torch.from_numpy(smp["ws"].astype(np.float64))
+    contrib = torch.from_numpy((smp["flag"] & 1).astype(bool))
+    a = torch.where(contrib, 1.0 - torch.exp(-ws * sigma), torch.zeros_like(sigma))
+    ray = torch.from_numpy(smp["ray"].astype(np.int64))
+    first = torch.ones(S, dtype=torch.bool)
+    first[1:] = ray[1:] != ray[:-1]
+    start = torch.nonzero(first).flatten()
+    seg = torch.cumsum(first.long(), 0) - 1
+    pos = torch.arange(S) - start[seg]
+    L = int(pos.max()) + 1
+    rays_with = ray[start]
+    A = torch.zeros((start.numel(), L), dtype=torch.float64).index_put((seg, pos), a)
+    alpha = torch.zeros(start.numel(), dtype=torch.float64)
+    weights = []
+    for k in range(L):
+        # a saturated ray takes no more samples (the kernel breaks out of its
+        # loop): leave its opacity as it is rather than clamp it again
+        sat = (alpha >= 1.0).detach()
+        na = torch.where(sat, alpha, torch.clamp(alpha + A[:, k], max=1.0))
+        weights.append(na - alpha)
+        alpha = na
+    W = torch.stack(weights, 1)[seg, pos]
+    rgb = torch.zeros((n, 3), dtype=torch.float64).index_add(
+        0, ray, (W * contrib.to(torch.float64))[:, None] * torch.clamp_min(col, 0.0))
+    alpha_r = torch.zeros(n, dtype=torch.float64).index_add(0, rays_with, alpha)
+    return rgb + (1.0 - alpha_r)[:, None] * bg
+
+
+def nvol_grads_fp64(grid: Grid, origins, dirs, opt: Opts, grad_out, clip=None,
+                    sparsity: float = 0.0, samples=None):
+    """(grad_density (R,), grad_sh (R, C), mask (R,)) of the reference's nvol
+    backward (trace_ray_nvol_backward, render_lerp_kernel_nvol.cu:109-292)
+    restated in FP64, sample by sample.  Pass 1 over the contributing samples
+    up to the saturating one: SH coefficient j of channel c gets
+    weight [colour_c >= 0] g_c basis_j (:171-199); unless the sample saturates,
+    the density gets ws e T + sparsity 4 sigma / (1 + 2 sigma^2) with e =
+    exp(-ws sigma), T = (t_0 + t_2) + t_1, t_c = g_c max(colour_c, 0) (:201-220);
+    the saturating sample's T, or bg ((g_0 + g_1) + g_2) if none saturates, is
+    L (:220-231).  Pass 2 if L != 0, over every sample up to the saturating
+    one: its own opacity accumulates, the pass stops at the sample that
+    reaches 1, and the others' densities get -ws e L (:232-291).  Every
+    density term marks its corners' rows."""
+    origins = np.asarray(origins, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    n = origins.shape[0]
+    density = torch.as_tensor(np.asarray(grid.density, np.float64).reshape(-1))
+    sh = torch.as_tensor(np.asarray(grid.sh, np.float64))
+    R, C = sh.shape
+    B = grid.basis_dim
+    gd = np.zeros(R)
+    gs = np.zeros((R, C))
+    mask = np.zeros(R, bool)
+    smp = nvol_samples(grid, origins, dirs, opt, clip) if samples is None else samples
+    if smp["ray"].size == 0:
+        return gd, gs, mask
+    sigma, col, lk, valid, w, basis, _ = _nvol_sample_terms(grid, dirs, smp, density, sh)
+    sigma, col, w, basis = sigma.numpy(), col.numpy(), w.numpy(), basis.numpy()
+    ws = smp["ws"].astype(np.float64)
+    contrib = (smp["flag"] & 1).astype(bool)
+    g = np.asarray(grad_out, np.float64)
+    bg = float(opt.background_brightness)
+    bounds = np.concatenate([[0], np.cumsum(np.bincount(smp["ray"], minlength=n))])
+
+    def add_density(i, v):
+        for k in range(8):
+            if valid[i, k]:
+                gd[lk[i, k]] += w[i, k] * v
+                mask[lk[i, k]] = True
+
+    for r in range(n):
+        idx = range(bounds[r], bounds[r + 1])
+        alpha, last, sat = 0.0, 0.0, None
+        for i in idx:
+            if not contrib[i]:
+                continue
+            e = math.exp(-ws[i] * sigma[i])
+            na = min(alpha + 1.0 - e, 1.0)
+            wgt = na - alpha
+            alpha = na
+            cl = np.maximum(col[i], 0.0)
+            tc = cl * g[r]
+            tot = (tc[0] + tc[2]) + tc[1]
+            gcom = wgt * (cl == col[i]) * g[r]
+            val = (gcom[:, None] * basis[r][None, :]).reshape(C)
+            for k in range(8):
+                if valid[i, k]:
+                    gs[lk[i, k]] += w[i, k] * val
+            if na < 1.0:
+                v = ws[i] * e * tot
+                if sparsity > 0.0:
+                    v += sparsity * (4.0 * sigma[i] / (1.0 + 2.0 * sigma[i] * sigma[i]))
+                add_density(i, v)
+            else:
+                last, sat = tot, i
+                break
+        if sat is None:
+            last = bg * ((g[r][0] + g[r][1]) + g[r][2])
+        if last != 0.0:
+            alpha2 = 0.0
+            for i in idx:
+                if sat is not None and i > sat:
+                    break
+                e = math.exp(-ws[i] * sigma[i])
+                alpha2 = min(alpha2 + 1.0 - e, 1.0)
+                if alpha2 >= 1.0:
+                    break
+                add_density(i, -ws[i] * e * last)
+    return gd, gs, mask
 
 
 def depth_fp64(grid: Grid, origins, dirs, opt: Opts, sigma_thresh: Optional[float] = None,

@@ -1745,6 +1745,230 @@ SVOX2_FWD_KERNEL void k_render_depth(GridView<DT> g, OptView o,
     if (lane == 0) depth_out[ray] = depth;
 }
 
+// ---------------------------------------------------------------------------
+// The nvol backend (render_lerp_kernel_nvol.cu): the colour march's sample
+// placement and densities, composited by accumulated opacity instead of
+// transmittance.  Each contributing sample (sigma > sigma_thresh) adds
+// 1 - exp(-ws sigma) to the ray's opacity, A = min(A + 1 - exp(-ws sigma), 1)
+// (:84-88), and weighs its colour by the increase of A; the ray stops at the
+// sample where A reaches 1 (:93); the background gets (1 - A) bg (:98-100).
+// There is no transmittance stop and no opaque last sample.
+//
+// The backward follows the reference's two passes (trace_ray_nvol_backward,
+// :109-292).  Pass 1, per contributing sample up to the saturating one: the
+// colour terms weight * [colour >= 0] * g_c * basis_j into the SH rows, and,
+// unless the sample saturated A, ws exp(-ws sigma) * sum_c g_c max(colour_c,
+// 0) (plus the sparsity term) into the densities.  The saturating sample's
+// sum_c g_c max(colour_c, 0), or bg * (g_0 + g_1 + g_2) when A never reached
+// 1, is the "last colour" L.  Pass 2, if L != 0: every evaluated sample up to
+// the saturating one, whatever its density, accumulates its own opacity the
+// same way, stops at the one that reaches 1, and adds -ws exp(-ws sigma) L to
+// the densities.  Density terms mark their corners' rows in the mask.
+// ---------------------------------------------------------------------------
+
+// Opacity accumulation over a chunk's samples in lane order, one per lane
+// where acc is set (e = exp(-ws sigma) of its sample): returns the lane's
+// weight (the increase of alpha at its sample) and sets sat to the first lane
+// whose sample brings alpha to 1 (32 if none).  alpha is the group's running
+// opacity, the same in every lane of the group.
+__device__ __forceinline__ float nvol_accumulate(int lane, bool acc, float e, float& alpha,
+                                                 int& sat) {
+    const uint32_t m = group_ballot(acc);
+    const int n_rep = wave_last_set(acc);
+    float my_w = 0.f;
+    sat = 32;
+    for (int j = 0; j < n_rep; ++j) {
+        const float ej = group_lane(e, j);
+        if (((m >> j) & 1u) && sat == 32) {
+            const float na = fminf(alpha + 1.f - ej, 1.f);
+            if (lane == j) my_w = na - alpha;
+            alpha = na;
+            if (alpha >= 1.f) sat = j;
+        }
+    }
+    return my_w;
+}
+
+// nvol forward of one ray per 32-lane group: rgb_out [n][3].
+template <typename DT, int B>
+SVOX2_FWD_KERNEL void k_render_nvol(GridView<DT> g, OptView o, const float* __restrict__ origins,
+                                    const float* __restrict__ dirs,
+                                    const float* __restrict__ clip, int64_t n,
+                                    float* __restrict__ rgb_out) {
+    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    const RayCtx<B> rc(wd);
+    Ray r;
+    setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
+    float alpha = 0.f, total[3] = {0.f, 0.f, 0.f};
+    if (r.tmin <= r.tmax) {
+        float t = r.tmin;
+        bool more = true;
+        int guard = 0;
+        while (more && ++guard < kMaxIters) {
+            Chunk c;
+            march_chunk(g, o, r, lane, t, 0.f, c);
+            const float e = c.contrib ? __expf(-c.ws * c.sigma) : 1.f;
+            int sat;
+            c.weight = nvol_accumulate(lane, c.contrib, e, alpha, sat);
+            c.live = c.contrib && lane <= sat;
+            c.live_mask = group_ballot(c.live);
+            if (c.live_mask != 0u) {
+                float cs[3], cacc[3] = {0.f, 0.f, 0.f}, sum[3];
+                chunk_colours<DT, B, false>(g, rc, c, lane, cs, cacc);
+                add_chunk_colour(total, cacc, sum);
+            }
+            if (sat < 32) break;
+            t = c.next_t;
+            more = c.more;
+        }
+    }
+    if (lane < 3) rgb_out[3 * ray + lane] = pick3(total, lane) + (1.f - alpha) * o.bg;
+}
+
+// Adds v * (trilinear weight of corner k at p) into column col of the rows
+// of the corners with a row.
+__device__ __forceinline__ void nvol_scatter(float* base, int64_t stride, int col, const int* lk,
+                                             const float* p, float v, int rows) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (static_cast<uint32_t>(lk[k]) < static_cast<uint32_t>(rows))
+            atomic_add_f32(base + lk[k] * stride + col, corner_weight(k, p) * v);
+}
+
+// nvol backward of one ray per 32-lane group.  grad_in: dL/drgb [n][3], or
+// with rgb_residual the targets (the fused MSE step: g = (color_cache -
+// grad_in) * norm).
+template <typename DT, int B>
+__global__ __launch_bounds__(kBlock) void k_render_nvol_backward(
+        GridView<DT> g, OptView o, const float* __restrict__ origins,
+        const float* __restrict__ dirs, const float* __restrict__ clip, int64_t n,
+        const float* __restrict__ grad_in, const float* __restrict__ color_cache,
+        int rgb_residual, float norm, float sparsity, GradOut go) {
+    constexpr int C = 3 * B;
+    const int64_t ray = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGroup;
+    const int lane = threadIdx.x % kGroup;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    const RayCtx<B> rc(wd);
+    Ray r;
+    setup_ray(g, o, wo, wd, clip ? clip + 2 * ray : nullptr, r);
+    if (r.tmin > r.tmax) return;
+    float gq[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        gq[a] = rgb_residual ? (color_cache[3 * ray + a] - grad_in[3 * ray + a]) * norm
+                             : grad_in[3 * ray + a];
+    // This lane's SH coefficient (lane < C): its channel and basis value.
+    float basis_j = 0.f;
+#pragma unroll
+    for (int j = 0; j < B; ++j) basis_j = (lane < C && lane % B == j) ? rc.basis[j] : basis_j;
+    const int my_c = lane < C ? lane / B : 0;
+
+    // Pass 1.
+    float alpha = 0.f, last = 0.f, t_sat = r.tmax;
+    bool saturated = false;
+    {
+        float t = r.tmin;
+        bool more = true;
+        int guard = 0;
+        while (more && ++guard < kMaxIters) {
+            Chunk c;
+            march_chunk(g, o, r, lane, t, 0.f, c);
+            const float e = c.contrib ? __expf(-c.ws * c.sigma) : 1.f;
+            int sat;
+            c.weight = nvol_accumulate(lane, c.contrib, e, alpha, sat);
+            c.live = c.contrib && lane <= sat;
+            c.live_mask = group_ballot(c.live);
+            float cs[3] = {0.f, 0.f, 0.f};
+            if (c.live_mask != 0u) chunk_colours<DT, B, true>(g, rc, c, lane, cs, nullptr);
+            // The sample's gradient scalars (live lanes).
+            float gc[3] = {0.f, 0.f, 0.f}, tot = 0.f;
+            if (c.live) {
+                float tc[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float cl = fmaxf(cs[a], 0.f);
+                    tc[a] = cl * gq[a];
+                    gc[a] = c.weight * (cl == cs[a] ? 1.f : 0.f) * gq[a];
+                }
+                tot = (tc[0] + tc[2]) + tc[1];
+                if (lane != sat) {  // not the saturating sample: its density term
+                    float gs = c.ws * e * tot;
+                    if (sparsity > 0.f)
+                        gs += sparsity * (4.f * c.sigma / (1.f + 2.f * (c.sigma * c.sigma)));
+                    if (go.gd) nvol_scatter(go.gd, go.gds, 0, c.lk, c.p, gs, g.rows);
+                    if (go.mask) {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if (static_cast<uint32_t>(c.lk[k]) < static_cast<uint32_t>(g.rows))
+                                go.mask[c.lk[k]] = 1;
+                    }
+                }
+            }
+            // SH terms, sample by sample: lane = coefficient.
+            uint32_t todo = c.live_mask;
+            while (todo != 0u) {
+                const int s = __ffs(todo) - 1;
+                todo &= todo - 1u;
+                float p[3];
+                int lk[8];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) p[a] = group_bcast(c.p[a], s);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) lk[k] = group_bcast(c.lk[k], s);
+                float v = group_bcast(gc[0], s);
+                const float g1 = group_bcast(gc[1], s), g2 = group_bcast(gc[2], s);
+                v = my_c == 0 ? v : (my_c == 1 ? g1 : g2);
+                if (lane < C && v != 0.f && go.gs)
+                    nvol_scatter(go.gs, go.gss, lane, lk, p, basis_j * v, g.rows);
+            }
+            if (sat < 32) {
+                last = group_bcast(tot, sat);
+                t_sat = group_bcast(c.t, sat);
+                saturated = true;
+                break;
+            }
+            t = c.next_t;
+            more = c.more;
+        }
+    }
+    if (!saturated) last = o.bg * ((gq[0] + gq[1]) + gq[2]);
+    if (last == 0.f || go.gd == nullptr) return;
+
+    // Pass 2: every evaluated sample up to the saturating one.
+    Ray r2 = r;
+    r2.tmax = t_sat;
+    float alpha2 = 0.f;
+    float t = r2.tmin;
+    bool more = true;
+    int guard = 0;
+    while (more && ++guard < kMaxIters) {
+        Chunk c;
+        march_chunk(g, o, r2, lane, t, 0.f, c);
+        const bool ev = lane < c.n_eval;
+        const float e = ev ? __expf(-c.ws * c.sigma) : 1.f;
+        int sat;
+        nvol_accumulate(lane, ev, e, alpha2, sat);
+        if (ev && lane < sat) {
+            nvol_scatter(go.gd, go.gds, 0, c.lk, c.p, -c.ws * e * last, g.rows);
+            if (go.mask) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (static_cast<uint32_t>(c.lk[k]) < static_cast<uint32_t>(g.rows))
+                        go.mask[c.lk[k]] = 1;
+            }
+        }
+        if (sat < 32) break;
+        t = c.next_t;
+        more = c.more;
+    }
+}
+
 // World ray of pixel (ix, iy) (cam2world_ray, render_util.cuh:526-544; NDC
 // warp world2ndc :504-524 when ndc_coeffx > 0).
 __device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, int iy,
@@ -2566,6 +2790,106 @@ extern "C" int svox2_volume_render_sigma_thresh(const svox2_grid_t* grid, const 
                                                 void* stream) {
     return render_depth(grid, origins, dirs, ray_clip, n_rays, opt, kDepthSigma, sigma_thresh,
                         depth_out, stream);
+}
+
+// nvol backend entry points (render_lerp_kernel_nvol.cu:411-502): the
+// colour march's placement without its transmittance stop or opaque last
+// sample.
+namespace svox2amd {
+namespace {
+
+int nvol_opt(const svox2_grid_t* grid, const float* origins, const float* dirs, int64_t n_rays,
+             const svox2_render_options_t* opt, OptView& o) {
+    int st = check_grid(grid, true);
+    if (st) return st;
+    if ((st = make_opt(opt, o))) return st;
+    if ((st = check_rays(origins, dirs, n_rays))) return st;
+    o.log_stop = -INFINITY;
+    o.last_opaque = 0;
+    return ok();
+}
+
+int nvol_backward(const svox2_grid_t* grid, const OptView& o, const float* origins,
+                  const float* dirs, const float* ray_clip, int64_t n_rays, const float* grad_in,
+                  const float* color_cache, int rgb_residual, float norm, float sparsity,
+                  const GradOut& go, hipStream_t s) {
+    const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    return dispatch(grid, [&](auto g, auto b, auto dtp) {
+        constexpr int B = decltype(b)::value;
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        hipLaunchKernelGGL((k_render_nvol_backward<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g,
+                           o, origins, dirs, ray_clip, n_rays, grad_in, color_cache,
+                           rgb_residual, norm, sparsity, go);
+    });
+}
+
+}  // namespace
+}  // namespace svox2amd
+
+extern "C" int svox2_volume_render_nvol(const svox2_grid_t* grid, const float* origins,
+                                        const float* dirs, const float* ray_clip,
+                                        int64_t n_rays, const svox2_render_options_t* opt,
+                                        float* rgb_out, void* stream) {
+    OptView o;
+    int st = nvol_opt(grid, origins, dirs, n_rays, opt, o);
+    if (st) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(rgb_out != nullptr, "rgb_out is NULL");
+    const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
+    const hipStream_t s = as_stream(stream);
+    st = dispatch(grid, [&](auto g, auto b, auto dtp) {
+        constexpr int B = decltype(b)::value;
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        hipLaunchKernelGGL((k_render_nvol<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
+                           origins, dirs, ray_clip, n_rays, rgb_out);
+    });
+    if (st) return st;
+    return check_launch("volume_render_nvol");
+}
+
+extern "C" int svox2_volume_render_nvol_backward(const svox2_grid_t* grid, const float* origins,
+                                                 const float* dirs, const float* ray_clip,
+                                                 int64_t n_rays,
+                                                 const svox2_render_options_t* opt,
+                                                 const float* grad_out,
+                                                 const float* color_cache,
+                                                 const svox2_grads_t* grads, void* stream) {
+    OptView o;
+    int st = nvol_opt(grid, origins, dirs, n_rays, opt, o);
+    if (st) return st;
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(grad_out != nullptr, "grad_out is NULL");
+    (void)color_cache;  // the reference's backward does not read it either
+    if ((st = nvol_backward(grid, o, origins, dirs, ray_clip, n_rays, grad_out, nullptr, 0, 1.f,
+                            0.f, go, as_stream(stream))))
+        return st;
+    return check_launch("volume_render_nvol_backward");
+}
+
+extern "C" int svox2_volume_render_nvol_fused(const svox2_grid_t* grid, const float* origins,
+                                              const float* dirs, const float* ray_clip,
+                                              int64_t n_rays,
+                                              const svox2_render_options_t* opt,
+                                              const float* rgb_gt, float beta_loss,
+                                              float sparsity_loss, float* rgb_out,
+                                              const svox2_grads_t* grads, void* stream) {
+    (void)beta_loss;  // not supported by the reference's nvol (render_lerp_kernel_nvol.cu:466)
+    OptView o;
+    int st = nvol_opt(grid, origins, dirs, n_rays, opt, o);
+    if (st) return st;
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
+    if ((st = svox2_volume_render_nvol(grid, origins, dirs, ray_clip, n_rays, opt, rgb_out, stream)))
+        return st;
+    const float norm = 2.f / static_cast<float>(3 * n_rays);
+    if ((st = nvol_backward(grid, o, origins, dirs, ray_clip, n_rays, rgb_gt, rgb_out, 1, norm,
+                            sparsity_loss, go, as_stream(stream))))
+        return st;
+    return check_launch("volume_render_nvol_fused");
 }
 
 extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const float* origins,
