@@ -72,6 +72,19 @@ typedef struct svox2_grid {
      * The march then decides skips and empty cells from it and reads the
      * links only for cells with an occupied corner.  Results are the same. */
     const uint16_t* cell_map;
+    /* Optional MSI background (NULL background_links = none), the
+     * background_links / background_data of SparseGridSpec
+     * (data_spec.hpp:26-27; SparseGrid, svox2.py:497-520): links
+     * [2 reso][reso] int32 (-1 = empty) into rows of background_data
+     * [background_rows][nlayers][4] float32, channels (r, g, b, sigma), the
+     * colours scaled by the SH DC factor.  nlayers >= 2.  The cuvol entry points
+     * composite it behind the grid (render_lerp_kernel_cuvol.cu:386-607);
+     * the nvol, depth and stats entry points ignore it, as the reference's do. */
+    const int32_t* background_links;
+    const float* background_data;
+    int64_t background_rows;
+    int32_t background_reso;
+    int32_t background_nlayers;
 } svox2_grid_t;
 
 /* RenderOptions (svox2/csrc/include/data_spec.hpp:115-135). */
@@ -101,6 +114,11 @@ typedef struct svox2_grads {
     uint8_t* mask_out;       /* rows, set to 1 for every row a sample touched */
     int64_t grad_density_stride; /* elements between rows (0 = 1) */
     int64_t grad_sh_stride;      /* elements between rows (0 = sh_data_dim) */
+    /* Background gradients (grids with a background): [background_rows]
+     * [nlayers][4], and the mask [background_rows][nlayers] of the layers a
+     * background density term touched. */
+    float* grad_background_out;
+    uint8_t* mask_background_out;
 } svox2_grads_t;
 
 /* Forward render of n_rays rays; rgb_out [n][3].  ray_clip, if not NULL, is
@@ -108,7 +126,9 @@ typedef struct svox2_grads {
  * narrows each ray's [tmin, tmax].  log_transmit_out, if not NULL, receives the
  * final log transmittance per ray.  ray_cost_out, if not NULL, receives each
  * ray's march chunk count, which svox2_volume_render_cuvol_backward can use
- * to order its rays.
+ * to order its rays.  A grid with a background needs log_transmit_out: it
+ * receives the grid's (foreground's) final log transmittance, behind which
+ * the background is composited.
  * Replaces volume_render_cuvol (svox2.cpp:17, bound at :78; render_lerp_kernel_cuvol.cu:918). */
 int svox2_volume_render_cuvol(const svox2_grid_t* grid, const float* origins,
                               const float* dirs, const float* ray_clip,
@@ -130,7 +150,8 @@ int svox2_volume_render_cuvol_ordered(const svox2_grid_t* grid, const float* ori
                                       uint32_t* ray_cost_out, void* workspace,
                                       size_t workspace_bytes, void* stream);
 
-/* Forward render of a pinhole image; rgb_out [height][width][3].
+/* Forward render of a pinhole image; rgb_out [height][width][3].  A grid
+ * with a background needs log_transmit_out ([height][width] floats).
  * Replaces volume_render_cuvol_image (svox2.cpp:18, bound at :79; render_lerp_kernel_cuvol.cu:960). */
 int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
                                     const svox2_camera_t* cam,
@@ -202,7 +223,11 @@ int svox2_volume_render_nvol_fused(const svox2_grid_t* grid, const float* origin
  * forward's ray_cost_out (ray_cost) and a device workspace of
  * svox2_ray_order_workspace_bytes(n_rays) bytes, batches of at least four
  * times the rays the device holds resident are processed in descending order
- * of cost (same results); otherwise, or with NULLs, in input order.
+ * of cost (same results); otherwise, or with NULLs, in input order.  A grid
+ * with a background needs a workspace of svox2_background_workspace_bytes(
+ * n_rays) bytes (it includes the ordering workspace): the call renders the
+ * grid's own colours and transmittance into it, which the background's
+ * gradients start from.
  * Replaces volume_render_cuvol_backward (svox2.cpp:20, bound at :80; render_lerp_kernel_cuvol.cu:1010). */
 int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                        const float* origins, const float* dirs,
@@ -235,6 +260,9 @@ int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
  *   for batches at or above it the forward also logs where it placed the
  *   samples of each ray's first 16 chunks, and the backward evaluates those
  *   chunks from the log instead of placing them again.
+ * A grid with a background needs a workspace of at least
+ * svox2_background_workspace_bytes(n_rays) bytes (the fused workspace size
+ * is never less); the call then re-marches every ray (no records, no log).
  * Replaces volume_render_cuvol_fused (svox2.cpp:22, bound at :81; render_lerp_kernel_cuvol.cu:1070). */
 int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                     const float* origins, const float* dirs,
@@ -259,6 +287,13 @@ size_t svox2_ray_order_workspace_bytes(int64_t n_rays);
  * log (about 2.3 KiB per ray) above it.  No
  * counterpart in the reference. */
 size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays);
+
+/* Device workspace bytes svox2_volume_render_cuvol_backward and
+ * svox2_volume_render_cuvol_fused need for an n_rays batch over a grid with
+ * a background: the ray-ordering workspace, then 16 bytes per ray (the
+ * grid's own colours or the background's, and log transmittance).  No
+ * counterpart in the reference (it allocates these per call). */
+size_t svox2_background_workspace_bytes(int64_t n_rays);
 
 /* svox2_volume_render_cuvol_fused without a caller workspace and without any
  * device allocation: the workspace is a fixed 40 MiB device array inside the
@@ -388,6 +423,23 @@ int svox2_lumisphere_tv_grad_sparse(const svox2_grid_t* grid,
                                     float scale, float ndc_coeffx, float ndc_coeffy,
                                     float dir_factor, const svox2_grads_t* grads,
                                     void* stream);
+
+/* Total variation of an MSI background: for each of the n_cells flat indices
+ * ((x * reso + y) * nlayers + z over the [2 reso][reso] links and nlayers
+ * layers) and each of the nchannels channels, the differences to the cell's
+ * +x and +y neighbours (wrapping around) and to layer z + 1 (a missing layer
+ * counts as the cell's own value for colours and as 0 for the last channel,
+ * sigma), their normalised gradient (axis factors 2 reso / 256, reso / 256,
+ * nlayers / 256; scale, or scale_last for the last channel, over n_cells)
+ * accumulated into grad_msi [msi_rows][nlayers][nchannels]; mask_out (NULL
+ * allowed, [msi_rows][nlayers]) marks every layer that receives a nonzero
+ * term.  Links < 0 read as 0 and receive nothing.
+ * Replaces msi_tv_grad_sparse (svox2.cpp:65, bound at :100;
+ * loss_kernel.cu:606). */
+int svox2_msi_tv_grad_sparse(const int32_t* links, int32_t reso, const float* msi,
+                             int64_t msi_rows, int32_t nlayers, int32_t nchannels,
+                             const int32_t* rand_cells, int64_t n_cells, uint8_t* mask_out,
+                             float scale, float scale_last, float* grad_msi, void* stream);
 
 /* RMSprop update of data (rows x dim, float32 or bf16 storage) with float32
  * rms and grad; grad is zeroed where updated.  Row strides (elements, 0 =

@@ -22,6 +22,9 @@ reference lines it follows:
 * ``nvol_fp64`` / ``nvol_grads_fp64`` (render_lerp_kernel_nvol.cu:31-292),
   the nvol backend's compositing (differentiable) and its two-pass backward
   (restated), over the kernel's sample list without the transmittance stop.
+* ``background_fp64`` / ``render_bg_fp64`` (render_lerp_kernel_cuvol.cu:
+  386-607), the MSI background behind the grid's log transmittance, and
+  ``msi_tv_grad_sparse`` (loss_kernel.cu:248-333,606-641), its TV.
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
   (loss_kernel.cu:178-246,558-604), ``lumisphere_tv_grad_sparse``
@@ -395,7 +398,8 @@ def _composite(grid: Grid, dirs: np.ndarray, opt: Opts, density: torch.Tensor,
         if extra is not None:
             z = torch.zeros(n, dtype=dt)
             extra.update(depth=z, t_end=torch.ones(n, dtype=dt), sigma=torch.zeros(0, dtype=dt),
-                         counted=torch.zeros(0, dtype=torch.bool))
+                         counted=torch.zeros(0, dtype=torch.bool),
+                         stopped=torch.zeros(n, dtype=torch.bool))
         return torch.full((n, 3), bg, dtype=dt) + 0 * (density.sum() + sh.sum())
     basis = eval_sh_bases(B, torch.from_numpy(dirs).to(dt))  # at the raw dirs, as the kernel
     lk = smp["lk"] if "lk" in smp else corner_gather(grid.links, smp["l"])
@@ -441,7 +445,8 @@ def _composite(grid: Grid, dirs: np.ndarray, opt: Opts, density: torch.Tensor,
     depth = torch.zeros(n, dtype=dt).index_add(0, ray, pcnt * alive.to(dt))
     t_end = torch.exp(-depth) * (~ray_stopped).to(dt)
     if extra is not None:
-        extra.update(depth=depth, t_end=t_end, sigma=sigma, counted=(alive & contrib))
+        extra.update(depth=depth, t_end=t_end, sigma=sigma, counted=(alive & contrib),
+                     stopped=ray_stopped)
     return rgb + t_end[:, None] * bg
 
 
@@ -483,7 +488,7 @@ def _compact_rows(grid: Grid, smp: dict, rows: int):
 
 def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
                 sh: torch.Tensor, clip=None, samples=None, thresholds: bool = True,
-                max_samples: int = 2_000_000):
+                max_samples: int = 2_000_000, return_log_t: bool = False):
     """FP64 colours (n, 3) of the reference compositing over the kernel's exact
     sample positions; differentiable in ``density`` (R,) and ``sh`` (R, C).
 
@@ -493,20 +498,29 @@ def render_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor,
     samples with sigma <= sigma_thresh are skipped and a ray stops after the
     first sample whose transmittance falls below stop_thresh (T_end = 0);
     without, sigma is clamped at 0 and every sample counts (the reference's
-    pure-PyTorch semantics).
+    pure-PyTorch semantics).  With ``return_log_t`` also each ray's final log
+    transmittance (n,): minus its optical depth, or -1e3 for a ray the stop
+    test ended (render_lerp_kernel_cuvol.cu:103-105), differentiable likewise.
     """
     origins = np.asarray(origins, np.float32)
     dirs = np.asarray(dirs, np.float32)
     n = origins.shape[0]
     compact = thresholds and opt.sigma_thresh >= 0
-    outs = []
+    outs, log_ts = [], []
     for b0, b1, bsub, used in _blocks(grid, origins, dirs, opt, clip, samples, max_samples,
                                       density.shape[0], compact):
         dens, shb = (density, sh) if used is None else (density[used], sh[used])
         for c0, c1, sub in _ray_chunks(bsub, b1 - b0, max_samples):
             r0, r1 = b0 + c0, b0 + c1
-            outs.append(_composite(grid, dirs[r0:r1], opt, dens, shb, sub, r1 - r0, thresholds))
-    return torch.cat(outs, 0) if outs else torch.zeros((0, 3), dtype=torch.float64)
+            extra = {}
+            outs.append(_composite(grid, dirs[r0:r1], opt, dens, shb, sub, r1 - r0, thresholds,
+                                   extra))
+            log_ts.append(torch.where(extra["stopped"], torch.full_like(extra["depth"], -1e3),
+                                      -extra["depth"]))
+    rgb = torch.cat(outs, 0) if outs else torch.zeros((0, 3), dtype=torch.float64)
+    if return_log_t:
+        return rgb, (torch.cat(log_ts, 0) if log_ts else torch.zeros(0, dtype=torch.float64))
+    return rgb
 
 
 def _blocks(grid: Grid, origins, dirs, opt: Opts, clip, samples, max_samples: int, rows: int,
@@ -954,6 +968,198 @@ def tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray, 
         cols = np.broadcast_to(ch[None, :], act.shape)[act]
         np.add.at(grad, (rows, cols), (val * idelta)[act])
         mask[rows] = True
+    return grad, mask
+
+
+@dataclass
+class Background:
+    """Host copy of an MSI background: links (2 reso, reso) int32 (-1 =
+    empty) into rows of data (rows, nlayers, 4) float64, channels (r, g, b,
+    sigma) as the GPU stores them (float32 values)."""
+    links: np.ndarray
+    data: np.ndarray
+
+    @property
+    def reso(self):
+        return self.links.shape[1]
+
+    @property
+    def nlayers(self):
+        return self.data.shape[1]
+
+
+def background_fp64(grid: Grid, bgm: Background, origins, dirs, opt: Opts,
+                    data: torch.Tensor, log_t: torch.Tensor, sparsity_energy: bool = False):
+    """FP64 colour the MSI background adds to each ray (n, 3) behind the
+    grid's log transmittance ``log_t`` (n,), restating render_background_forward
+    (render_lerp_kernel_cuvol.cu:386-476; kernel :775-796), differentiable in
+    ``data`` (rows, nlayers, 4) and ``log_t``:
+
+    * the ray in the frame where the grid spans [-1, 1] (ray_find_bounds_bg,
+      render_util.cuh:630-657): o_s = (o_grid + 0.5) 2 / size - 1, d_s =
+      normalised d_grid 2 / size, ws = the world length per unit of d_s;
+    * first radius max(|o_s x d_s| + 1e-3, 1); n_steps = int(nlayers /
+      step_size) + 2 (in float32); sample i at radius n_steps / (n_steps - i -
+      0.5), on the sphere's far intersection (ConcentricSpheresIntersector,
+      :546-576), skipped below the first radius or without intersection;
+    * pixel (2 reso (0.5 + lon / 2 pi), reso (0.5 - lat / pi)), lat = asin(u_y),
+      lon = atan2(u_x, u_z) of the unit position u; layer clamp((1 - 1 / r)
+      nlayers - 0.5, 0, nlayers - 1); trilinear over columns (x, y), (x, y + 1),
+      (x + 1, y), (x + 1, y + 1) wrapping at the image edges (trilerp_bg_one,
+      :157-189), empty columns reading 0;
+    * a sample with sigma > 0 has optical depth (1 / r_prev - 1 / r) ws sigma,
+      r_prev the radius of the previous placed sample; weight T (1 - e^-depth),
+      colour max(C0 rgb + 0.5, 0); the ray stops after the sample where T falls
+      below stop_thresh; the remaining T times background_brightness is added;
+    * rays with log_t < -25 get nothing.
+    With ``sparsity_energy`` also sum over contributing samples of
+    log(1 + 2 sigma^2), whose gradient is the Cauchy sparsity term the fused
+    backward adds (:584-588)."""
+    o = np.asarray(origins, np.float64)
+    d = np.asarray(dirs, np.float64)
+    n = o.shape[0]
+    size = np.asarray(grid.size, np.float64)
+    og = o * grid.scaling + grid.offset
+    dg = d * grid.scaling
+    ws = 1.0 / np.linalg.norm(dg, axis=1)
+    dg = dg * ws[:, None]
+    sph = 2.0 / size
+    os_ = (og + 0.5) * sph - 1.0
+    ds = dg * sph
+    inorm = 1.0 / np.linalg.norm(ds, axis=1)
+    ws = ws * inorm
+    ds = ds * inorm[:, None]
+    q2a = 2.0 * (ds * ds).sum(1)
+    qb = 2.0 * (os_ * ds).sum(1)
+    f = qb * qb - 2.0 * q2a * (os_ * os_).sum(1)
+    inner = np.maximum(np.linalg.norm(np.cross(os_, ds), axis=1) + 1e-3, 1.0)
+    reso, nl = bgm.reso, bgm.nlayers
+    links = np.asarray(bgm.links, np.int64)
+    rows = data.shape[0]
+    n_steps = int(np.float32(nl) / np.float32(opt.step_size)) + 2
+    active = (log_t.detach().numpy() >= -25.0)
+    stopped = np.zeros(n, bool)
+    invr_last = 1.0 / inner
+    lt = log_t
+    out = torch.zeros((n, 3), dtype=torch.float64)
+    energy = torch.zeros((), dtype=torch.float64)
+    ws_t = torch.from_numpy(ws)
+    for i in range(n_steps):
+        rad = n_steps / (n_steps - i - 0.5)
+        det = f + 2.0 * q2a * rad * rad
+        valid = active & ~stopped & (rad >= inner) & (det >= 0.0)
+        t = (-qb + np.sqrt(np.maximum(det, 0.0))) / q2a
+        p = os_ + t[:, None] * ds
+        invr = 1.0 / np.linalg.norm(p, axis=1)
+        u = p * invr[:, None]
+        lat = np.arcsin(np.clip(u[:, 1], -1.0, 1.0))
+        lon = np.arctan2(u[:, 0], u[:, 2])
+        x = 2 * reso * (0.5 + lon * 0.5 / np.pi)
+        y = reso * (0.5 - lat / np.pi)
+        z = np.clip((1.0 - invr) * nl - 0.5, 0.0, nl - 1.0)
+        l0 = np.clip(np.trunc(x).astype(np.int64), 0, 2 * reso - 1)
+        l1 = np.clip(np.trunc(y).astype(np.int64), 0, reso - 1)
+        l2 = np.clip(np.trunc(z).astype(np.int64), 0, nl - 2)
+        fx, fy, fz = (torch.from_numpy(v) for v in (x - l0, y - l1, z - l2))
+        nx = np.where(l0 < 2 * reso - 1, l0 + 1, 0)
+        ny = np.where(l1 < reso - 1, l1 + 1, 0)
+
+        def column(cx, cy):
+            lk = links[cx, cy]
+            ok = (lk >= 0) & (lk < rows)
+            r = torch.from_numpy(np.where(ok, lk, 0))
+            lz = torch.from_numpy(l2)
+            v0, v1 = data[r, lz], data[r, lz + 1]
+            v = v0 + fz[:, None] * (v1 - v0)
+            return v * torch.from_numpy(ok)[:, None].to(torch.float64)
+
+        c0 = column(l0, l1) + fy[:, None] * (column(l0, ny) - column(l0, l1))
+        c1 = column(nx, l1) + fy[:, None] * (column(nx, ny) - column(nx, l1))
+        v = c0 + fx[:, None] * (c1 - c0)
+        sigma = v[:, 3]
+        contrib = torch.from_numpy(valid) & (sigma > 0).detach()
+        step = torch.from_numpy(invr_last - invr) * ws_t
+        pcnt = torch.where(contrib, step * sigma, torch.zeros_like(sigma))
+        w = torch.exp(lt) * (1.0 - torch.exp(-pcnt))
+        col = torch.clamp_min(v[:, :3] * SH_C0 + 0.5, 0.0)
+        out = out + (w * contrib.to(torch.float64))[:, None] * col
+        if sparsity_energy:
+            energy = energy + torch.where(contrib, torch.log(1.0 + 2.0 * sigma * sigma),
+                                          torch.zeros_like(sigma)).sum()
+        lt = lt - pcnt
+        stopped |= (contrib & (torch.exp(lt) < opt.stop_thresh)).numpy()
+        invr_last = np.where(valid, invr, invr_last)
+    out = out + (torch.exp(lt) * float(opt.background_brightness)
+                 * torch.from_numpy(active).to(torch.float64))[:, None]
+    return (out, energy) if sparsity_energy else out
+
+
+def render_bg_fp64(grid: Grid, bgm: Background, origins, dirs, opt: Opts,
+                   density: torch.Tensor, sh: torch.Tensor, data: torch.Tensor, clip=None,
+                   sparsity_energy: bool = False):
+    """FP64 colours (n, 3) of a grid with an MSI background: the grid's
+    compositing with background brightness 0 (render_lerp_kernel_cuvol.cu:43,
+    111-113) plus ``background_fp64`` behind its final log transmittance;
+    differentiable in density, sh and the background data."""
+    import dataclasses
+    fg, log_t = render_fp64(grid, origins, dirs, dataclasses.replace(opt, background_brightness=0.0),
+                            density, sh, clip=clip, return_log_t=True)
+    bg = background_fp64(grid, bgm, origins, dirs, opt, data, log_t, sparsity_energy)
+    if sparsity_energy:
+        return fg + bg[0], bg[1]
+    return fg + bg
+
+
+def msi_tv_grad_sparse(links: np.ndarray, msi: np.ndarray, rand_cells: np.ndarray,
+                       scale: float, scale_last: float):
+    """(grad (rows, nlayers, C), mask (rows, nlayers)) of msi_tv_grad_sparse
+    (loss_kernel.cu:248-333,606-641) in FP64: for each listed flat index
+    (x reso + y) nlayers + z and channel c, with v00 = the cell's value (0 for
+    an empty column), its +x and +y neighbours (wrapping) and layer z + 1
+    (missing: v00 for colours, 0 for the last channel): dx, dy, dz = the
+    differences, idelta = (scale or scale_last for the last channel) / n_cells
+    / sqrt(1e-9 + dx^2 + dy^2 + dz^2) of the unscaled differences, then
+    dx, dy, dz scaled by 2 reso / 256, reso / 256, nlayers / 256, and
+    -(dx + dy + dz), dz, dy, dx times idelta added to the cell, its layer
+    z + 1, its +y and its +x neighbours (nonempty columns, nonzero terms,
+    each marking its layer)."""
+    links = np.asarray(links, np.int64)
+    msi = np.asarray(msi, np.float64)
+    rows, nl, C = msi.shape
+    reso = links.shape[1]
+    grad = np.zeros_like(msi)
+    mask = np.zeros((rows, nl), bool)
+    cells = np.asarray(rand_cells, np.int64)
+    n = cells.size
+    for idx in cells:
+        z = idx % nl
+        tmp = idx // nl
+        y = tmp % reso
+        x = tmp // reso
+        nx = 0 if x == 2 * reso - 1 else x + 1
+        ny = 0 if y == reso - 1 else y + 1
+        l00, l01, l10 = links[x, y], links[x, ny], links[nx, y]
+        for c in range(C):
+            last = c == C - 1
+            v00 = msi[l00, z, c] if l00 >= 0 else 0.0
+            if l00 >= 0 and z + 1 < nl:
+                vz = msi[l00, z + 1, c]
+            else:
+                vz = 0.0 if last else v00
+            v01 = msi[l01, z, c] if l01 >= 0 else 0.0
+            v10 = msi[l10, z, c] if l10 >= 0 else 0.0
+            dx, dy, dz = v10 - v00, v01 - v00, vz - v00
+            idelta = (scale_last if last else scale) / n / math.sqrt(1e-9 + dx * dx + dy * dy
+                                                                     + dz * dz)
+            dx *= 2 * reso / 256.0
+            dy *= reso / 256.0
+            dz *= nl / 256.0
+            for lk, zz, val in ((l00, z, -(dx + dy + dz)), (l00, z + 1, dz), (l01, z, dy),
+                                (l10, z, dx)):
+                if lk < 0 or zz >= nl or val == 0.0:
+                    continue
+                grad[lk, zz, c] += val * idelta
+                mask[lk, zz] = True
     return grad, mask
 
 

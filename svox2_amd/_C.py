@@ -47,14 +47,14 @@ EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol_backward", "svox2_volume_render_cuvol_fused",
     "svox2_volume_render_cuvol_fused_noalloc",
     "svox2_ray_order_workspace_bytes", "svox2_volume_render_cuvol_fused_workspace_bytes",
-    "svox2_cell_map_bytes",
+    "svox2_background_workspace_bytes", "svox2_cell_map_bytes",
     "svox2_build_cell_map", "svox2_volume_render_cuvol_tiles",
     "svox2_volume_render_cuvol_stats", "svox2_volume_render_expected_term",
     "svox2_volume_render_sigma_thresh", "svox2_volume_render_nvol",
     "svox2_volume_render_nvol_backward", "svox2_volume_render_nvol_fused",
     "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_tv_grad_sparse_run", "svox2_lumisphere_tv_grad_sparse",
-    "svox2_rmsprop_step", "svox2_sgd_step",
+    "svox2_msi_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
     "svox2_rmsprop_step_grouped", "svox2_accel_dist_prop_workspace_bytes",
     "svox2_accel_dist_prop", "svox2_resample_density", "svox2_resample_rows",
     "svox2_grid_weight_render", "svox2_threshold_mask", "svox2_dilate",
@@ -82,6 +82,11 @@ class _Grid(ctypes.Structure):
         ("density_stride", ctypes.c_int64),
         ("sh_stride", ctypes.c_int64),
         ("cell_map", ctypes.c_void_p),
+        ("background_links", ctypes.c_void_p),
+        ("background_data", ctypes.c_void_p),
+        ("background_rows", ctypes.c_int64),
+        ("background_reso", ctypes.c_int32),
+        ("background_nlayers", ctypes.c_int32),
     ]
 
 
@@ -122,6 +127,8 @@ class _Grads(ctypes.Structure):
         ("mask_out", ctypes.c_void_p),
         ("grad_density_stride", ctypes.c_int64),
         ("grad_sh_stride", ctypes.c_int64),
+        ("grad_background_out", ctypes.c_void_p),
+        ("mask_background_out", ctypes.c_void_p),
     ]
 
 
@@ -168,6 +175,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_tv_grad_sparse_run": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, i64,
                                      i64, vp, i32, i32, f32, i32, i32, vp, vp],
         "svox2_lumisphere_tv_grad_sparse": [G, vp, i64, vp, vp, f32, f32, f32, f32, Gr, vp],
+        "svox2_msi_tv_grad_sparse": [vp, i32, vp, i64, i32, i32, vp, i64, vp, f32, f32, vp, vp],
         "svox2_rmsprop_step": [vp, i32, vp, vp, i64, i32, i64, i64, i64, i32, vp, i64, f32, f32,
                                f32, f32, f32, vp],
         "svox2_sgd_step": [vp, i32, vp, i64, i32, i64, i64, i32, vp, i64, f32, f32, vp],
@@ -197,6 +205,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.svox2_volume_render_cuvol_fused_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_ray_order_workspace_bytes.argtypes = [i64]
     lib.svox2_ray_order_workspace_bytes.restype = ctypes.c_size_t
+    lib.svox2_background_workspace_bytes.argtypes = [i64]
+    lib.svox2_background_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_mask_compact_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
     lib.svox2_mask_compact_workspace_bytes.restype = ctypes.c_size_t
     lib.svox2_accel_dist_prop_workspace_bytes.argtypes = [ctypes.POINTER(i32)]
@@ -384,8 +394,6 @@ def _grid_struct(g: SparseGridSpec, need_sh: bool = True, need_map: bool = True)
             raise RuntimeError("density_data and sh_data row counts differ")
     if dens.size(1) != 1:
         raise RuntimeError("density_data must be (rows, 1)")
-    if g.background_links is not None and g.background_links.numel() > 0:
-        raise RuntimeError("background (MSI) models are not supported")
     if g.basis_type != BASIS_TYPE_SH:
         raise RuntimeError(f"basis_type {g.basis_type} not supported (SH only)")
     s = _Grid()
@@ -414,6 +422,25 @@ def _grid_struct(g: SparseGridSpec, need_sh: bool = True, need_map: bool = True)
         if cmap.numel() * 2 < need or cmap.device != links.device:
             raise RuntimeError(f"cell_map must hold {need} bytes on the links' device")
         s.cell_map = cmap.data_ptr()
+    bl, bd = g.background_links, g.background_data
+    if bl is not None and bl.numel() > 0:
+        # MSI background (data_spec.hpp:36-43): links (2 reso, reso) int32,
+        # data (rows, nlayers, 4) float32.
+        bl = _check(bl, "background_links", (torch.int32,), 2)
+        if bd is None:
+            raise RuntimeError("background_links given without background_data")
+        bd = _check(bd, "background_data", (torch.float32,), 3)
+        if bl.size(0) != 2 * bl.size(1):
+            raise RuntimeError("background_links must be (2 reso, reso)")
+        if bd.size(2) != 4 or bd.size(1) < 2:
+            raise RuntimeError("background_data must be (rows, nlayers >= 2, 4)")
+        if bl.device != links.device or bd.device != links.device:
+            raise RuntimeError("background tensors must be on the links' device")
+        s.background_links = bl.data_ptr()
+        s.background_data = bd.data_ptr()
+        s.background_rows = bd.size(0)
+        s.background_reso = bl.size(1)
+        s.background_nlayers = bd.size(1)
     return s
 
 
@@ -481,7 +508,8 @@ def _rays(r: RaysSpec):
     return o, d, clip
 
 
-def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
+def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int, grid: Optional[_Grid] = None
+                  ) -> _Grads:
     s = _Grads()
     if gr.grad_density_out is not None:
         t = _check_rows(gr.grad_density_out, "grad_density_out", (torch.float32,), 1)
@@ -500,8 +528,22 @@ def _grads_struct(gr: GridOutputGrads, rows: int, sh_dim: int) -> _Grads:
         if t.numel() != rows:
             raise RuntimeError("mask_out must have one entry per row")
         s.mask_out = t.data_ptr()
-    if gr.grad_basis_out is not None or gr.grad_background_out is not None:
-        raise RuntimeError("basis / background gradients are not supported")
+    if gr.grad_basis_out is not None:
+        raise RuntimeError("basis gradients are not supported (SH basis only)")
+    # Background gradients, for the entry points that composite a background
+    # (the others ignore them, as the reference's do).
+    if grid is not None and grid.background_nlayers > 0:
+        shape = (grid.background_rows, grid.background_nlayers)
+        if gr.grad_background_out is not None:
+            t = _check(gr.grad_background_out, "grad_background_out", (torch.float32,), 3)
+            if tuple(t.shape) != shape + (4,):
+                raise RuntimeError("grad_background_out must be (rows, nlayers, 4)")
+            s.grad_background_out = t.data_ptr()
+        if gr.mask_background_out is not None and gr.mask_background_out.numel() > 0:
+            t = _check(gr.mask_background_out, "mask_background_out", (torch.bool, torch.uint8))
+            if t.numel() != shape[0] * shape[1]:
+                raise RuntimeError("mask_background_out must have rows * nlayers entries")
+            s.mask_background_out = t.data_ptr()
     return s
 
 
@@ -527,6 +569,8 @@ def volume_render_cuvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions
             _check(t, name, (torch.float32,) if t is log_transmit_out else (torch.int32,))
             if t.numel() != o.size(0):
                 raise RuntimeError(f"{name} must have one value per ray")
+    if g.background_nlayers > 0 and log_transmit_out is None:
+        log_transmit_out = torch.empty(o.size(0), dtype=torch.float32, device=o.device)
     nbytes = int(load_library().svox2_ray_order_workspace_bytes(o.size(0))) if ordered else 0
     ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
@@ -600,6 +644,11 @@ def volume_render_cuvol_image(grid: SparseGridSpec, cam: CameraSpec, opt: Render
     out = torch.empty((int(cam.height), int(cam.width), 3), dtype=torch.float32, device=dev)
     if log_transmit_out is not None:
         _check(log_transmit_out, "log_transmit_out", (torch.float32,))
+        if log_transmit_out.numel() != int(cam.height) * int(cam.width):
+            raise RuntimeError("log_transmit_out must have one value per pixel")
+    elif g.background_nlayers > 0:
+        log_transmit_out = torch.empty(int(cam.height) * int(cam.width), dtype=torch.float32,
+                                       device=dev)
     with _on_device(grid.sh_data):
         _call("svox2_volume_render_cuvol_image", ctypes.byref(g), ctypes.byref(_cam_struct(cam)),
               ctypes.byref(_opt_struct(opt)), _ptr(out), _ptr(log_transmit_out),
@@ -622,14 +671,17 @@ def volume_render_cuvol_backward(grid: SparseGridSpec, rays: RaysSpec, opt: Rend
     _check(color_cache, "color_cache", (torch.float32,), 2)
     if grad_out.shape != o.shape or color_cache.shape != o.shape:
         raise RuntimeError("grad_out / color_cache must be (n, 3)")
-    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim, g)
     ws, nbytes = None, 0
     if ray_cost is not None:
         _check(ray_cost, "ray_cost", (torch.int32,))
         if ray_cost.numel() != o.size(0):
             raise RuntimeError("ray_cost must have one value per ray")
         nbytes = int(load_library().svox2_ray_order_workspace_bytes(o.size(0)))
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
+    if g.background_nlayers > 0:
+        nbytes = int(load_library().svox2_background_workspace_bytes(o.size(0)))
+    if nbytes:
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
     with _on_device(o):
         _call("svox2_volume_render_cuvol_backward", ctypes.byref(g), _ptr(o), _ptr(d),
               _ptr(clip), o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(grad_out),
@@ -654,7 +706,7 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
     _check(rgb_out, "rgb_out", (torch.float32,), 2)
     if rgb_gt.shape != o.shape or rgb_out.shape != o.shape:
         raise RuntimeError("rgb_gt / rgb_out must be (n, 3)")
-    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim, g)
     log_t = (torch.empty(o.size(0), dtype=torch.float32, device=o.device)
              if beta_loss > 0 else None)
     # Workspace for the ray ordering and, for small batches, the record
@@ -666,6 +718,8 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
         nbytes = lib.svox2_ray_order_workspace_bytes(o.size(0))
     else:
         nbytes = 0
+    if g.background_nlayers > 0:  # the background's per-ray buffers
+        nbytes = max(nbytes, lib.svox2_background_workspace_bytes(o.size(0)))
     ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device) if nbytes else None
     with _on_device(o):
         _call("svox2_volume_render_cuvol_fused", ctypes.byref(g), _ptr(o), _ptr(d), _ptr(clip),
@@ -843,6 +897,32 @@ def lumisphere_tv_grad_sparse(grid: SparseGridSpec, rand_cells: torch.Tensor,
               rand_cells.numel(), _ptr(basis_fn), _ptr(basis_fn_u), float(scale),
               float(ndc_coeffx), float(ndc_coeffy), float(dir_factor), ctypes.byref(gr),
               _stream(grid.sh_data))
+
+
+def msi_tv_grad_sparse(links: torch.Tensor, msi: torch.Tensor, rand_cells: torch.Tensor,
+                       mask_out: torch.Tensor, scale: float, scale_last: float,
+                       grad_msi: torch.Tensor) -> None:
+    """msi_tv_grad_sparse (loss_kernel.cu:606-641): total variation of an MSI
+    background, links (2 reso, reso) int32, msi and grad_msi (rows, nlayers,
+    channels) float32, rand_cells int32 flat indices over (2 reso, reso,
+    nlayers); mask_out (rows, nlayers) bool, or empty for none."""
+    links = _check(links, "links", (torch.int32,), 2)
+    msi = _check(msi, "msi", (torch.float32,), 3)
+    grad_msi = _check(grad_msi, "grad_msi", (torch.float32,), 3)
+    rand_cells = _check(rand_cells, "rand_cells", (torch.int32,), 1)
+    if links.size(0) != 2 * links.size(1):
+        raise RuntimeError("links must be (2 reso, reso)")
+    if grad_msi.shape != msi.shape:
+        raise RuntimeError("grad_msi must have msi's shape")
+    mask = None
+    if mask_out is not None and mask_out.numel() > 0:
+        mask = _check(mask_out, "mask_out", (torch.bool, torch.uint8))
+        if mask.numel() != msi.size(0) * msi.size(1):
+            raise RuntimeError("mask_out must be (rows, nlayers)")
+    with _on_device(msi):
+        _call("svox2_msi_tv_grad_sparse", _ptr(links), links.size(1), _ptr(msi), msi.size(0),
+              msi.size(1), msi.size(2), _ptr(rand_cells), rand_cells.numel(), _ptr(mask),
+              float(scale), float(scale_last), _ptr(grad_msi), _stream(msi))
 
 
 def _indexer_args(indexer: torch.Tensor, rows: int):

@@ -96,6 +96,41 @@ __device__ __forceinline__ float lerpf(float a, float b, float w) {
     return fmaf(w, b - a, a);
 }
 
+// World ray of pixel (ix, iy) (cam2world_ray, render_util.cuh:526-544; NDC
+// warp world2ndc :504-524 when ndc_coeffx > 0).
+__device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, int iy,
+                                           float* dir, float* origin) {
+    float x = (static_cast<float>(ix) + 0.5f - cam.cx) / cam.fx;
+    float y = (static_cast<float>(iy) + 0.5f - cam.cy) / cam.fy;
+    float z = sqrtf(static_cast<float>(static_cast<double>(x * x + y * y) + 1.0));
+    x /= z;
+    y /= z;
+    z = 1.0f / z;
+    const float* m = cam.c2w;
+    dir[0] = m[0] * x + m[1] * y + m[2] * z;
+    dir[1] = m[4] * x + m[5] * y + m[6] * z;
+    dir[2] = m[8] * x + m[9] * y + m[10] * z;
+    origin[0] = m[3];
+    origin[1] = m[7];
+    origin[2] = m[11];
+    if (cam.ndc_coeffx > 0.f) {
+        const float nearp = 1.f;
+        const float t = (nearp - origin[2]) / dir[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) origin[i] = fmaf(t, dir[i], origin[i]);
+        dir[0] = cam.ndc_coeffx * (dir[0] / dir[2] - origin[0] / origin[2]);
+        dir[1] = cam.ndc_coeffy * (dir[1] / dir[2] - origin[1] / origin[2]);
+        dir[2] = 2.f * nearp / origin[2];
+        origin[0] = cam.ndc_coeffx * (origin[0] / origin[2]);
+        origin[1] = cam.ndc_coeffy * (origin[1] / origin[2]);
+        origin[2] = 1.f - 2.f * nearp / origin[2];
+        const float rn = 1.0f / sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        dir[0] *= rn;
+        dir[1] *= rn;
+        dir[2] *= rn;
+    }
+}
+
 // No-return float atomic add; lowers to global_atomic_add_f32.
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
     unsafeAtomicAdd(p, v);
@@ -170,6 +205,26 @@ __host__ __device__ inline GridView<DT> make_view(const svox2_grid_t& g) {
 
 // Validates a grid descriptor; `need_sh` requires sh_data to be present.
 int check_grid(const svox2_grid_t* g, bool need_sh);
+
+// MSI background of a grid (background.hip).  grid_has_background: whether
+// the grid carries one.  check_background validates it with the render
+// options.  launch_background adds each ray's background colour (rays from
+// origins / dirs, or from cam when not NULL) to rgb behind the grid's log
+// transmittance log_t_fg, writing it to bg_rgb_out too when not NULL.
+// launch_background_backward accumulates the background's gradients (see
+// k_background_backward for grad_in / rgb_residual and bg_src / bg_src_is_fg).
+bool grid_has_background(const svox2_grid_t* g);
+int check_background(const svox2_grid_t* g, const svox2_render_options_t* opt);
+int launch_background(const svox2_grid_t* g, const svox2_render_options_t* opt,
+                      const float* origins, const float* dirs, const svox2_camera_t* cam,
+                      int64_t n, const float* log_t_fg, float* rgb, float* bg_rgb_out,
+                      hipStream_t s);
+int launch_background_backward(const svox2_grid_t* g, const svox2_render_options_t* opt,
+                               const float* origins, const float* dirs, int64_t n,
+                               const float* grad_in, const float* color_cache, int rgb_residual,
+                               float norm, const float* log_t_fg, const float* bg_src,
+                               int bg_src_is_fg, float sparsity, const svox2_grads_t* grads,
+                               hipStream_t s);
 
 // Resolved row strides of a gradient descriptor.
 inline int64_t grad_dstride(const svox2_grads_t& g) {

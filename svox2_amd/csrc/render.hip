@@ -1969,41 +1969,6 @@ __global__ __launch_bounds__(kBlock) void k_render_nvol_backward(
     }
 }
 
-// World ray of pixel (ix, iy) (cam2world_ray, render_util.cuh:526-544; NDC
-// warp world2ndc :504-524 when ndc_coeffx > 0).
-__device__ __forceinline__ void camera_ray(const svox2_camera_t& cam, int ix, int iy,
-                                           float* dir, float* origin) {
-    float x = (static_cast<float>(ix) + 0.5f - cam.cx) / cam.fx;
-    float y = (static_cast<float>(iy) + 0.5f - cam.cy) / cam.fy;
-    float z = sqrtf(static_cast<float>(static_cast<double>(x * x + y * y) + 1.0));
-    x /= z;
-    y /= z;
-    z = 1.0f / z;
-    const float* m = cam.c2w;
-    dir[0] = m[0] * x + m[1] * y + m[2] * z;
-    dir[1] = m[4] * x + m[5] * y + m[6] * z;
-    dir[2] = m[8] * x + m[9] * y + m[10] * z;
-    origin[0] = m[3];
-    origin[1] = m[7];
-    origin[2] = m[11];
-    if (cam.ndc_coeffx > 0.f) {
-        const float nearp = 1.f;
-        const float t = (nearp - origin[2]) / dir[2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) origin[i] = fmaf(t, dir[i], origin[i]);
-        dir[0] = cam.ndc_coeffx * (dir[0] / dir[2] - origin[0] / origin[2]);
-        dir[1] = cam.ndc_coeffy * (dir[1] / dir[2] - origin[1] / origin[2]);
-        dir[2] = 2.f * nearp / origin[2];
-        origin[0] = cam.ndc_coeffx * (origin[0] / origin[2]);
-        origin[1] = cam.ndc_coeffy * (origin[1] / origin[2]);
-        origin[2] = 1.f - 2.f * nearp / origin[2];
-        const float rn = 1.0f / sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        dir[0] *= rn;
-        dir[1] *= rn;
-        dir[2] *= rn;
-    }
-}
-
 // t of lane k of a chunk whose lane 0 has t0: the float chain t0 + step + ...
 // + step (k additions), by one fma when every addition is exact (see
 // place_chunk), else replayed.
@@ -2656,6 +2621,15 @@ int render_rays(const svox2_grid_t* grid, const float* origins, const float* dir
     if (n_rays == 0) return ok();
     SVOX2_REQUIRE(rgb_out != nullptr, "rgb_out is NULL");
     if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
+    // A background is composited behind the grid's own colours, which are
+    // rendered with brightness 0 (render_lerp_kernel_cuvol.cu:43,111-113).
+    const bool background = grid_has_background(grid);
+    if (background) {
+        if ((st = check_background(grid, opt))) return st;
+        SVOX2_REQUIRE(log_transmit_out != nullptr,
+                      "a grid with a background needs log_transmit_out (n floats)");
+        o.bg = 0.f;
+    }
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
@@ -2672,6 +2646,9 @@ int render_rays(const svox2_grid_t* grid, const float* origins, const float* dir
                            static_cast<const uint32_t*>(w.order), ChunkLog{});
     });
     if (st) return st;
+    if (background && (st = launch_background(grid, opt, origins, dirs, nullptr, n_rays,
+                                              log_transmit_out, rgb_out, nullptr, s)))
+        return st;
     return check_launch(what);
 }
 
@@ -2930,6 +2907,13 @@ extern "C" int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
     const int64_t n = static_cast<int64_t>(cam->width) * cam->height;
     if (n == 0) return ok();
     SVOX2_REQUIRE(rgb_out != nullptr, "rgb_out is NULL");
+    const bool background = grid_has_background(grid);
+    if (background) {
+        if ((st = check_background(grid, opt))) return st;
+        SVOX2_REQUIRE(log_transmit_out != nullptr,
+                      "a grid with a background needs log_transmit_out (width * height floats)");
+        o.bg = 0.f;
+    }
     const unsigned blocks = blocks_for(n * kGroup, kBlock);
     const svox2_camera_t c = *cam;
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
@@ -2939,6 +2923,10 @@ extern "C" int svox2_volume_render_cuvol_image(const svox2_grid_t* grid,
                            as_stream(stream), g, o, c, rgb_out, log_transmit_out);
     });
     if (st) return st;
+    if (background && (st = launch_background(grid, opt, nullptr, nullptr, &c, n,
+                                              log_transmit_out, rgb_out, nullptr,
+                                              as_stream(stream))))
+        return st;
     return check_launch("volume_render_cuvol_image");
 }
 
@@ -2983,6 +2971,13 @@ extern "C" int svox2_volume_render_cuvol_tiles(const svox2_grid_t* grids, int32_
     return fail(SVOX2_ERR_UNSUPPORTED, "basis_dim %d not supported (1, 4 or 9)", basis_dim);
 }
 
+// A background's per-call buffers (16 bytes per ray) follow the ordering
+// workspace.
+size_t background_offset(int64_t n) { return (order_workspace_bytes(n) + 255) & ~size_t{255}; }
+float* background_buffer(void* workspace, int64_t n) {
+    return reinterpret_cast<float*>(static_cast<char*>(workspace) + background_offset(n));
+}
+
 extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
                                                   const float* origins, const float* dirs,
                                                   const float* ray_clip, int64_t n_rays,
@@ -3003,6 +2998,17 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     SVOX2_REQUIRE(grad_out != nullptr && color_cache != nullptr,
                   "grad_out / color_cache are NULL");
     if ((st = check_order_workspace(workspace, workspace_bytes, n_rays))) return st;
+    const bool background = grid_has_background(grid);
+    float* bg_buf = nullptr;  // the grid's own colours [n][3], then its log transmittance [n]
+    if (background) {
+        if ((st = check_background(grid, opt))) return st;
+        SVOX2_REQUIRE(workspace != nullptr &&
+                          workspace_bytes >= svox2_background_workspace_bytes(n_rays),
+                      "a grid with a background needs a workspace of "
+                      "svox2_background_workspace_bytes(n_rays) = %zu bytes",
+                      svox2_background_workspace_bytes(n_rays));
+        bg_buf = background_buffer(workspace, n_rays);
+    }
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
@@ -3013,17 +3019,33 @@ extern "C" int svox2_volume_render_cuvol_backward(const svox2_grid_t* grid,
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
+        if (background) {
+            // The reference's grid backward hands its final accum and log
+            // transmittance to the background's (:1032-1064); here the grid's
+            // own colours and transmittance come from a forward render.
+            OptView ofg = o;
+            ofg.bg = 0.f;
+            hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
+                               g, ofg, origins, dirs, ray_clip, n_rays, bg_buf,
+                               bg_buf + 3 * n_rays, (int32_t*)nullptr, (uint32_t*)nullptr,
+                               RecStream{}, (const uint32_t*)nullptr, ChunkLog{});
+        }
         hipLaunchKernelGGL((k_render_backward<DT, B>), dim3(blocks), dim3(kBlock), 0, s, g, o,
                            origins, dirs, ray_clip, n_rays, grad_out, color_cache,
                            static_cast<const uint32_t*>(w.order), go);
     });
     if (st) return st;
+    if (background &&
+        (st = launch_background_backward(grid, opt, origins, dirs, n_rays, grad_out, color_cache,
+                                         0, 1.f, bg_buf + 3 * n_rays, bg_buf, 1, 0.f, grads, s)))
+        return st;
     return check_launch("volume_render_cuvol_backward");
 }
 
 extern "C" size_t svox2_ray_order_workspace_bytes(int64_t n_rays) {
     return n_rays > 0 ? order_workspace_bytes(n_rays) : 0;
 }
+
 
 // Whether the fused call records its samples for a batch of n rays (given a
 // workspace of fused_workspace_bytes(n)).  Replaying records saves each ray's
@@ -3037,9 +3059,16 @@ bool fused_records(int64_t n) {
            n * (kRecFixed + kRecOverflow) < INT32_MAX;
 }
 
+extern "C" size_t svox2_background_workspace_bytes(int64_t n_rays) {
+    if (n_rays <= 0) return 0;
+    return background_offset(n_rays) + 16 * static_cast<size_t>(n_rays);
+}
+
 extern "C" size_t svox2_volume_render_cuvol_fused_workspace_bytes(int64_t n_rays) {
     if (n_rays <= 0) return 0;
-    return fused_records(n_rays) ? fused_workspace_bytes(n_rays) : logged_workspace_bytes(n_rays);
+    return std::max(fused_records(n_rays) ? fused_workspace_bytes(n_rays)
+                                          : logged_workspace_bytes(n_rays),
+                    svox2_background_workspace_bytes(n_rays));
 }
 
 extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
@@ -3067,6 +3096,24 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     const float beta = beta_loss / static_cast<float>(n_rays);
     const unsigned blocks = blocks_for(n_rays * kGroup, kBlock);
     float* lt = beta_loss > 0.f ? log_transmit_out : nullptr;
+    // A background (render_lerp_kernel_cuvol.cu:1088-1150): the grid renders
+    // with brightness 0, the background adds its colour (kept in the workspace
+    // for its backward) before the grid's backward reads the total, and the
+    // background's backward runs last.  Every ray re-marches: the records'
+    // replay composes a ray's colour from the grid's samples alone.
+    const bool background = grid_has_background(grid);
+    float* bg_rgb = nullptr;
+    if (background) {
+        if ((st = check_background(grid, opt))) return st;
+        SVOX2_REQUIRE(workspace != nullptr &&
+                          workspace_bytes >= svox2_background_workspace_bytes(n_rays),
+                      "a grid with a background needs a workspace of "
+                      "svox2_background_workspace_bytes(n_rays) = %zu bytes",
+                      svox2_background_workspace_bytes(n_rays));
+        bg_rgb = background_buffer(workspace, n_rays);
+        lt = log_transmit_out != nullptr ? log_transmit_out : bg_rgb + 3 * n_rays;
+        o.bg = 0.f;
+    }
     const hipStream_t s = as_stream(stream);
     OrderWorkspace w;
     if (workspace != nullptr && n_rays >= order_min_rays()) w = order_workspace(workspace, n_rays);
@@ -3074,15 +3121,16 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
     ChunkLog log{};
     // Batches that re-march log the forward's chunk placements when the
     // workspace has room for them.
-    if (workspace != nullptr && !fused_records(n_rays) &&
+    if (workspace != nullptr && !background && !fused_records(n_rays) &&
         workspace_bytes >= logged_workspace_bytes(n_rays))
         log = chunk_log(workspace, n_rays);
-    if (workspace != nullptr && fused_records(n_rays) &&
+    if (workspace != nullptr && !background && fused_records(n_rays) &&
         workspace_bytes >= fused_workspace_bytes(n_rays)) {
         rs = record_stream(workspace, n_rays);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kSortBlock), 0, s,
                            reinterpret_cast<uint32_t*>(rs.next_block), 3 * kRecParts);
     }
+    int bg_st = 0;
     st = dispatch(grid, [&](auto g, auto b, auto dtp) {
         constexpr int B = decltype(b)::value;
         using DT = std::remove_pointer_t<decltype(dtp)>;
@@ -3107,6 +3155,9 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
             hipLaunchKernelGGL((k_render_rays<DT, B, false>), dim3(blocks), dim3(kBlock), 0, s,
                                g, o, origins, dirs, ray_clip, n_rays, rgb_out, lt,
                                (int32_t*)nullptr, w.cost, rs, order, log);
+        if (background)
+            bg_st = launch_background(grid, opt, origins, dirs, nullptr, n_rays, lt, rgb_out,
+                                      bg_rgb, s);
         // The key kernel cleared both histograms; this sort uses the second.
         if (w.order != nullptr) launch_cost_order(w.cost, w, n_rays, s, 1, true);
         if (rs.rec != nullptr)
@@ -3128,6 +3179,11 @@ extern "C" int svox2_volume_render_cuvol_fused(const svox2_grid_t* grid,
                                static_cast<const uint32_t*>(w.order), go, rs, log);
     });
     if (st) return st;
+    if (bg_st) return bg_st;
+    if (background &&
+        (st = launch_background_backward(grid, opt, origins, dirs, n_rays, rgb_gt, rgb_out, 1,
+                                         norm, lt, bg_rgb, 0, sparsity_loss, grads, s)))
+        return st;
     return check_launch("volume_render_cuvol_fused");
 }
 

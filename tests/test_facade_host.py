@@ -48,8 +48,15 @@ def test_grid_construction_layouts():
     assert gb.sh_data.dtype == torch.bfloat16 and gb.density_data.grad_dtype == torch.float32
     with pytest.raises(NotImplementedError):
         svox2_amd.SparseGrid(reso=4, basis_type=svox2_amd.BASIS_TYPE_MLP)
-    with pytest.raises(NotImplementedError):
-        svox2_amd.SparseGrid(reso=4, background_nlayers=4)
+    with pytest.raises(ValueError):
+        svox2_amd.SparseGrid(reso=4, background_nlayers=1)
+    # An MSI background: one row per pixel column in raster order, zeroed data
+    # (svox2.py:499-513).
+    gbg = svox2_amd.SparseGrid(reso=4, background_nlayers=3, background_reso=5)
+    assert gbg.use_background and gbg.background_links.shape == (10, 5)
+    assert torch.equal(gbg.background_links.flatten(), torch.arange(50, dtype=torch.int32))
+    assert gbg.background_data.shape == (50, 3, 4) and not gbg.background_data.any()
+    assert not svox2_amd.SparseGrid(reso=4).use_background
 
 
 @pytest.mark.parametrize("size,z_order", [((8, 8, 8), False), ((8, 8, 8), True),
@@ -200,8 +207,18 @@ def test_load_legacy_data_layout(tmp_path):
     assert np.array_equal(g.sh_data.data.numpy(), data[:, 1:])
     assert g.basis_dim == 9 and g.capacity == 6
     assert torch.equal(g.radius, torch.ones(3)) and torch.equal(g.center, torch.zeros(3))
-    # background models and other bases are refused, not half-loaded
-    np.savez(path, links=links, data=data, background_data=np.zeros((1, 2, 4), np.float32),
-             background_links=np.zeros((2, 1), np.int32))
-    with pytest.raises(NotImplementedError):
-        svox2_amd.SparseGrid.load(path)
+    # a background model loads with its links and data (svox2.py:1570-1622)
+    rng = np.random.default_rng(4)
+    bl = np.arange(18, dtype=np.int32).reshape(6, 3)
+    bl[2, 1] = -1
+    bd = rng.normal(size=(18, 2, 4)).astype(np.float32)
+    np.savez(path, links=links, data=data, background_data=bd, background_links=bl)
+    g = svox2_amd.SparseGrid.load(path)
+    assert g.use_background and g.background_nlayers == 2 and g.background_reso == 3
+    assert np.array_equal(g.background_links.numpy(), bl)
+    assert np.array_equal(g.background_data.data.numpy(), bd)
+    # and saves them again
+    path2 = str(tmp_path / "bg_round_trip.npz")
+    g.save(path2)
+    z = np.load(path2)
+    assert np.array_equal(z["background_links"], bl) and np.array_equal(z["background_data"], bd)
