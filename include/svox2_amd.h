@@ -218,6 +218,42 @@ int svox2_volume_render_nvol_fused(const svox2_grid_t* grid, const float* origin
                                    float sparsity_loss, float* rgb_out,
                                    const svox2_grads_t* grads, void* stream);
 
+/* svox1 backend forward: nearest-neighbour rendering by a walk over whole
+ * cells (the grid origin shifted by +0.5; each occupied cell one sample of
+ * its own density and SH colour, weighted by its chord plus 1e-2; empty
+ * blocks of the accelerated links skipped); a ray whose transmittance before
+ * a sample is <= stop_thresh stops after it with its colour divided by 1 -
+ * that transmittance.  rgb_out [n][3].  ray_clip must be NULL (the reference
+ * takes none); a background is ignored.
+ * Replaces volume_render_svox1 (svox2.cpp:39, bound at :89;
+ * render_svox1_kernel.cu:334). */
+int svox2_volume_render_svox1(const svox2_grid_t* grid, const float* origins,
+                              const float* dirs, const float* ray_clip, int64_t n_rays,
+                              const svox2_render_options_t* opt, float* rgb_out, void* stream);
+
+/* svox1 backward given dL/drgb (grad_out) and the forward colours
+ * (color_cache): SH and density gradients of every sample up to the one
+ * after which the transmittance is <= stop_thresh, as the reference computes
+ * them; grads->mask_out marks the rows of those samples.
+ * Replaces volume_render_svox1_backward (svox2.cpp:40, bound at :90;
+ * render_svox1_kernel.cu:351). */
+int svox2_volume_render_svox1_backward(const svox2_grid_t* grid, const float* origins,
+                                       const float* dirs, const float* ray_clip,
+                                       int64_t n_rays, const svox2_render_options_t* opt,
+                                       const float* grad_out, const float* color_cache,
+                                       const svox2_grads_t* grads, void* stream);
+
+/* svox1 forward into rgb_out, then its backward for dL/drgb = (rgb_out -
+ * rgb_gt) * 2 / (3 n); beta_loss and sparsity_loss are ignored, as in the
+ * reference.
+ * Replaces volume_render_svox1_fused (svox2.cpp:42, bound at :91;
+ * render_svox1_kernel.cu:380). */
+int svox2_volume_render_svox1_fused(const svox2_grid_t* grid, const float* origins,
+                                    const float* dirs, const float* ray_clip, int64_t n_rays,
+                                    const svox2_render_options_t* opt, const float* rgb_gt,
+                                    float beta_loss, float sparsity_loss, float* rgb_out,
+                                    const svox2_grads_t* grads, void* stream);
+
 /* Backward of svox2_volume_render_cuvol given dL/drgb (grad_out [n][3]) and
  * the forward colours (color_cache [n][3]); accumulates into grads.  With the
  * forward's ray_cost_out (ray_cost) and a device workspace of

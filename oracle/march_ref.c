@@ -23,6 +23,8 @@
  *                the GPU stats kernel produces;
  *   ref_samples  every evaluated (not skipped) sample of every ray, ignoring
  *                the stop test, for the FP64 oracle to integrate;
+ *   ref_svox1_samples  the svox1 backend's nearest-neighbour walk, every
+ *                occupied-cell sample with its step and stop flags;
  *   ref_grid_weight_render  per-voxel maximum weight of a camera's rays
  *                through a dense density volume (resample's weight render);
  *   ref_fused    forward colour plus the MSE gradient (render_ray_backward_kernel
@@ -340,6 +342,105 @@ int64_t ref_samples(const ref_grid_t* g, const float* origins, const float* dirs
                 if (log_t < log_stop) stopped = 1;
             }
             t += o->step;
+        }
+    }
+    return count;
+}
+
+/* The svox1 backend's nearest-neighbour walk (render_svox1_kernel.cu:50-160,
+ * trace_ray; compute_skip_dist_nn :23-48; _intersect_aabb_unit,
+ * render_util.cuh:448-461), in FP32 as the kernel: the grid origin shifted by
+ * +0.5, t from max(0, box entry), each occupied cell a sample whose step is
+ * the distance to the cell's exit plus 1e-2, each empty block (link < 0,
+ * 2^(-link - 1) cells on a side) skipped to its exit plus 1e-2.  Every
+ * occupied-cell sample up to the forward's stop is listed: ray, link, world
+ * step dt (step times the world scale) and flag bits: 1 density (+1e9 for an
+ * opaque last sample) above sigma_thresh; 2 the forward stops after it (its
+ * transmittance before it <= stop_thresh); 4 the backward stops after it
+ * (transmittance after it <= stop_thresh); 8 opaque last sample.  Links >=
+ * rows count as empty single cells.  Returns the sample count (entries past
+ * cap are not written). */
+int64_t ref_svox1_samples(const ref_grid_t* g, const float* origins, const float* dirs,
+                          int64_t n, const ref_opt_t* o, int64_t cap, int32_t* ray_out,
+                          int32_t* link_out, float* dt_out, uint8_t* flag_out) {
+    int64_t count = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        float og[3], d[3], inv[3];
+        for (int a = 0; a < 3; ++a) {
+            og[a] = fmaf(origins[3 * i + a], g->scaling[a], g->offset[a]);
+            d[a] = dirs[3 * i + a] * g->scaling[a];
+        }
+        const float ds = 1.0f / sqrtf(dot3(d, d));
+        for (int a = 0; a < 3; ++a) {
+            d[a] *= ds;
+            inv[a] = d[a] == 0.f ? 1e9f : (float)(1.0 / (double)d[a]);
+        }
+        float t = 0.f, tmax = 1e9f;
+        for (int a = 0; a < 3; ++a) {
+            og[a] += 0.5f;
+            const float t1 = (0.0f - og[a]) * inv[a];
+            const float t2 = ((float)g->size[a] - 1.f - og[a]) * inv[a];
+            t = fmaxf(t, fminf(t1, t2));
+            tmax = fminf(tmax, fmaxf(t1, t2));
+        }
+        if (t > tmax) continue;
+        float log_t = 0.f;
+        int guard = 0;
+        while (t < tmax && ++guard < (1 << 22)) {
+            float p[3];
+            int l[3];
+            for (int a = 0; a < 3; ++a) {
+                p[a] = og[a] + t * d[a];
+                p[a] = fminf(fmaxf(p[a], 0.f), (float)g->size[a] - 1.f);
+                l[a] = (int)p[a];
+                if (l[a] > g->size[a] - 1) l[a] = g->size[a] - 1;
+                p[a] -= (float)l[a];
+            }
+            const int link = g->links[((int64_t)l[0] * g->size[1] + l[1]) * g->size[2] + l[2]];
+            if (link >= 0 && (int64_t)link < g->rows) {
+                float dt = 1e9f;
+                for (int a = 0; a < 3; ++a) {
+                    const float t1 = -p[a] * inv[a];
+                    const float t2 = t1 + inv[a];
+                    dt = fminf(dt, fmaxf(t1, t2));
+                }
+                dt += 1e-2f;
+                t += dt;
+                float sigma = g->density[link];
+                int flag = 0;
+                if (o->last_opaque && t + o->step > tmax) {
+                    sigma += 1e9f;
+                    flag |= 8;
+                }
+                if (sigma > o->sigma_thresh) {
+                    flag |= 1;
+                    const float lt = -(dt * ds) * sigma;
+                    const float tr = expf(log_t);
+                    log_t += lt;
+                    if (tr <= o->stop_thresh) flag |= 2;
+                    if (expf(log_t) <= o->stop_thresh) flag |= 4;
+                }
+                if (count < cap) {
+                    ray_out[count] = (int32_t)i;
+                    link_out[count] = link;
+                    dt_out[count] = dt * ds;
+                    flag_out[count] = (uint8_t)flag;
+                }
+                ++count;
+                if (flag & 2) break;
+            } else {
+                const uint32_t dist = link < 0 ? (uint32_t)(-link) : 1u;
+                const uint32_t shift = dist - 1 > 30 ? 30 : dist - 1;
+                const uint32_t side = 1u << shift;
+                float smax = 1e9f;
+                for (int a = 0; a < 3; ++a) {
+                    const int ul = ((l[a] >> shift) << shift) - l[a];
+                    const float t1 = ((float)ul - p[a]) * inv[a];
+                    const float t2 = ((float)(int64_t)(ul + (int64_t)side) - p[a]) * inv[a];
+                    if (d[a] != 0.f) smax = fminf(smax, fmaxf(t1, t2));
+                }
+                t += fmaxf(smax, 0.f) + 1e-2f;
+            }
         }
     }
     return count;

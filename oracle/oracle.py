@@ -25,6 +25,9 @@ reference lines it follows:
 * ``background_fp64`` / ``render_bg_fp64`` (render_lerp_kernel_cuvol.cu:
   386-607), the MSI background behind the grid's log transmittance, and
   ``msi_tv_grad_sparse`` (loss_kernel.cu:248-333,606-641), its TV.
+* ``svox1_fp64`` / ``svox1_grads_fp64`` (render_svox1_kernel.cu:50-275),
+  the svox1 backend's nearest-neighbour compositing (differentiable) and its
+  backward (restated), over ``ref_svox1_samples``'s FP32 walk.
 * ``sample_trilerp`` (svox2.py:604-653), ``eval_sh_bases`` (utils.py:115-162),
   ``morton_code_3``/``gen_morton`` (utils.py:45-79), ``tv_grad_sparse``
   (loss_kernel.cu:178-246,558-604), ``lumisphere_tv_grad_sparse``
@@ -139,6 +142,8 @@ class MarchRef:
         lib.ref_render.restype = ctypes.c_int
         lib.ref_samples.argtypes = [G, vp, vp, vp, i64, O, i64, vp, vp, vp, vp, vp, vp]
         lib.ref_samples.restype = ctypes.c_int64
+        lib.ref_svox1_samples.argtypes = [G, vp, vp, i64, O, i64, vp, vp, vp, vp]
+        lib.ref_svox1_samples.restype = ctypes.c_int64
         lib.ref_ray_setup.argtypes = [G, vp, vp, vp, i64, O, vp]
         lib.ref_ray_setup.restype = ctypes.c_int
         lib.ref_camera_rays.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -209,6 +214,26 @@ class MarchRef:
         out = {k: np.concatenate([q[2][k] for q in parts]) for k in parts[0][2]}
         out["ray"] = np.concatenate([q[2]["ray"] + q[0] for q in parts])
         return out
+
+    def svox1_samples(self, g: Grid, origins, dirs, opt: Opts) -> dict:
+        """The svox1 walk's occupied-cell samples (ref_svox1_samples): dict of
+        ray, link, dt (world step), flag (1 counted, 2 forward stop, 4
+        backward stop, 8 opaque last)."""
+        keep: list = []
+        gs = self._grid(g, keep)
+        o, d, _ = self._rays(origins, dirs, None, keep)
+        ops = self._opt(opt)
+        cap = max(1024, o.shape[0] * 64)
+        while True:
+            out = dict(ray=np.empty(cap, np.int32), link=np.empty(cap, np.int32),
+                       dt=np.empty(cap, np.float32), flag=np.empty(cap, np.uint8))
+            cnt = self.lib.ref_svox1_samples(ctypes.byref(gs), _c_ptr(o), _c_ptr(d), o.shape[0],
+                                             ctypes.byref(ops), cap,
+                                             *(_c_ptr(out[k]) for k in ("ray", "link", "dt",
+                                                                         "flag")))
+            if cnt <= cap:
+                return {k: v[:cnt] for k, v in out.items()}
+            cap = int(cnt)
 
     def sample_blocks(self, g: Grid, origins, dirs, opt: Opts, clip=None, skip: bool = True,
                       block: int = 2048, post=None):
@@ -969,6 +994,126 @@ def tv_grad_sparse(links: np.ndarray, data: np.ndarray, rand_cells: np.ndarray, 
         np.add.at(grad, (rows, cols), (val * idelta)[act])
         mask[rows] = True
     return grad, mask
+
+
+def _svox1_terms(grid: Grid, dirs, smp: dict, density: torch.Tensor, sh: torch.Tensor):
+    """Per-sample FP64 density (plus 1e9 for an opaque last sample) and
+    colour + 0.5 before clamping (S, 3), and the SH basis per sample."""
+    B = grid.basis_dim
+    link = torch.from_numpy(smp["link"].astype(np.int64))
+    ray = torch.from_numpy(smp["ray"].astype(np.int64))
+    basis = eval_sh_bases(B, torch.from_numpy(np.asarray(dirs, np.float32)).to(torch.float64))[ray]
+    sigma = density[link] + torch.from_numpy((smp["flag"] & 8 != 0) * 1e9)
+    col = (sh[link].view(-1, 3, B) * basis[:, None, :]).sum(-1) + 0.5
+    return sigma, col, basis
+
+
+def _segments(ray: np.ndarray, n: int):
+    """(ray (S,) tensor, position within its ray, padded (n, L) shape) for
+    per-ray prefix sums over samples grouped by ray."""
+    r = torch.from_numpy(ray.astype(np.int64))
+    S = r.numel()
+    first = torch.ones(S, dtype=torch.bool)
+    first[1:] = r[1:] != r[:-1]
+    start = torch.nonzero(first).flatten()
+    seg = torch.cumsum(first.long(), 0) - 1
+    pos = torch.arange(S) - start[seg]
+    return r, r, pos, (n, int(pos.max()) + 1 if S else 1)
+
+
+def _ray_prefix_excl(v: torch.Tensor, r: torch.Tensor, pos: torch.Tensor, shape) -> torch.Tensor:
+    """Exclusive prefix sum of v (S,) within each ray, by shifting the
+    inclusive sums (no subtraction, which would lose the small terms next to
+    an opaque sample's 1e9)."""
+    pad = torch.zeros((shape[0], shape[1] + 1), dtype=v.dtype).index_put((r, pos + 1), v)
+    return torch.cumsum(pad, 1)[r, pos]
+
+
+def svox1_fp64(grid: Grid, origins, dirs, opt: Opts, density: torch.Tensor, sh: torch.Tensor,
+               samples: Optional[dict] = None) -> torch.Tensor:
+    """FP64 colours (n, 3) of the svox1 backend (trace_ray,
+    render_svox1_kernel.cu:50-160) over the FP32 walk's samples
+    (ref_svox1_samples), differentiable in density and sh: counted samples
+    composite with weight T (1 - exp(-dt sigma)), colour max(sh . basis +
+    0.5, 0); a ray whose transmittance before a counted sample is <=
+    stop_thresh stops after that sample and its colour is divided by
+    1 - that transmittance (no background); other rays add T_end bg; rays that
+    miss the box get bg."""
+    n = np.asarray(origins).shape[0]
+    smp = march().svox1_samples(grid, origins, dirs, opt) if samples is None else samples
+    bg = float(opt.background_brightness)
+    out = torch.full((n, 3), bg, dtype=torch.float64) + 0 * (density.sum() + sh.sum())
+    if smp["ray"].size == 0:
+        return out
+    sigma, col, _ = _svox1_terms(grid, dirs, smp, density, sh)
+    counted = torch.from_numpy((smp["flag"] & 1) != 0)
+    dt = torch.from_numpy(smp["dt"].astype(np.float64))
+    pcnt = torch.where(counted, dt * sigma, torch.zeros_like(sigma))
+    r, _, pos, shape = _segments(smp["ray"], n)
+    t_before = torch.exp(-_ray_prefix_excl(pcnt, r, pos, shape))
+    w = t_before * (1.0 - torch.exp(-pcnt)) * counted.to(torch.float64)
+    rgb = torch.zeros((n, 3), dtype=torch.float64).index_add(0, r, w[:, None] * torch.clamp_min(col, 0.0))
+    depth = torch.zeros(n, dtype=torch.float64).index_add(0, r, pcnt)
+    hit = torch.zeros(n, dtype=torch.bool)
+    hit[r] = True
+    stop = torch.from_numpy((smp["flag"] & 2) != 0)
+    stopped = torch.zeros(n, dtype=torch.bool)
+    stopped[r[stop]] = True
+    scale = torch.ones(n, dtype=torch.float64)
+    scale = scale.index_put((r[stop],), 1.0 / (1.0 - t_before[stop]))
+    lit = torch.where(stopped[:, None], rgb * scale[:, None],
+                      rgb + (torch.exp(-depth) * bg)[:, None])
+    return torch.where(hit[:, None], lit, out)
+
+
+def svox1_grads_fp64(grid: Grid, origins, dirs, opt: Opts, grad_out, color_cache,
+                     samples: Optional[dict] = None):
+    """(grad_density (R,), grad_sh (R, C), mask (R,)) of the svox1 backward
+    (trace_ray_backward, render_svox1_kernel.cu:162-275) restated in FP64
+    over the FP32 walk's samples: over the counted samples up to the one
+    after which the transmittance is <= stop_thresh, SH coefficient i of
+    channel k gets weight g_k basis_i where colour_k > 0; the density gets
+    dt (T_after sum_k [colour_k > 0] colour_k g_k - accum_after), accum
+    starting at color_cache . g and losing each sample's weight times that
+    sum; every counted sample marks its row.  Rays that miss the box get
+    nothing."""
+    n = np.asarray(origins).shape[0]
+    R, C = grid.sh.shape
+    B = grid.basis_dim
+    gd = np.zeros(R)
+    gs = np.zeros((R, C))
+    mask = np.zeros(R, bool)
+    smp = march().svox1_samples(grid, origins, dirs, opt) if samples is None else samples
+    if smp["ray"].size == 0:
+        return gd, gs, mask
+    density = torch.from_numpy(np.asarray(grid.density, np.float64).reshape(-1))
+    sh = torch.from_numpy(np.asarray(grid.sh, np.float64))
+    sigma, col, basis = (x.numpy() for x in _svox1_terms(grid, dirs, smp, density, sh))
+    g = np.asarray(grad_out, np.float64)
+    cc = np.asarray(color_cache, np.float64)
+    ray, link, flag = smp["ray"], smp["link"].astype(np.int64), smp["flag"]
+    dt = smp["dt"].astype(np.float64)
+    bounds = np.concatenate([[0], np.cumsum(np.bincount(ray, minlength=n))])
+    for r in range(n):
+        accum = float(cc[r] @ g[r])
+        log_t = 0.0
+        for i in range(bounds[r], bounds[r + 1]):
+            if not flag[i] & 1:
+                continue
+            lt = -dt[i] * sigma[i]
+            w = math.exp(log_t) * (1.0 - math.exp(lt))
+            pos = col[i] > 0.0
+            total = float((col[i] * g[r] * pos).sum())
+            for k in range(3):
+                if pos[k]:
+                    gs[link[i], k * B:(k + 1) * B] += basis[i] * (w * g[r][k])
+            log_t += lt
+            accum -= w * total
+            mask[link[i]] = True
+            gd[link[i]] += dt[i] * (total * math.exp(log_t) - accum)
+            if flag[i] & 4:
+                break
+    return gd, gs, mask
 
 
 @dataclass

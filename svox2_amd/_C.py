@@ -52,6 +52,8 @@ EXPORTED_SYMBOLS = (
     "svox2_volume_render_cuvol_stats", "svox2_volume_render_expected_term",
     "svox2_volume_render_sigma_thresh", "svox2_volume_render_nvol",
     "svox2_volume_render_nvol_backward", "svox2_volume_render_nvol_fused",
+    "svox2_volume_render_svox1", "svox2_volume_render_svox1_backward",
+    "svox2_volume_render_svox1_fused",
     "svox2_sample_grid", "svox2_sample_grid_backward",
     "svox2_tv_grad_sparse", "svox2_tv_grad_sparse_run", "svox2_lumisphere_tv_grad_sparse",
     "svox2_msi_tv_grad_sparse", "svox2_rmsprop_step", "svox2_sgd_step",
@@ -168,6 +170,9 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         "svox2_volume_render_nvol": [G, vp, vp, vp, i64, O, vp, vp],
         "svox2_volume_render_nvol_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
         "svox2_volume_render_nvol_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, Gr, vp],
+        "svox2_volume_render_svox1": [G, vp, vp, vp, i64, O, vp, vp],
+        "svox2_volume_render_svox1_backward": [G, vp, vp, vp, i64, O, vp, vp, Gr, vp],
+        "svox2_volume_render_svox1_fused": [G, vp, vp, vp, i64, O, vp, f32, f32, vp, Gr, vp],
         "svox2_sample_grid": [G, vp, i64, i32, vp, vp, vp],
         "svox2_sample_grid_backward": [G, vp, i64, vp, vp, vp, vp, i64, i64, i32, vp],
         "svox2_tv_grad_sparse": [vp, ctypes.POINTER(i32), vp, i32, i64, i32, i64, i64, vp, i64,
@@ -726,6 +731,60 @@ def volume_render_cuvol_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderO
               o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(rgb_gt), float(beta_loss),
               float(sparsity_loss), _ptr(rgb_out), _ptr(log_t), ctypes.byref(gs), _ptr(ws),
               nbytes, _stream(o))
+
+
+def _no_clip(rays: RaysSpec, what: str):
+    if getattr(rays, "clip", None) is not None:
+        raise RuntimeError(f"{what}: the svox1 backend takes no ray clips")
+
+
+def volume_render_svox1(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions) -> torch.Tensor:
+    """volume_render_svox1 (render_svox1_kernel.cu:334-349): (n, 3) colours."""
+    _no_clip(rays, "volume_render_svox1")
+    g = _grid_struct(grid)
+    o, d, _ = _rays(rays)
+    out = torch.empty(o.size(0), 3, dtype=torch.float32, device=o.device)
+    with _on_device(o):
+        _call("svox2_volume_render_svox1", ctypes.byref(g), _ptr(o), _ptr(d), None, o.size(0),
+              ctypes.byref(_opt_struct(opt)), _ptr(out), _stream(o))
+    return out
+
+
+def volume_render_svox1_backward(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                                 grad_out: torch.Tensor, color_cache: torch.Tensor,
+                                 grads: GridOutputGrads) -> None:
+    """volume_render_svox1_backward (render_svox1_kernel.cu:351-378)."""
+    _no_clip(rays, "volume_render_svox1_backward")
+    g = _grid_struct(grid)
+    o, d, _ = _rays(rays)
+    _check(grad_out, "grad_out", (torch.float32,), 2)
+    _check(color_cache, "color_cache", (torch.float32,), 2)
+    if grad_out.shape != o.shape or color_cache.shape != o.shape:
+        raise RuntimeError("grad_out / color_cache must be (n, 3)")
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    with _on_device(o):
+        _call("svox2_volume_render_svox1_backward", ctypes.byref(g), _ptr(o), _ptr(d), None,
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(grad_out), _ptr(color_cache),
+              ctypes.byref(gs), _stream(o))
+
+
+def volume_render_svox1_fused(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions,
+                              rgb_gt: torch.Tensor, beta_loss: float, sparsity_loss: float,
+                              rgb_out: torch.Tensor, grads: GridOutputGrads) -> None:
+    """volume_render_svox1_fused (render_svox1_kernel.cu:380-419); beta_loss and
+    sparsity_loss are ignored, as in the reference."""
+    _no_clip(rays, "volume_render_svox1_fused")
+    g = _grid_struct(grid)
+    o, d, _ = _rays(rays)
+    _check(rgb_gt, "rgb_gt", (torch.float32,), 2)
+    _check(rgb_out, "rgb_out", (torch.float32,), 2)
+    if rgb_gt.shape != o.shape or rgb_out.shape != o.shape:
+        raise RuntimeError("rgb_gt / rgb_out must be (n, 3)")
+    gs = _grads_struct(grads, g.rows, g.sh_data_dim)
+    with _on_device(o):
+        _call("svox2_volume_render_svox1_fused", ctypes.byref(g), _ptr(o), _ptr(d), None,
+              o.size(0), ctypes.byref(_opt_struct(opt)), _ptr(rgb_gt), float(beta_loss),
+              float(sparsity_loss), _ptr(rgb_out), ctypes.byref(gs), _stream(o))
 
 
 def volume_render_nvol(grid: SparseGridSpec, rays: RaysSpec, opt: RenderOptions) -> torch.Tensor:

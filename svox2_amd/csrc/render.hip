@@ -55,6 +55,7 @@ struct OptView {
     int step_ulp_max;  // largest float exponent field e with step a whole multiple of 2^(e - 150)
     float sigma_thresh;
     float log_stop;  // stop when log transmittance < log_stop
+    float stop;      // the stop threshold itself (the svox1 backend compares transmittance)
     float near_clip;
     int spheric;
     int last_opaque;
@@ -1969,6 +1970,219 @@ __global__ __launch_bounds__(kBlock) void k_render_nvol_backward(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The svox1 backend (render_svox1_kernel.cu): nearest-neighbour rendering by a
+// walk over whole cells.  The grid origin is shifted by +0.5 and the box is
+// [0, size - 1]; from t = max(0, box entry) an occupied cell is one sample of
+// its own density and colour, whose step is the distance to the cell's exit
+// plus 1e-2; an empty block (link < 0: 2^(-link - 1) cells on a side, as
+// accelerate() writes) is skipped to its exit plus 1e-2.  Each step depends
+// on the one before, so a thread walks one ray; the float operations are in
+// the reference's order (no contraction) so that cell decisions match the C
+// restatement bit for bit.  Links >= rows count as empty single cells.
+// ---------------------------------------------------------------------------
+constexpr int kNnBlock = 256;
+
+struct NnRay {
+    float o[3], d[3], inv[3];
+    float ds;       // world length per grid unit
+    float t, tmax;  // walk interval (grid units)
+};
+
+// Ray set-up (trace_ray, :55-92).  skip_flat: the backward's box test, which
+// leaves out axes the direction does not move along (:185-198); the forward
+// applies all three.  Returns whether the ray meets the box.
+template <typename DT>
+__device__ __forceinline__ bool nn_setup(const GridView<DT>& g, const float* wo, const float* wd,
+                                         bool skip_flat, NnRay& r) {
+    float d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.o[a] = fmaf(wo[a], g.scaling[a], g.offset[a]);
+        d[a] = wd[a] * g.scaling[a];
+    }
+    r.ds = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.d[a] = d[a] * r.ds;
+        r.inv[a] = r.d[a] == 0.f ? 1e9f : static_cast<float>(1.0 / static_cast<double>(r.d[a]));
+    }
+    r.t = 0.f;
+    r.tmax = 1e9f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.o[a] += 0.5f;
+        const float t1 = (0.0f - r.o[a]) * r.inv[a];
+        const float t2 = (static_cast<float>(g.size[a]) - 1.f - r.o[a]) * r.inv[a];
+        if (!skip_flat || r.d[a] != 0.f) {
+            r.t = fmaxf(r.t, fminf(t1, t2));
+            r.tmax = fminf(r.tmax, fmaxf(t1, t2));
+        }
+    }
+    return !(r.t > r.tmax);
+}
+
+// One step of the walk from r.t: an occupied cell's link, with its step dt
+// (_intersect_aabb_unit, render_util.cuh:448-461, plus 1e-2), or -1 after
+// skipping an empty block (compute_skip_dist_nn, :23-48).  Advances r.t.
+template <typename DT>
+__device__ __forceinline__ int nn_step(const GridView<DT>& g, NnRay& r, float& dt) {
+    float p[3];
+    int l[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        p[a] = r.o[a] + r.t * r.d[a];
+        p[a] = fminf(fmaxf(p[a], 0.f), static_cast<float>(g.size[a]) - 1.f);
+        l[a] = min(static_cast<int>(p[a]), g.size[a] - 1);
+        p[a] -= static_cast<float>(l[a]);
+    }
+    const int link = g.links[l[0] * g.stride_x + l[1] * g.stride_y + l[2]];
+    if (g.valid(link)) {
+        float e = 1e9f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float t1 = -p[a] * r.inv[a];
+            const float t2 = t1 + r.inv[a];
+            e = fminf(e, fmaxf(t1, t2));
+        }
+        dt = e + 1e-2f;
+        r.t += dt;
+        return link;
+    }
+    const uint32_t dist = link < 0 ? static_cast<uint32_t>(-link) : 1u;
+    const int shift = static_cast<int>(min(dist - 1u, 30u));
+    const int side = 1 << shift;
+    float smax = 1e9f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ul = ((l[a] >> shift) << shift) - l[a];
+        const float t1 = (static_cast<float>(ul) - p[a]) * r.inv[a];
+        const float t2 = (static_cast<float>(ul + side) - p[a]) * r.inv[a];
+        if (r.d[a] != 0.f) smax = fminf(smax, fmaxf(t1, t2));
+    }
+    r.t += fmaxf(smax, 0.f) + 1e-2f;
+    return -1;
+}
+
+// Channel k's colour + 0.5 of a row, summed coefficient by coefficient.
+template <typename DT, int B>
+__device__ __forceinline__ float nn_colour(const GridView<DT>& g, int link, int k,
+                                           const float* basis) {
+    const int64_t base = static_cast<int64_t>(link) * g.sstride + k * B;
+    float v = 0.5f;
+#pragma unroll
+    for (int i = 0; i < B; ++i) v += basis[i] * ld(g.sh, base + i);
+    return v;
+}
+
+// svox1 forward (trace_ray, :50-160): rgb_out [n][3], one thread per ray.
+template <typename DT, int B>
+__global__ __launch_bounds__(kNnBlock) void k_render_svox1(GridView<DT> g, OptView o,
+                                                           const float* __restrict__ origins,
+                                                           const float* __restrict__ dirs,
+                                                           int64_t n, float* __restrict__ rgb_out) {
+    const int64_t ray = static_cast<int64_t>(blockIdx.x) * kNnBlock + threadIdx.x;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    float* out = rgb_out + 3 * ray;
+    NnRay r;
+    if (!nn_setup(g, wo, wd, false, r)) {
+        out[0] = out[1] = out[2] = o.bg;
+        return;
+    }
+    const RayCtx<B> rc(wd);
+    float acc[3] = {0.f, 0.f, 0.f}, log_t = 0.f;
+    int guard = 0;
+    while (r.t < r.tmax && ++guard < kMaxIters) {
+        float dt;
+        const int link = nn_step(g, r, dt);
+        if (link < 0) continue;
+        float sigma = ld(g.density, static_cast<int64_t>(link) * g.dstride);
+        if (o.last_opaque && r.t + o.step > r.tmax) sigma += 1e9f;
+        if (!(sigma > o.sigma_thresh)) continue;
+        const float lt = -(dt * r.ds) * sigma;
+        const float tr = expf(log_t);
+        const float w = tr * (1.f - expf(lt));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += w * fmaxf(nn_colour<DT, B>(g, link, k, rc.basis), 0.f);
+        log_t += lt;
+        if (tr <= o.stop) {
+            // Full opacity: the colour is renormalised and the background
+            // left out (:137-145).
+            const float scale = static_cast<float>(1.0 / (1.0 - static_cast<double>(tr)));
+            out[0] = acc[0] * scale;
+            out[1] = acc[1] * scale;
+            out[2] = acc[2] * scale;
+            return;
+        }
+    }
+    const float tb = expf(log_t) * o.bg;
+    out[0] = acc[0] + tb;
+    out[1] = acc[1] + tb;
+    out[2] = acc[2] + tb;
+}
+
+// svox1 backward (trace_ray_backward, :162-275), one thread per ray.
+// grad_in: dL/drgb [n][3], or with rgb_residual the targets (the fused MSE
+// step: g = (color_cache - grad_in) * norm).  It stops after the sample whose
+// transmittance after it is <= stop_thresh, as the reference's does (its
+// forward stops one counted sample later and renormalises; the backward
+// differentiates neither).
+template <typename DT, int B>
+__global__ __launch_bounds__(kNnBlock) void k_render_svox1_backward(
+        GridView<DT> g, OptView o, const float* __restrict__ origins,
+        const float* __restrict__ dirs, int64_t n, const float* __restrict__ grad_in,
+        const float* __restrict__ color_cache, int rgb_residual, float norm, GradOut go) {
+    const int64_t ray = static_cast<int64_t>(blockIdx.x) * kNnBlock + threadIdx.x;
+    if (ray >= n) return;
+    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
+    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    NnRay r;
+    if (!nn_setup(g, wo, wd, true, r)) return;
+    float gq[3];
+    const float* cc = color_cache + 3 * ray;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        gq[k] = rgb_residual ? (cc[k] - grad_in[3 * ray + k]) * norm : grad_in[3 * ray + k];
+    const RayCtx<B> rc(wd);
+    float accum = cc[0] * gq[0] + cc[1] * gq[1] + cc[2] * gq[2];
+    float log_t = 0.f;
+    int guard = 0;
+    while (r.t < r.tmax && ++guard < kMaxIters) {
+        float dt;
+        const int link = nn_step(g, r, dt);
+        if (link < 0) continue;
+        float sigma = ld(g.density, static_cast<int64_t>(link) * g.dstride);
+        if (o.last_opaque && r.t + o.step > r.tmax) sigma += 1e9f;
+        if (!(sigma > o.sigma_thresh)) continue;
+        const float dtw = dt * r.ds;
+        const float lt = -dtw * sigma;
+        const float w = expf(log_t) * (1.f - expf(lt));
+        float total = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float c = nn_colour<DT, B>(g, link, k, rc.basis);
+            if (c > 0.f) {
+                total += c * gq[k];
+                const float gw = w * gq[k];
+                if (go.gs != nullptr) {
+                    float* row = go.gs + static_cast<int64_t>(link) * go.gss + k * B;
+#pragma unroll
+                    for (int i = 0; i < B; ++i) atomic_add_f32(row + i, rc.basis[i] * gw);
+                }
+            }
+        }
+        log_t += lt;
+        accum -= w * total;
+        if (go.mask != nullptr) go.mask[link] = 1;
+        if (go.gd != nullptr)
+            atomic_add_f32(go.gd + static_cast<int64_t>(link) * go.gds,
+                           dtw * (total * expf(log_t) - accum));
+        if (expf(log_t) <= o.stop) return;
+    }
+}
+
 // t of lane k of a chunk whose lane 0 has t0: the float chain t0 + step + ...
 // + step (k additions), by one fma when every addition is exact (see
 // place_chunk), else replayed.
@@ -2570,6 +2784,7 @@ int make_opt(const svox2_render_options_t* opt, OptView& o) {
     }
     o.sigma_thresh = opt->sigma_thresh;
     o.log_stop = log_stop_threshold(opt->stop_thresh);
+    o.stop = opt->stop_thresh;
     o.near_clip = opt->near_clip;
     o.spheric = opt->use_spheric_clip != 0;
     o.last_opaque = opt->last_sample_opaque != 0;
@@ -2867,6 +3082,107 @@ extern "C" int svox2_volume_render_nvol_fused(const svox2_grid_t* grid, const fl
                             sparsity_loss, go, as_stream(stream))))
         return st;
     return check_launch("volume_render_nvol_fused");
+}
+
+// svox1 backend entry points (render_svox1_kernel.cu:334-419).  The reference
+// takes no ray clips and no background for this backend: a clip is refused,
+// a background ignored.
+namespace svox2amd {
+namespace {
+
+int svox1_opt(const svox2_grid_t* grid, const float* origins, const float* dirs,
+              const float* ray_clip, int64_t n_rays, const svox2_render_options_t* opt,
+              OptView& o) {
+    int st = check_grid(grid, true);
+    if (st) return st;
+    if ((st = make_opt(opt, o))) return st;
+    if ((st = check_rays(origins, dirs, n_rays))) return st;
+    SVOX2_REQUIRE(ray_clip == nullptr, "the svox1 backend takes no ray clips");
+    return ok();
+}
+
+int svox1_backward(const svox2_grid_t* grid, const OptView& o, const float* origins,
+                   const float* dirs, int64_t n_rays, const float* grad_in,
+                   const float* color_cache, int rgb_residual, float norm, const GradOut& go,
+                   hipStream_t s) {
+    const unsigned blocks = blocks_for(n_rays, kNnBlock);
+    return dispatch(grid, [&](auto g, auto b, auto dtp) {
+        constexpr int B = decltype(b)::value;
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        hipLaunchKernelGGL((k_render_svox1_backward<DT, B>), dim3(blocks), dim3(kNnBlock), 0, s,
+                           g, o, origins, dirs, n_rays, grad_in, color_cache, rgb_residual, norm,
+                           go);
+    });
+}
+
+}  // namespace
+}  // namespace svox2amd
+
+extern "C" int svox2_volume_render_svox1(const svox2_grid_t* grid, const float* origins,
+                                         const float* dirs, const float* ray_clip,
+                                         int64_t n_rays, const svox2_render_options_t* opt,
+                                         float* rgb_out, void* stream) {
+    OptView o;
+    int st = svox1_opt(grid, origins, dirs, ray_clip, n_rays, opt, o);
+    if (st) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(rgb_out != nullptr, "rgb_out is NULL");
+    const unsigned blocks = blocks_for(n_rays, kNnBlock);
+    st = dispatch(grid, [&](auto g, auto b, auto dtp) {
+        constexpr int B = decltype(b)::value;
+        using DT = std::remove_pointer_t<decltype(dtp)>;
+        hipLaunchKernelGGL((k_render_svox1<DT, B>), dim3(blocks), dim3(kNnBlock), 0,
+                           as_stream(stream), g, o, origins, dirs, n_rays, rgb_out);
+    });
+    if (st) return st;
+    return check_launch("volume_render_svox1");
+}
+
+extern "C" int svox2_volume_render_svox1_backward(const svox2_grid_t* grid, const float* origins,
+                                                  const float* dirs, const float* ray_clip,
+                                                  int64_t n_rays,
+                                                  const svox2_render_options_t* opt,
+                                                  const float* grad_out,
+                                                  const float* color_cache,
+                                                  const svox2_grads_t* grads, void* stream) {
+    OptView o;
+    int st = svox1_opt(grid, origins, dirs, ray_clip, n_rays, opt, o);
+    if (st) return st;
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(grad_out != nullptr && color_cache != nullptr,
+                  "grad_out / color_cache are NULL");
+    if ((st = svox1_backward(grid, o, origins, dirs, n_rays, grad_out, color_cache, 0, 1.f, go,
+                             as_stream(stream))))
+        return st;
+    return check_launch("volume_render_svox1_backward");
+}
+
+extern "C" int svox2_volume_render_svox1_fused(const svox2_grid_t* grid, const float* origins,
+                                               const float* dirs, const float* ray_clip,
+                                               int64_t n_rays,
+                                               const svox2_render_options_t* opt,
+                                               const float* rgb_gt, float beta_loss,
+                                               float sparsity_loss, float* rgb_out,
+                                               const svox2_grads_t* grads, void* stream) {
+    (void)beta_loss;  // not supported by the reference's svox1 (:395-396)
+    (void)sparsity_loss;
+    OptView o;
+    int st = svox1_opt(grid, origins, dirs, ray_clip, n_rays, opt, o);
+    if (st) return st;
+    GradOut go;
+    if ((st = check_grads(grads, grid->sh_data_dim, go))) return st;
+    if (n_rays == 0) return ok();
+    SVOX2_REQUIRE(rgb_gt != nullptr && rgb_out != nullptr, "rgb_gt / rgb_out are NULL");
+    if ((st = svox2_volume_render_svox1(grid, origins, dirs, nullptr, n_rays, opt, rgb_out,
+                                        stream)))
+        return st;
+    const float norm = 2.f / static_cast<float>(3 * n_rays);
+    if ((st = svox1_backward(grid, o, origins, dirs, n_rays, rgb_gt, rgb_out, 1, norm, go,
+                             as_stream(stream))))
+        return st;
+    return check_launch("volume_render_svox1_fused");
 }
 
 extern "C" int svox2_volume_render_cuvol_stats(const svox2_grid_t* grid, const float* origins,
