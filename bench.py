@@ -16,7 +16,7 @@ cores: the C port of the reference's fused forward+backward
 path (_volume_render_gradcheck_lerp + autograd) on the 64^3 dense workload.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
-  python bench.py --only {small,side,image,serve,stress}   (one side measurement)
+  python bench.py --only {small,side,image,serve,stress,next}   (one side measurement)
 """
 from __future__ import annotations
 
@@ -580,7 +580,92 @@ def pmc_run(reps: int = 3):
                                   "copy_bytes": 2 ** 30}}), flush=True)
 
 
-ONLY_PARTS = ("small", "side", "image", "serve", "stress")
+ONLY_PARTS = ("small", "side", "image", "serve", "stress", "next")
+
+
+def next_components(wl, reps: int = 10) -> dict:
+    """The components built around the march, timed on the headline workload
+    (the 512^3 SH-2 float32 grid, one 65536-ray batch) with HIP events on
+    fixed inputs, ms per call:
+    - depth_expected / depth_sigma: volume_render_depth in both modes;
+    - cuvol_fused, nvol_fused, svox1_fused: the three backends' fused MSE
+      steps through SparseGrid.volume_render_fused;
+    - cuvol_fused_background: the fused step with a 64-layer MSI background
+      of 512 x 256 pixels (131072 columns, 134 MB of float32);
+    - msi_tv: the background's TV over 1% of its cells (a contiguous run),
+      with algorithmic bytes per cell: 3 links (12 B) and, per channel, 4
+      values read (16 B), up to 4 gradient atomics (32 B) and a mask byte;
+    - lumisphere_tv: the lumisphere TV kernel over 1% of the grid's cells
+      with fixed basis values (cells per second);
+    - sgd: sgd_step over every row of density and SH (16 B per element: data
+      and gradient each read and written; lr 0 leaves the grid unchanged).
+    The grid is left as it was found (no background, cuvol backend)."""
+    import svox2_amd
+    from svox2_amd import _C
+    from torch import nn
+    g = wl.grid(torch.float32)
+    dev = wl.dev
+    o, d, gt = wl.batches(BATCH, 1)[0]
+    rays = svox2_amd.Rays(o, d)
+    n = o.size(0)
+
+    def rate(ms):
+        return dict(ms=ms, rays_per_s=n / (ms * 1e-3))
+
+    out = {}
+    with torch.no_grad():
+        out["depth_expected"] = rate(_event_ms(lambda: g.volume_render_depth(rays), reps))
+        out["depth_sigma"] = rate(_event_ms(lambda: g.volume_render_depth(rays, sigma_thresh=1.0),
+                                            reps))
+    out["cuvol_fused"] = rate(_event_ms(lambda: g.volume_render_fused(rays, gt), reps))
+    try:
+        for backend in ("nvol", "svox1"):
+            g.opt.backend = backend
+            out[f"{backend}_fused"] = rate(_event_ms(lambda: g.volume_render_fused(rays, gt),
+                                                     reps))
+    finally:
+        g.opt.backend = "cuvol"
+
+    reso, nl = 256, 64
+    gen = torch.Generator(device=dev).manual_seed(11)
+    data = torch.empty(2 * reso * reso, nl, 4, device=dev)
+    data[..., :3] = torch.randn(2 * reso * reso, nl, 3, generator=gen, device=dev) * 0.5
+    data[..., 3] = torch.rand(2 * reso * reso, nl, generator=gen, device=dev) * 2.0
+    try:
+        g.background_nlayers, g.background_reso = nl, reso
+        g.background_links = torch.arange(2 * reso * reso, dtype=torch.int32,
+                                          device=dev).view(2 * reso, reso)
+        g.background_data = nn.Parameter(data)
+        out["cuvol_fused_background"] = rate(_event_ms(lambda: g.volume_render_fused(rays, gt),
+                                                       reps))
+        cells = g._get_rand_cells_background(0.01, contiguous=True)
+        ms = _event_ms(lambda: _C.msi_tv_grad_sparse(
+            g.background_links, g.background_data.data, cells, g.sparse_background_indexer,
+            1e-3, 1e-3, g.background_data.grad), reps)
+        out["msi_tv"] = dict(_roof(ms, cells.numel() * (12 + 4 * (16 + 32 + 1))),
+                             cells=int(cells.numel()))
+    finally:
+        g.background_nlayers, g.background_links = 0, None
+        g.background_data = nn.Parameter(torch.empty(0, 0, 0, device=dev), requires_grad=False)
+        g.sparse_background_indexer = None
+
+    cells = g._get_rand_cells(0.01, contiguous=True)
+    basis = torch.full((g.basis_dim,), 0.3, device=dev)
+    basis_u = torch.full((g.basis_dim,), 0.25, device=dev)
+    holder = _C.GridOutputGrads()
+    holder.grad_sh_out = g.sh_data.grad
+    spec = g._to_cpp()
+    ms = _event_ms(lambda: _C.lumisphere_tv_grad_sparse(spec, cells, basis, basis_u, 1e-3, -1.0,
+                                                        -1.0, 1.0, holder), reps)
+    out["lumisphere_tv"] = dict(ms=ms, cells=int(cells.numel()),
+                                cells_per_s=cells.numel() / (ms * 1e-3))
+    every = torch.empty((), device=dev)
+    elems = g.density_data.numel() + g.sh_data.numel()
+    ms = _event_ms(lambda: (_C.sgd_step(g.density_data.data, g.density_data.grad, every, 0.0,
+                                        0.0),
+                            _C.sgd_step(g.sh_data.data, g.sh_data.grad, every, 0.0, 0.0)), reps)
+    out["sgd"] = dict(_roof(ms, 16 * elems), elements=int(elems))
+    return out
 
 
 def run_only(part: str, dev, steps: int, warmup: int) -> dict:
@@ -588,7 +673,7 @@ def run_only(part: str, dev, steps: int, warmup: int) -> dict:
     fused steps (float32 and bfloat16), `side` the 256^3 training step and
     the TV / RMSprop / accel_dist_prop / sample kernels, `image` the 800x800
     forward images, `serve` the serving loop at 200 requests/s, `stress` the
-    million-ray autograd step."""
+    million-ray autograd step, `next` the components around the march."""
     wl = Workload(dev)
     if part == "small":
         small = wl.batches(5000, POOL)
@@ -603,6 +688,8 @@ def run_only(part: str, dev, steps: int, warmup: int) -> dict:
         return serve_bench(wl, qps_list=(200.0,), n=200)
     if part == "stress":
         return stress_1m(wl, wl.grid(torch.float32))
+    if part == "next":
+        return next_components(wl)
     raise ValueError(part)
 
 
@@ -683,6 +770,10 @@ def main():
         extra["image_800_bf16"] = image_forward(gb, wl.cams)
         log(f"800x800 image: fp32 {extra['image_800_fp32']['ms']:.3f} ms, bf16 "
             f"{extra['image_800_bf16']['ms']:.3f} ms")
+        log("components around the march (depth, nvol, svox1, background, MSI / lumisphere TV, SGD)")
+        extra["next"] = next_components(wl)
+        for k, v in extra["next"].items():
+            log(f"{k}: {v['ms']:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline")
