@@ -20,7 +20,9 @@
 // of the samples' optical depths, and the stop threshold cuts the chunk at the
 // first sample whose transmittance falls below it.  The reference walks the
 // samples one by one in a thread per ray; the sums here are the same terms
-// added in scan order.
+// added in scan order.  The backward then scatters sample by sample with a
+// lane per (column, layer, channel), so that each atomic instruction covers
+// 8 consecutive floats of each column.
 #include "common.hpp"
 
 namespace svox2amd {
@@ -173,28 +175,6 @@ __device__ __forceinline__ float4 bg_value(const BgView& b, const BgSample& s,
     const float4 v10 = bg_column(b, c.row[2], s.f[2]);
     const float4 v11 = bg_column(b, c.row[3], s.f[2]);
     return lerp4(lerp4(v00, v01, s.f[1]), lerp4(v10, v11, s.f[1]), s.f[0]);
-}
-
-// Adds v times each column's trilinear weight into channel ch of its two
-// layers; the density channel also marks the layers (trilerp_backward_bg_one,
-// render_util.cuh:191-232).
-__device__ __forceinline__ void bg_scatter(const BgView& b, const BgSample& s,
-                                           const BgCorners& c, int ch, float v, bool mark) {
-    const float wx[2] = {1.f - s.f[0], s.f[0]};
-    const float wy[2] = {1.f - s.f[1], s.f[1]};
-    const float az = 1.f - s.f[2];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (c.row[k] < 0) continue;
-        const float val = wy[k & 1] * (wx[k >> 1] * v);
-        float* gp = b.grad + c.row[k] * 4 + ch;
-        atomic_add_f32(gp, val * az);
-        atomic_add_f32(gp + 4, val * s.f[2]);
-        if (mark && b.mask != nullptr) {
-            b.mask[c.row[k]] = 1;
-            b.mask[c.row[k] + 1] = 1;
-        }
-    }
 }
 
 // 32-lane group helpers (the group is one half of the wavefront; every lane of
@@ -360,15 +340,57 @@ __global__ __launch_bounds__(kBgBlock) void k_background_backward(
             for (int c = 0; c < 3; ++c) total += fmaxf(col[c], 0.f) * gq[c];
         }
         const float acc_incl = bg_scan(w * total, lane);
+        // This sample's terms: per channel (colours where the colour is
+        // positive, the density term always) and its trilinear weights.
+        float v[4] = {0.f, 0.f, 0.f, 0.f}, wxy[4] = {0.f, 0.f, 0.f, 0.f};
+        float wz0 = 0.f, wz1 = 0.f;
+        int rows4[4] = {-1, -1, -1, -1};
         if (k.live) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (col[c] > 0.f) bg_scatter(b, k.s, k.c, c, kShC0 * w * gq[c], false);
+            for (int c = 0; c < 3; ++c) v[c] = col[c] > 0.f ? kShC0 * w * gq[c] : 0.f;
             const float accum_after = accum - acc_incl;
             float gs = r.ws * (k.il - k.s.invr) * (total * __expf(k.lt_after) - accum_after);
             if (sparsity > 0.f)
                 gs += sparsity * (4.f * k.v.w / (1.f + 2.f * (k.v.w * k.v.w)));
-            bg_scatter(b, k.s, k.c, 3, gs, true);
+            v[3] = gs;
+            const float ax = 1.f - k.s.f[0], ay = 1.f - k.s.f[1];
+            wxy[0] = ay * ax;
+            wxy[1] = k.s.f[1] * ax;
+            wxy[2] = ay * k.s.f[0];
+            wxy[3] = k.s.f[1] * k.s.f[0];
+            wz0 = 1.f - k.s.f[2];
+            wz1 = k.s.f[2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rows4[q] = static_cast<int>(k.c.row[q]);
+        }
+        // Scatter sample by sample, lane = (column, layer, channel): one
+        // atomic instruction per sample whose lanes add into 8 consecutive
+        // floats (2 layers x 4 channels) of each of the 4 columns, so they
+        // coalesce into a few line requests (a lane per sample issuing its
+        // own 32 atomics made each one a request of its own).
+        const int q = lane >> 3, layer = (lane >> 2) & 1, ch = lane & 3;
+        uint32_t todo = bg_ballot(k.live);
+        while (todo != 0u) {
+            const int j = __ffs(todo) - 1;
+            todo &= todo - 1u;
+            const float vj[4] = {__shfl(v[0], j, kBgGroup), __shfl(v[1], j, kBgGroup),
+                                 __shfl(v[2], j, kBgGroup), __shfl(v[3], j, kBgGroup)};
+            const float wj[4] = {__shfl(wxy[0], j, kBgGroup), __shfl(wxy[1], j, kBgGroup),
+                                 __shfl(wxy[2], j, kBgGroup), __shfl(wxy[3], j, kBgGroup)};
+            const int rj[4] = {__shfl(rows4[0], j, kBgGroup), __shfl(rows4[1], j, kBgGroup),
+                               __shfl(rows4[2], j, kBgGroup), __shfl(rows4[3], j, kBgGroup)};
+            // Both shuffles run in every lane: a lane whose source lane is
+            // switched off by a branch would read 0.
+            const float wz0j = __shfl(wz0, j, kBgGroup), wz1j = __shfl(wz1, j, kBgGroup);
+            const float wzj = layer ? wz1j : wz0j;
+            const int row = q == 0 ? rj[0] : (q == 1 ? rj[1] : (q == 2 ? rj[2] : rj[3]));
+            const float wq = q == 0 ? wj[0] : (q == 1 ? wj[1] : (q == 2 ? wj[2] : wj[3]));
+            const float vc = ch == 0 ? vj[0] : (ch == 1 ? vj[1] : (ch == 2 ? vj[2] : vj[3]));
+            if (row >= 0) {
+                const float val = (wq * vc) * wzj;
+                if (val != 0.f) atomic_add_f32(b.grad + static_cast<int64_t>(row + layer) * 4 + ch, val);
+                if (ch == 0 && b.mask != nullptr) b.mask[row + layer] = 1;
+            }
         }
         lt -= __shfl(k.incl, k.stop_lane, kBgGroup);
         accum -= __shfl(acc_incl, k.stop_lane, kBgGroup);
@@ -438,8 +460,9 @@ int make_bg_view(const svox2_grid_t* g, float step, float bg, float stop, BgView
     SVOX2_REQUIRE(g->background_nlayers >= 2, "background_nlayers %d must be >= 2",
                   g->background_nlayers);
     SVOX2_REQUIRE(g->background_rows >= 0 &&
-                      g->background_rows * g->background_nlayers * 4 < (INT64_C(1) << 40),
-                  "background_rows %lld out of range", (long long)g->background_rows);
+                      g->background_rows * g->background_nlayers < INT_MAX,
+                  "background_rows %lld x nlayers %d exceeds int32 layer indices",
+                  (long long)g->background_rows, g->background_nlayers);
     SVOX2_REQUIRE(reinterpret_cast<uintptr_t>(g->background_data) % 16 == 0,
                   "background_data must be 16-byte aligned");
     SVOX2_REQUIRE(step > 0.f, "step_size must be > 0");
