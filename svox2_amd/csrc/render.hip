@@ -1774,6 +1774,19 @@ SVOX2_FWD_KERNEL void k_render_depth(GridView<DT> g, OptView o,
 // opacity, the same in every lane of the group.
 __device__ __forceinline__ float nvol_accumulate(int lane, bool acc, float e, float& alpha,
                                                  int& sat) {
+    // Every accumulated increase 1 - e non-negative (e <= 1) in the whole
+    // wavefront: min(A + x, 1) applied in lane order is min(A + prefix sum,
+    // 1), so one inclusive scan replaces the serial walk.
+    if (__ballot(acc && e > 1.f) == 0ull) {
+        const float a = acc ? 1.f - e : 0.f;
+        const float incl = group_scan(a);
+        const float after = fminf(alpha + incl, 1.f);
+        const float before = fminf(alpha + (incl - a), 1.f);
+        const uint32_t sm = group_ballot(acc && after >= 1.f);
+        sat = sm ? __ffs(sm) - 1 : 32;
+        alpha = fminf(alpha + group_lane(incl, 31), 1.f);
+        return acc && lane <= sat ? after - before : 0.f;
+    }
     const uint32_t m = group_ballot(acc);
     const int n_rep = wave_last_set(acc);
     float my_w = 0.f;
