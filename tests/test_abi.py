@@ -46,6 +46,11 @@ int main(void) {
   printf("%zu %zu\n", sizeof(svox2_render_options_t), offsetof(svox2_render_options_t, last_sample_opaque));
   printf("%zu %zu %zu\n", sizeof(svox2_camera_t), offsetof(svox2_camera_t, width), offsetof(svox2_camera_t, ndc_coeffy));
   printf("%zu %zu\n", sizeof(svox2_grads_t), offsetof(svox2_grads_t, grad_sh_stride));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(svox2_grid_t, background_links),
+         offsetof(svox2_grid_t, background_rows), offsetof(svox2_grid_t, background_reso),
+         offsetof(svox2_grid_t, background_nlayers),
+         offsetof(svox2_grads_t, grad_background_out),
+         offsetof(svox2_grads_t, mask_background_out));
   return 0;
 }
 ''')
@@ -58,7 +63,10 @@ int main(void) {
     py = [ctypes.sizeof(G), G.rows.offset, G.size.offset, G.storage.offset, G.offset.offset,
           G.scaling.offset, G.sh_stride.offset, G.cell_map.offset, ctypes.sizeof(O),
           O.last_sample_opaque.offset, ctypes.sizeof(Cm),
-          Cm.width.offset, Cm.ndc_coeffy.offset, ctypes.sizeof(Gr), Gr.grad_sh_stride.offset]
+          Cm.width.offset, Cm.ndc_coeffy.offset, ctypes.sizeof(Gr), Gr.grad_sh_stride.offset,
+          G.background_links.offset, G.background_rows.offset, G.background_reso.offset,
+          G.background_nlayers.offset, Gr.grad_background_out.offset,
+          Gr.mask_background_out.offset]
     assert c == py
 
 
