@@ -2136,63 +2136,103 @@ __global__ __launch_bounds__(kNnBlock) void k_render_svox1(GridView<DT> g, OptVi
     out[2] = acc[2] + tb;
 }
 
-// svox1 backward (trace_ray_backward, :162-275), one thread per ray.
+// svox1 backward (trace_ray_backward, :162-275), one ray per lane.
 // grad_in: dL/drgb [n][3], or with rgb_residual the targets (the fused MSE
 // step: g = (color_cache - grad_in) * norm).  It stops after the sample whose
 // transmittance after it is <= stop_thresh, as the reference's does (its
 // forward stops one counted sample later and renormalises; the backward
-// differentiates neither).
+// differentiates neither).  The lanes step their walks together under a
+// wave-uniform loop, and after each step the wave scatters that step's
+// counted samples one by one with a lane per SH coefficient (and one for the
+// density): each sample's 27 gradient atomics then fall in one row in one
+// instruction, instead of 27 instructions whose lanes each hit a row of
+// their own.
 template <typename DT, int B>
 __global__ __launch_bounds__(kNnBlock) void k_render_svox1_backward(
         GridView<DT> g, OptView o, const float* __restrict__ origins,
         const float* __restrict__ dirs, int64_t n, const float* __restrict__ grad_in,
         const float* __restrict__ color_cache, int rgb_residual, float norm, GradOut go) {
+    constexpr int C = 3 * B;
     const int64_t ray = static_cast<int64_t>(blockIdx.x) * kNnBlock + threadIdx.x;
-    if (ray >= n) return;
-    const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
-    const float wd[3] = {dirs[3 * ray], dirs[3 * ray + 1], dirs[3 * ray + 2]};
+    const int lane = threadIdx.x & 63;
+    bool active = ray < n;
+    float wd[3] = {0.f, 0.f, 1.f};
     NnRay r;
-    if (!nn_setup(g, wo, wd, true, r)) return;
-    float gq[3];
-    const float* cc = color_cache + 3 * ray;
+    float gq[3] = {0.f, 0.f, 0.f}, accum = 0.f;
+    if (active) {
+        const float wo[3] = {origins[3 * ray], origins[3 * ray + 1], origins[3 * ray + 2]};
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-        gq[k] = rgb_residual ? (cc[k] - grad_in[3 * ray + k]) * norm : grad_in[3 * ray + k];
+        for (int a = 0; a < 3; ++a) wd[a] = dirs[3 * ray + a];
+        active = nn_setup(g, wo, wd, true, r);
+    }
+    if (active) {
+        const float* cc = color_cache + 3 * ray;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            gq[k] = rgb_residual ? (cc[k] - grad_in[3 * ray + k]) * norm : grad_in[3 * ray + k];
+        accum = cc[0] * gq[0] + cc[1] * gq[1] + cc[2] * gq[2];
+    }
     const RayCtx<B> rc(wd);
-    float accum = cc[0] * gq[0] + cc[1] * gq[1] + cc[2] * gq[2];
     float log_t = 0.f;
     int guard = 0;
-    while (r.t < r.tmax && ++guard < kMaxIters) {
-        float dt;
-        const int link = nn_step(g, r, dt);
-        if (link < 0) continue;
-        float sigma = ld(g.density, static_cast<int64_t>(link) * g.dstride);
-        if (o.last_opaque && r.t + o.step > r.tmax) sigma += 1e9f;
-        if (!(sigma > o.sigma_thresh)) continue;
-        const float dtw = dt * r.ds;
-        const float lt = -dtw * sigma;
-        const float w = expf(log_t) * (1.f - expf(lt));
-        float total = 0.f;
+    while (__ballot(active) != 0ull && ++guard < kMaxIters) {
+        int link = -1;
+        bool counted = false;
+        float gw[3] = {0.f, 0.f, 0.f}, gdens = 0.f;
+        if (active && !(r.t < r.tmax)) active = false;
+        if (active) {
+            float dt;
+            link = nn_step(g, r, dt);
+            if (link >= 0) {
+                float sigma = ld(g.density, static_cast<int64_t>(link) * g.dstride);
+                if (o.last_opaque && r.t + o.step > r.tmax) sigma += 1e9f;
+                if (sigma > o.sigma_thresh) {
+                    counted = true;
+                    const float dtw = dt * r.ds;
+                    const float lt = -dtw * sigma;
+                    const float w = expf(log_t) * (1.f - expf(lt));
+                    float total = 0.f;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float c = nn_colour<DT, B>(g, link, k, rc.basis);
-            if (c > 0.f) {
-                total += c * gq[k];
-                const float gw = w * gq[k];
-                if (go.gs != nullptr) {
-                    float* row = go.gs + static_cast<int64_t>(link) * go.gss + k * B;
-#pragma unroll
-                    for (int i = 0; i < B; ++i) atomic_add_f32(row + i, rc.basis[i] * gw);
+                    for (int k = 0; k < 3; ++k) {
+                        const float c = nn_colour<DT, B>(g, link, k, rc.basis);
+                        if (c > 0.f) {
+                            total += c * gq[k];
+                            gw[k] = w * gq[k];
+                        }
+                    }
+                    log_t += lt;
+                    accum -= w * total;
+                    gdens = dtw * (total * expf(log_t) - accum);
+                    if (expf(log_t) <= o.stop) active = false;
                 }
             }
         }
-        log_t += lt;
-        accum -= w * total;
-        if (go.mask != nullptr) go.mask[link] = 1;
-        if (go.gd != nullptr)
-            atomic_add_f32(go.gd + static_cast<int64_t>(link) * go.gds,
-                           dtw * (total * expf(log_t) - accum));
-        if (expf(log_t) <= o.stop) return;
+        unsigned long long todo = __ballot(counted);
+        while (todo != 0ull) {
+            const int j = __ffsll(todo) - 1;
+            todo &= todo - 1ull;
+            const int lj = __builtin_amdgcn_readlane(link, j);
+            const float g0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gw[0]), j));
+            const float g1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gw[1]), j));
+            const float g2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gw[2]), j));
+            const float gdj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gdens), j));
+            float bl = 0.f;
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                const float bi =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rc.basis[i]), j));
+                if (lane % B == i) bl = bi;
+            }
+            if (lane < C) {
+                const int k = lane / B;
+                const float gk = k == 0 ? g0 : (k == 1 ? g1 : g2);
+                if (gk != 0.f && go.gs != nullptr)
+                    atomic_add_f32(go.gs + static_cast<int64_t>(lj) * go.gss + lane, bl * gk);
+            } else if (lane == C) {
+                if (go.gd != nullptr) atomic_add_f32(go.gd + static_cast<int64_t>(lj) * go.gds, gdj);
+                if (go.mask != nullptr) go.mask[lj] = 1;
+            }
+        }
     }
 }
 
